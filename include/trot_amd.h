@@ -1,0 +1,142 @@
+/*
+ * trot_amd: MI355X (gfx950) kernels for the phaseless AFQMC walker step and
+ * local-energy measurement of the `trot` package.
+ *
+ * C ABI.  Conventions:
+ *   - all array arguments are device pointers; complex arrays are interleaved
+ *     (re, im) doubles, i.e. the layout of complex128 / torch.complex128;
+ *   - walkers are (W, N, n) complex, row-major (orbital row, occupied column);
+ *   - trial orbitals are (N, n) real (the reference's RHF/UHF mo_coeff from
+ *     a real SCF);
+ *   - "green_t" is the transposed half-rotated Green's function
+ *     GT = w (C^T w)^-1, (W, N, n) complex: G_half[p, j] = GT[j, p];
+ *   - every function returns TROT_OK or a negative TROT_ERR_* code, never
+ *     throws, and records a message retrievable with trot_last_error();
+ *   - the caller owns all memory; scratch comes in as (ws, ws_bytes), sized
+ *     by the matching *_workspace() query.  No entry point allocates;
+ *   - `stream` is a hipStream_t (NULL: default stream).  Work is enqueued
+ *     asynchronously on it.  Calls on one stream are ordered.
+ *
+ * Each entry point names the reference function (file:line, in the trot
+ * repository) whose per-walker arithmetic it performs for all W walkers.
+ */
+#ifndef TROT_AMD_H
+#define TROT_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TROT_OK 0
+#define TROT_ERR_INVALID (-1)     /* bad pointer or size */
+#define TROT_ERR_UNSUPPORTED (-2) /* valid input outside what the kernels cover (e.g. nocc > 64) */
+#define TROT_ERR_HIP (-3)         /* HIP runtime error (launch or configuration) */
+#define TROT_ERR_WORKSPACE (-4)   /* workspace smaller than the *_workspace() query */
+
+const char* trot_version(void);
+const char* trot_last_error(void);
+
+/* Overlap matrix m = C^T w, det(m) and GT = w m^-1 for every walker.
+ * Replaces: overlap_r / _det (trot/trial/rhf.py:35-47) for the determinant and
+ * _half_green_from_overlap_matrix (trot/meas/rhf.py:21-25, called at
+ * meas/rhf.py:121-122) for the Green's function.  green_t may be NULL. */
+size_t trot_overlap_green_workspace(int W, int N, int n);
+int trot_overlap_green(const double* trial, const double* walkers, int W, int N, int n, double* det,
+                       double* green_t, void* ws, size_t ws_bytes, void* stream);
+
+/* overlap[w] = det_a[w] * det_b[w]; det_b NULL gives det_a^2.
+ * Replaces: overlap_r (trot/trial/rhf.py:45-47), overlap_u (rhf.py:50-54,
+ * trot/trial/uhf.py:54-58). */
+int trot_overlap_combine(const double* det_a, const double* det_b, double* overlap, int W, void* stream);
+
+/* fb (+)= alpha * GT_flat . R'^T with R' (G, N*n) real, R'[g, j*n+p] = rot_chol[g, p, j].
+ * Replaces: force_bias_kernel_rw_rh (trot/meas/rhf.py:119-125, alpha = 2),
+ * force_bias_kernel_uw_rh (rhf.py:141-153; trot/meas/uhf.py:43-57: two calls, accumulate). */
+int trot_force_bias(const double* green_t, const double* rot_chol_t, int W, int G, long Nn, double alpha,
+                    int accumulate, double* fb, void* stream);
+
+/* fs = -sqrt(dt)(i fb - mf); xbar = x - fs; shift_term = sum_g xbar mf; fb_term = sum_g (x fs - fs^2/2).
+ * Replaces: trot/prop/afqmc.py:99-103.  fields (W, G) real; fb, xbar (W, G) complex; mf (G) complex. */
+int trot_field_shift(const double* fields, const double* fb, const double* mf, double sqrt_dt, int W, int G,
+                     double* xbar, double* shift_term, double* fb_term, void* stream);
+
+/* V (W, N*N) complex = xbar (W, G) complex . chol_flat (G, N*N) real.
+ * Replaces: _make_vhs_split_flat (trot/prop/chol_afqmc_ops.py:83-87). */
+int trot_vhs_build(const double* xbar, const double* chol_flat, int W, int G, long N2, double* vhs, void* stream);
+
+/* w_out[b] = expm (N, N real) . w_in[b]; not in place.
+ * Replaces: _apply_one_body_half_array (trot/prop/chol_afqmc_ops.py:130-131). */
+int trot_one_body_apply(const double* expm, const double* w_in, int W, int N, int n, double* w_out, void* stream);
+
+/* w <- sum_{k=0..K} (i sqrt(dt) V)^k / k! w via term <- (i sqrt(dt)/k) V term; out += term.
+ * Replaces: taylor_expm_action (trot/prop/utils.py:20-29) as used in
+ * _apply_two_body_array (trot/prop/chol_afqmc_ops.py:152-162). */
+size_t trot_taylor_workspace(int W, int N, int n);
+int trot_taylor_apply(const double* vhs, double* w, int W, int N, int n, double sqrt_dt, int n_terms, void* ws,
+                      size_t ws_bytes, void* stream);
+
+/* In place: w_s <- E . taylor(i sqrt(dt), V(xbar), E . w_s, K) for one or two spin sets sharing V,
+ * in walker chunks of `chunk` (<= 0: all).  walkers_b NULL for restricted walkers.
+ * Replaces: _apply_trotter_r / _apply_trotter_u (trot/prop/chol_afqmc_ops.py:198-222). */
+size_t trot_apply_trotter_workspace(int W, int N, int n_max, int chunk);
+int trot_apply_trotter(double* walkers_a, int na, double* walkers_b, int nb, const double* xbar,
+                       const double* chol_flat, const double* expm, int W, int N, int G, double sqrt_dt,
+                       int n_terms, int chunk, void* ws, size_t ws_bytes, void* stream);
+
+/* Phaseless importance, floor/cap masking, node count and population-control shift.
+ * weights in/out (W real); scalars[0] = pop_control_ene_shift (in/out), scalars[1] = e_estimate;
+ * nodes[0] += node encounters.  Replaces: trot/prop/afqmc.py:108-134. */
+int trot_phaseless_update(double* weights, const double* ov_old, const double* ov_new, const double* shift_term,
+                          const double* fb_term, double* scalars, long long* nodes, int W, double dt, double h0p_re,
+                          double h0p_im, double w_floor, double w_cap, double damping, void* stream);
+
+/* E_L per walker (complex) from cached GT.  rot_chol_s (G, n_s, N) real, rot_h1t_s (N, n_s) real.
+ * Second spin NULL: RHF trial with restricted walkers (factor 2 for both spins).
+ * Replaces: energy_kernel_rw_rh / _two_body_energy_restricted (trot/meas/rhf.py:41-60, 128-138),
+ * energy_kernel_uw_rh (rhf.py:156-172, 63-98) and trot/meas/uhf.py:205-231. */
+size_t trot_local_energy_workspace(int W, int G);
+int trot_local_energy(const double* rot_chol_a, const double* rot_chol_b, const double* green_t_a,
+                      const double* green_t_b, const double* rot_h1t_a, const double* rot_h1t_b, int W, int N,
+                      int na, int nb, int G, double h0, double* energy, void* ws, size_t ws_bytes, void* stream);
+
+/* Block energy tail: Re E_L clamped to e_estimate when non-finite or |E - e_est| > sqrt(2/dt); non-finite
+ * walkers get weight 0 (weights in/out); out[0] = weighted mean (e_est if sum w = 0), out[1] = sum w;
+ * scalars[1] (e_estimate) <- (1 - shift_ema) e_est + shift_ema out[0].
+ * Replaces: trot/prop/blocks.py:236-257. */
+int trot_block_energy(const double* energy, double* weights, double* scalars, double* out, int W, double dt,
+                      double shift_ema, void* stream);
+
+/* In place Q factor of w with real positive diag(R) (two Cholesky-QR passes).
+ * Replaces: _qr / orthonormalize (trot/walkers.py:137-179). */
+size_t trot_orthonormalize_workspace(int W, int N, int n);
+int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t ws_bytes, void* stream);
+
+/* Comb reconfiguration: idx[i] = searchsorted(cumsum|w|, tot (i + zeta)/W, 'left'), weights_new = tot/W.
+ * Replaces: _sr_indices / stochastic_reconfiguration (trot/walkers.py:249-293). */
+size_t trot_sr_workspace(int W);
+int trot_sr_comb(const double* weights, double zeta, int W, int* idx, double* weights_new, void* ws,
+                 size_t ws_bytes, void* stream);
+
+/* dst[i, :] = src[idx[i], :] for rows of row_doubles doubles (walkers, Green's functions, overlaps).
+ * Replaces: the gather w[idx] of trot/walkers.py:272-278. */
+int trot_gather_rows(const double* src, double* dst, const int* idx, int W, long row_doubles, void* stream);
+
+/* rot_chol[g] = C^T L_g (G, n, N) and rot_chol_t[g] = L_g^T C (G, N, n); either output may be NULL.
+ * Replaces: build_meas_ctx (trot/meas/rhf.py:308-319; trot/meas/uhf.py:318-336). */
+int trot_build_rot_chol(const double* chol, const double* trial, int G, int N, int n, double* rot_chol,
+                        double* rot_chol_t, void* stream);
+
+/* mf[g] = i tr(L_g dm) (complex), mfl = sum_g mf_g L_g (N x N complex), v0m = 1/2 sum_g L_g L_g (real).
+ * Replaces: _mf_shifts and the contractions of _get_h1_eff
+ * (trot/prop/chol_afqmc_ops.py:74-76, 90-100). */
+size_t trot_build_prop_workspace(int G, int N);
+int trot_build_prop_tensors(const double* chol, const double* dm, int G, int N, double* mf, double* mfl,
+                            double* v0m, void* ws, size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TROT_AMD_H */

@@ -1,0 +1,346 @@
+"""ctypes binding of libtrot_amd.so (include/trot_amd.h) with host-side shape checks.
+
+The library is the only compute path: if it is missing, importing the
+compute entry points raises.  There is no CPU or PyTorch fallback.  Every
+wrapper checks device, dtype, contiguity and shapes of its tensors against
+what the kernel and its launch grid assume before calling into HIP.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libtrot_amd.so"
+
+TROT_OK = 0
+MAX_NOCC = 64
+
+
+class TrotError(RuntimeError):
+    """A trot_amd C entry point returned a non-zero status."""
+
+
+_lib = None
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_l = ctypes.c_long
+_d = ctypes.c_double
+_sz = ctypes.c_size_t
+
+_SIGS = {
+    "trot_version": (ctypes.c_char_p, []),
+    "trot_last_error": (ctypes.c_char_p, []),
+    "trot_overlap_green_workspace": (_sz, [_i, _i, _i]),
+    "trot_overlap_green": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _sz, _vp]),
+    "trot_overlap_combine": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "trot_force_bias": (_i, [_vp, _vp, _i, _i, _l, _d, _i, _vp, _vp]),
+    "trot_field_shift": (_i, [_vp, _vp, _vp, _d, _i, _i, _vp, _vp, _vp, _vp]),
+    "trot_vhs_build": (_i, [_vp, _vp, _i, _i, _l, _vp, _vp]),
+    "trot_one_body_apply": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
+    "trot_taylor_workspace": (_sz, [_i, _i, _i]),
+    "trot_taylor_apply": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
+    "trot_apply_trotter_workspace": (_sz, [_i, _i, _i, _i]),
+    "trot_apply_trotter": (_i, [_vp, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _i, _d, _i, _i, _vp, _sz, _vp]),
+    "trot_phaseless_update": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _d, _d, _vp]),
+    "trot_local_energy_workspace": (_sz, [_i, _i]),
+    "trot_local_energy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _d, _vp, _vp, _sz, _vp]),
+    "trot_block_energy": (_i, [_vp, _vp, _vp, _vp, _i, _d, _d, _vp]),
+    "trot_orthonormalize_workspace": (_sz, [_i, _i, _i]),
+    "trot_orthonormalize": (_i, [_vp, _i, _i, _i, _vp, _sz, _vp]),
+    "trot_sr_workspace": (_sz, [_i]),
+    "trot_sr_comb": (_i, [_vp, _d, _i, _vp, _vp, _vp, _sz, _vp]),
+    "trot_gather_rows": (_i, [_vp, _vp, _vp, _i, _l, _vp]),
+    "trot_build_rot_chol": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
+    "trot_build_prop_workspace": (_sz, [_i, _i]),
+    "trot_build_prop_tensors": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
+}
+
+
+def lib():
+    """Load libtrot_amd.so (once).  Raises ImportError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m trot_amd.build` "
+                "(trot_amd has no fallback compute path)"
+            )
+        h = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def _check(rc: int, name: str) -> None:
+    if rc != TROT_OK:
+        msg = lib().trot_last_error().decode(errors="replace")
+        raise TrotError(f"{name} failed with status {rc}: {msg}")
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _req(t: torch.Tensor, name: str, dtype: torch.dtype, shape: tuple) -> None:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch.Tensor")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name}: must be a GPU tensor, got {t.device}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name}: expected {dtype}, got {t.dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: expected shape {tuple(shape)}, got {tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+
+
+def _ws(nbytes: int, device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+
+
+C128 = torch.complex128
+F64 = torch.float64
+
+
+# ---------------------------------------------------------------------------
+# wrappers (one per C entry point)
+# ---------------------------------------------------------------------------
+
+
+def overlap_green(trial: torch.Tensor, walkers: torch.Tensor, want_green: bool = True):
+    """(det (W,) c128, green_t (W,N,n) c128 or None) for one spin."""
+    W, N, n = walkers.shape
+    _req(walkers, "walkers", C128, (W, N, n))
+    _req(trial, "trial", F64, (N, n))
+    if n > MAX_NOCC:
+        raise ValueError(f"nocc={n} > {MAX_NOCC} is not supported by the GPU kernels")
+    L = lib()
+    det = torch.empty((W,), dtype=C128, device=walkers.device)
+    gt = torch.empty((W, N, n), dtype=C128, device=walkers.device) if want_green else None
+    ws = _ws(L.trot_overlap_green_workspace(W, N, n), walkers.device)
+    _check(L.trot_overlap_green(_p(trial), _p(walkers), W, N, n, _p(det), _p(gt), _p(ws), ws.numel(), _stream()),
+           "trot_overlap_green")
+    return det, gt
+
+
+def overlap_combine(det_a: torch.Tensor, det_b: torch.Tensor | None) -> torch.Tensor:
+    W = det_a.shape[0]
+    _req(det_a, "det_a", C128, (W,))
+    if det_b is not None:
+        _req(det_b, "det_b", C128, (W,))
+    out = torch.empty((W,), dtype=C128, device=det_a.device)
+    _check(lib().trot_overlap_combine(_p(det_a), _p(det_b), _p(out), W, _stream()), "trot_overlap_combine")
+    return out
+
+
+def force_bias(green_t: torch.Tensor, rot_chol_t: torch.Tensor, alpha: float, out: torch.Tensor | None = None,
+               accumulate: bool = False) -> torch.Tensor:
+    W, N, n = green_t.shape
+    G = rot_chol_t.shape[0]
+    _req(green_t, "green_t", C128, (W, N, n))
+    _req(rot_chol_t, "rot_chol_t", F64, (G, N, n))
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs an output tensor")
+        out = torch.empty((W, G), dtype=C128, device=green_t.device)
+    _req(out, "fb", C128, (W, G))
+    _check(lib().trot_force_bias(_p(green_t), _p(rot_chol_t), W, G, N * n, float(alpha), int(accumulate), _p(out),
+                                 _stream()), "trot_force_bias")
+    return out
+
+
+def field_shift(fields: torch.Tensor, fb: torch.Tensor, mf: torch.Tensor, sqrt_dt: float):
+    W, G = fields.shape
+    _req(fields, "fields", F64, (W, G))
+    _req(fb, "force_bias", C128, (W, G))
+    _req(mf, "mf_shifts", C128, (G,))
+    dev = fields.device
+    xbar = torch.empty((W, G), dtype=C128, device=dev)
+    shift = torch.empty((W,), dtype=C128, device=dev)
+    fbt = torch.empty((W,), dtype=C128, device=dev)
+    _check(lib().trot_field_shift(_p(fields), _p(fb), _p(mf), float(sqrt_dt), W, G, _p(xbar), _p(shift), _p(fbt),
+                                  _stream()), "trot_field_shift")
+    return xbar, shift, fbt
+
+
+def vhs_build(xbar: torch.Tensor, chol_flat: torch.Tensor) -> torch.Tensor:
+    W, G = xbar.shape
+    N2 = chol_flat.shape[1]
+    _req(xbar, "xbar", C128, (W, G))
+    _req(chol_flat, "chol_flat", F64, (G, N2))
+    out = torch.empty((W, N2), dtype=C128, device=xbar.device)
+    _check(lib().trot_vhs_build(_p(xbar), _p(chol_flat), W, G, N2, _p(out), _stream()), "trot_vhs_build")
+    return out
+
+
+def one_body_apply(expm: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    _req(expm, "exp_h1_half", F64, (N, N))
+    out = torch.empty_like(w)
+    _check(lib().trot_one_body_apply(_p(expm), _p(w), W, N, n, _p(out), _stream()), "trot_one_body_apply")
+    return out
+
+
+def taylor_apply(vhs: torch.Tensor, w: torch.Tensor, sqrt_dt: float, n_terms: int) -> torch.Tensor:
+    """Returns a new tensor: sum_k (i sqrt(dt) V)^k/k! w."""
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    _req(vhs, "vhs", C128, (W, N, N))
+    out = w.clone()
+    L = lib()
+    ws = _ws(L.trot_taylor_workspace(W, N, n), w.device)
+    _check(L.trot_taylor_apply(_p(vhs), _p(out), W, N, n, float(sqrt_dt), int(n_terms), _p(ws), ws.numel(),
+                               _stream()), "trot_taylor_apply")
+    return out
+
+
+def apply_trotter(w_a: torch.Tensor, w_b: torch.Tensor | None, xbar: torch.Tensor, chol_flat: torch.Tensor,
+                  expm: torch.Tensor, sqrt_dt: float, n_terms: int, chunk: int = 0):
+    """New walkers E T_K(i sqrt(dt) V(xbar)) E w for one or two spin sets sharing V."""
+    W, N, na = w_a.shape
+    G = xbar.shape[1]
+    _req(w_a, "walkers", C128, (W, N, na))
+    nb = 0
+    if w_b is not None:
+        nb = w_b.shape[2]
+        _req(w_b, "walkers_b", C128, (W, N, nb))
+    _req(xbar, "xbar", C128, (W, G))
+    _req(chol_flat, "chol_flat", F64, (G, N * N))
+    _req(expm, "exp_h1_half", F64, (N, N))
+    out_a = w_a.clone()
+    out_b = w_b.clone() if w_b is not None else None
+    L = lib()
+    ws = _ws(L.trot_apply_trotter_workspace(W, N, max(na, nb), int(chunk)), w_a.device)
+    _check(L.trot_apply_trotter(_p(out_a), na, _p(out_b), nb, _p(xbar), _p(chol_flat), _p(expm), W, N, G,
+                                float(sqrt_dt), int(n_terms), int(chunk), _p(ws), ws.numel(), _stream()),
+           "trot_apply_trotter")
+    return out_a, out_b
+
+
+def phaseless_update(weights, ov_old, ov_new, shift, fbt, scalars, nodes, dt, h0_prop, w_floor, w_cap, damping):
+    W = weights.shape[0]
+    _req(weights, "weights", F64, (W,))
+    for t, nm in ((ov_old, "ov_old"), (ov_new, "ov_new"), (shift, "shift_term"), (fbt, "fb_term")):
+        _req(t, nm, C128, (W,))
+    _req(scalars, "scalars", F64, (2,))
+    _req(nodes, "nodes", torch.int64, (1,))
+    h0p = complex(h0_prop)
+    _check(lib().trot_phaseless_update(_p(weights), _p(ov_old), _p(ov_new), _p(shift), _p(fbt), _p(scalars),
+                                       _p(nodes), W, float(dt), h0p.real, h0p.imag, float(w_floor), float(w_cap),
+                                       float(damping), _stream()), "trot_phaseless_update")
+
+
+def local_energy(rot_chol_a, green_t_a, rot_h1t_a, h0: float, rot_chol_b=None, green_t_b=None, rot_h1t_b=None):
+    W, N, na = green_t_a.shape
+    G = rot_chol_a.shape[0]
+    _req(green_t_a, "green_t_a", C128, (W, N, na))
+    _req(rot_chol_a, "rot_chol_a", F64, (G, na, N))
+    _req(rot_h1t_a, "rot_h1t_a", F64, (N, na))
+    nb = 0
+    if green_t_b is not None:
+        nb = green_t_b.shape[2]
+        _req(green_t_b, "green_t_b", C128, (W, N, nb))
+        _req(rot_chol_b, "rot_chol_b", F64, (G, nb, N))
+        _req(rot_h1t_b, "rot_h1t_b", F64, (N, nb))
+    if max(na, nb) > MAX_NOCC:
+        raise ValueError(f"nocc > {MAX_NOCC} is not supported by the GPU energy kernel")
+    L = lib()
+    out = torch.empty((W,), dtype=C128, device=green_t_a.device)
+    ws = _ws(L.trot_local_energy_workspace(W, G), green_t_a.device)
+    _check(L.trot_local_energy(_p(rot_chol_a), _p(rot_chol_b), _p(green_t_a), _p(green_t_b), _p(rot_h1t_a),
+                               _p(rot_h1t_b), W, N, na, nb, G, float(h0), _p(out), _p(ws), ws.numel(), _stream()),
+           "trot_local_energy")
+    return out
+
+
+def block_energy(energy: torch.Tensor, weights: torch.Tensor, scalars: torch.Tensor, dt: float, shift_ema: float):
+    """In place on weights / scalars; returns a (2,) tensor (e_block, sum w)."""
+    W = energy.shape[0]
+    _req(energy, "energy", C128, (W,))
+    _req(weights, "weights", F64, (W,))
+    _req(scalars, "scalars", F64, (2,))
+    out = torch.empty((2,), dtype=F64, device=energy.device)
+    _check(lib().trot_block_energy(_p(energy), _p(weights), _p(scalars), _p(out), W, float(dt), float(shift_ema),
+                                   _stream()), "trot_block_energy")
+    return out
+
+
+def orthonormalize(w: torch.Tensor) -> torch.Tensor:
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    if n > MAX_NOCC:
+        raise ValueError(f"nocc={n} > {MAX_NOCC} is not supported")
+    out = w.clone()
+    L = lib()
+    ws = _ws(L.trot_orthonormalize_workspace(W, N, n), w.device)
+    _check(L.trot_orthonormalize(_p(out), W, N, n, _p(ws), ws.numel(), _stream()), "trot_orthonormalize")
+    return out
+
+
+def sr_comb(weights: torch.Tensor, zeta: float):
+    W = weights.shape[0]
+    _req(weights, "weights", F64, (W,))
+    L = lib()
+    idx = torch.empty((W,), dtype=torch.int32, device=weights.device)
+    w_new = torch.empty_like(weights)
+    ws = _ws(L.trot_sr_workspace(W), weights.device)
+    _check(L.trot_sr_comb(_p(weights), float(zeta), W, _p(idx), _p(w_new), _p(ws), ws.numel(), _stream()),
+           "trot_sr_comb")
+    return idx, w_new
+
+
+def gather_rows(src: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+    W = src.shape[0]
+    if not src.is_contiguous() or src.device.type != "cuda":
+        raise ValueError("gather_rows: src must be a contiguous GPU tensor")
+    _req(idx, "idx", torch.int32, (W,))
+    row_bytes = src[0].numel() * src.element_size()
+    if row_bytes % 8:
+        raise ValueError("gather_rows: rows must be a whole number of doubles")
+    dst = torch.empty_like(src)
+    _check(lib().trot_gather_rows(_p(src), _p(dst), _p(idx), W, row_bytes // 8, _stream()), "trot_gather_rows")
+    return dst
+
+
+def build_rot_chol(chol: torch.Tensor, trial: torch.Tensor, want_rot: bool = True, want_rot_t: bool = True):
+    G, N, _ = chol.shape
+    n = trial.shape[1]
+    _req(chol, "chol", F64, (G, N, N))
+    _req(trial, "trial", F64, (N, n))
+    if G > 65535:
+        raise ValueError("nchol > 65535 is not supported")
+    rc = torch.empty((G, n, N), dtype=F64, device=chol.device) if want_rot else None
+    rct = torch.empty((G, N, n), dtype=F64, device=chol.device) if want_rot_t else None
+    _check(lib().trot_build_rot_chol(_p(chol), _p(trial), G, N, n, _p(rc), _p(rct), _stream()), "trot_build_rot_chol")
+    return rc, rct
+
+
+def build_prop_tensors(chol: torch.Tensor, dm: torch.Tensor):
+    G, N, _ = chol.shape
+    _req(chol, "chol", F64, (G, N, N))
+    _req(dm, "dm", F64, (N, N))
+    dev = chol.device
+    L = lib()
+    mf = torch.empty((G,), dtype=C128, device=dev)
+    mfl = torch.empty((N, N), dtype=C128, device=dev)
+    v0m = torch.empty((N, N), dtype=F64, device=dev)
+    ws = _ws(L.trot_build_prop_workspace(G, N), dev)
+    _check(L.trot_build_prop_tensors(_p(chol), _p(dm), G, N, _p(mf), _p(mfl), _p(v0m), _p(ws), ws.numel(), _stream()),
+           "trot_build_prop_tensors")
+    return mf, mfl, v0m

@@ -1,0 +1,67 @@
+"""One propagation + measurement block (trot/prop/blocks.py:195-287) on the GPU."""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Callable
+
+import torch
+
+from . import _lib
+from . import walkers as wk
+from .core import MeasOps, System, TrialOps
+from .meas import _h0
+from .prop import PropState
+
+
+@dataclass
+class BlockObs:
+    """trot/prop/blocks.py (BlockObs): scalars {"energy", "weight"} as 0-dim GPU tensors."""
+
+    scalars: dict
+    observables: dict = field(default_factory=dict)
+
+
+def block(state: PropState, *, sys: System, params, ham_data, trial_data, trial_ops: TrialOps, meas_ops: MeasOps,
+          meas_ctx, prop_ops, prop_ctx, sr_fn: Callable = wk.stochastic_reconfiguration,
+          observable_names: tuple = (), fields_seq=None) -> tuple[PropState, BlockObs]:
+    """Steps, QR, overlap recompute, energy, clamp / weighted mean / EMA, SR, overlap recompute.
+
+    ``fields_seq`` (optional) is a sequence of n_prop_steps (W, G) field tensors
+    that replaces the provider, for step-by-step comparisons.
+    """
+    for name in observable_names:
+        raise KeyError(f"missing requested observable '{name}'. available: []")
+    for s in range(params.n_prop_steps):  # blocks.py:213-228
+        f = None if fields_seq is None else fields_seq[s]
+        state = prop_ops.step(state, params=params, ham_data=ham_data, trial_data=trial_data, trial_ops=trial_ops,
+                              meas_ops=meas_ops, meas_ctx=meas_ctx, prop_ctx=prop_ctx, fields=f)
+    engine = meas_ops.engine
+    walkers_new = wk.orthonormalize(state.walkers, sys.walker_kind)  # blocks.py:230
+    overlaps_new, greens = engine.overlap_green(walkers_new, trial_data)  # blocks.py:231-234
+    e_samples = engine.energy_from_green(greens, _h0(ham_data), meas_ctx)  # blocks.py:236-240
+    weights = state.weights.clone()
+    scal = torch.stack([state.pop_control_ene_shift, state.e_estimate]).to(torch.float64).contiguous()
+    out = _lib.block_energy(e_samples, weights, scal, params.dt, params.shift_ema)  # blocks.py:240-257
+    zeta, key = prop_ctx.fields.uniform(state.rng_key)  # blocks.py:270-271
+    if sr_fn is wk.stochastic_reconfiguration:
+        idx, weights_sr = wk.sr_indices(weights, zeta)
+        w_sr = wk.gather(walkers_new, idx)
+        overlaps_sr = wk.gather(torch.view_as_real(overlaps_new).contiguous(), idx)
+        overlaps_sr = torch.view_as_complex(overlaps_sr)
+        greens_sr = wk.gather(greens, idx)
+    else:
+        w_sr, weights_sr = sr_fn(walkers_new, weights, zeta, sys.walker_kind)
+        w_sr = wk.as_device_walkers(w_sr)
+        overlaps_sr, greens_sr = engine.overlap_green(w_sr, trial_data)  # blocks.py:273-276
+    state = PropState(
+        walkers=w_sr,
+        weights=weights_sr,
+        overlaps=overlaps_sr,
+        rng_key=key,
+        pop_control_ene_shift=scal[0],
+        e_estimate=scal[1],
+        node_encounters=state.node_encounters,
+        green=greens_sr,
+    )
+    return state, BlockObs(scalars={"energy": out[0], "weight": out[1]}, observables={})

@@ -1,0 +1,419 @@
+// C-ABI of trot_amd: extern "C" entry points over plain device pointers.
+//
+// Conventions (also in include/trot_amd.h):
+//  * complex arrays are interleaved (re, im) doubles (complex128 layout);
+//  * every entry point returns TROT_OK (0) or a negative status and never
+//    throws; trot_last_error() gives a message for the calling thread;
+//  * the caller owns every buffer; scratch memory comes in through a
+//    (ws, ws_bytes) pair sized by the matching *_workspace() query.  No entry
+//    point allocates device memory;
+//  * `stream` is a hipStream_t (NULL = default stream); all work is enqueued
+//    on it asynchronously.
+#include <stdio.h>
+#include <string.h>
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "kernels.hpp"
+#include "../../include/trot_amd.h"
+
+static thread_local char g_err[512];
+
+static int fail(int code, const char* fmt, const char* a = "", long b = 0) {
+  snprintf(g_err, sizeof(g_err), fmt, a, b);
+  return code;
+}
+static int hip_status(hipError_t e, const char* where) {
+  if (e == hipSuccess) return TROT_OK;
+  snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
+  return TROT_ERR_HIP;
+}
+#define HIPCHK(expr)                                    \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return hip_status(_e, #expr); \
+  } while (0)
+
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// bump allocator over the caller's workspace
+struct Carve {
+  char* base;
+  size_t cap, used;
+  template <typename T>
+  T* take(size_t count) {
+    size_t bytes = align256(count * sizeof(T));
+    if (used + bytes > cap) return nullptr;
+    T* p = reinterpret_cast<T*>(base + used);
+    used += bytes;
+    return p;
+  }
+};
+
+extern "C" {
+
+const char* trot_version(void) { return "trot_amd 0.1 (gfx950, FP64 MFMA)"; }
+const char* trot_last_error(void) { return g_err; }
+
+// ---------------------------------------------------------------------------
+// overlap + half Green's function
+// ---------------------------------------------------------------------------
+size_t trot_overlap_green_workspace(int W, int N, int n) {
+  (void)N;
+  return 2 * align256((size_t)W * n * n * 16);
+}
+
+int trot_overlap_green(const double* trial, const double* walkers, int W, int N, int n, double* det,
+                       double* green_t, void* ws, size_t ws_bytes, void* stream) {
+  if (!trial || !walkers || !det) return fail(TROT_ERR_INVALID, "overlap_green: null pointer%s%ld");
+  if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "overlap_green: bad sizes%s%ld");
+  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "overlap_green: nocc > 64 unsupported%s%ld", "", n);
+  Carve c{(char*)ws, ws_bytes, 0};
+  double* m = c.take<double>((size_t)W * n * n * 2);
+  double* minv = c.take<double>((size_t)W * n * n * 2);
+  if (!m || !minv) return fail(TROT_ERR_WORKSPACE, "overlap_green: workspace too small%s%ld");
+  hipStream_t s = (hipStream_t)stream;
+  // m[b] = C^T w[b]  (n x n)
+  GemmP p{};
+  p.A = trial; p.lda = n; p.sA = 0;
+  p.B = walkers; p.ldb = n; p.sB = (long)N * n;
+  p.C = m; p.ldc = n; p.sC = (long)n * n;
+  p.M = n; p.N = n; p.K = N; p.ar = 1.0;
+  HIPCHK((launch_gemm_f64<64, 64, false, true, false, true, false, EPI_STORE>(p, W, s)));
+  HIPCHK(launch_inv_det(m, minv, det, n, W, s));
+  if (green_t) {
+    // GT[b] = w[b] m[b]^-1  (N x n)
+    GemmP q{};
+    q.A = walkers; q.lda = n; q.sA = (long)N * n;
+    q.B = minv; q.ldb = n; q.sB = (long)n * n;
+    q.C = green_t; q.ldc = n; q.sC = (long)N * n;
+    q.M = N; q.N = n; q.K = n; q.ar = 1.0;
+    HIPCHK((launch_gemm_f64<64, 64, true, true, true, true, false, EPI_STORE>(q, W, s)));
+  }
+  return TROT_OK;
+}
+
+int trot_overlap_combine(const double* det_a, const double* det_b, double* overlap, int W, void* stream) {
+  if (!det_a || !overlap || W < 0) return fail(TROT_ERR_INVALID, "overlap_combine: bad arguments%s%ld");
+  HIPCHK(launch_overlap_combine(det_a, det_b, overlap, W, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// force bias: fb (+)= alpha * GT_flat (W x Nn) . R'^T, R' = (G, Nn) real
+// ---------------------------------------------------------------------------
+int trot_force_bias(const double* green_t, const double* rot_chol_t, int W, int G, long Nn, double alpha,
+                    int accumulate, double* fb, void* stream) {
+  if (!green_t || !rot_chol_t || !fb) return fail(TROT_ERR_INVALID, "force_bias: null pointer%s%ld");
+  if (W < 1 || G < 1 || Nn < 1) return fail(TROT_ERR_INVALID, "force_bias: bad sizes%s%ld");
+  GemmP p{};
+  p.A = green_t; p.lda = Nn; p.sA = 0;
+  p.B = rot_chol_t; p.ldb = Nn; p.sB = 0;
+  p.C = fb; p.ldc = G; p.sC = 0;
+  p.M = W; p.N = G; p.K = (int)Nn; p.ar = alpha; p.beta = accumulate ? 1.0 : 0.0;
+  HIPCHK((launch_gemm_f64<64, 64, true, false, true, false, false, EPI_STORE>(p, 1, (hipStream_t)stream)));
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// shifted fields
+// ---------------------------------------------------------------------------
+int trot_field_shift(const double* fields, const double* fb, const double* mf, double sqrt_dt, int W, int G,
+                     double* xbar, double* shift_term, double* fb_term, void* stream) {
+  if (!fields || !fb || !mf || !xbar || !shift_term || !fb_term)
+    return fail(TROT_ERR_INVALID, "field_shift: null pointer%s%ld");
+  if (W < 1 || G < 1) return fail(TROT_ERR_INVALID, "field_shift: bad sizes%s%ld");
+  HIPCHK(launch_shift(fields, fb, mf, sqrt_dt, W, G, xbar, nullptr, shift_term, fb_term, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// VHS: V (W x N^2 complex) = xbar (W x G complex) . Lflat (G x N^2 real)
+// ---------------------------------------------------------------------------
+int trot_vhs_build(const double* xbar, const double* chol_flat, int W, int G, long N2, double* vhs, void* stream) {
+  if (!xbar || !chol_flat || !vhs) return fail(TROT_ERR_INVALID, "vhs_build: null pointer%s%ld");
+  if (W < 1 || G < 1 || N2 < 1) return fail(TROT_ERR_INVALID, "vhs_build: bad sizes%s%ld");
+  GemmP p{};
+  p.A = xbar; p.lda = G;
+  p.B = chol_flat; p.ldb = N2;
+  p.C = vhs; p.ldc = N2;
+  p.M = W; p.N = (int)N2; p.K = G; p.ar = 1.0;
+  HIPCHK((launch_gemm_f64<64, 64, true, false, true, true, false, EPI_STORE>(p, 1, (hipStream_t)stream)));
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// one-body half step: w_out[b] = E (N x N real) . w_in[b]
+// ---------------------------------------------------------------------------
+int trot_one_body_apply(const double* expm, const double* w_in, int W, int N, int n, double* w_out, void* stream) {
+  if (!expm || !w_in || !w_out) return fail(TROT_ERR_INVALID, "one_body_apply: null pointer%s%ld");
+  if (w_in == w_out) return fail(TROT_ERR_INVALID, "one_body_apply: in-place not supported%s%ld");
+  if (W < 1 || N < 1 || n < 1) return fail(TROT_ERR_INVALID, "one_body_apply: bad sizes%s%ld");
+  GemmP p{};
+  p.A = expm; p.lda = N; p.sA = 0;
+  p.B = w_in; p.ldb = n; p.sB = (long)N * n;
+  p.C = w_out; p.ldc = n; p.sC = (long)N * n;
+  p.M = N; p.N = n; p.K = N; p.ar = 1.0;
+  HIPCHK((launch_gemm_f64<64, 64, false, true, true, true, false, EPI_STORE>(p, W, (hipStream_t)stream)));
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Taylor action: w <- sum_{k=0..K} (i sqrt(dt) V)^k / k! w  (term recursion)
+// ---------------------------------------------------------------------------
+size_t trot_taylor_workspace(int W, int N, int n) { return 2 * align256((size_t)W * N * n * 16); }
+
+int trot_taylor_apply(const double* vhs, double* w, int W, int N, int n, double sqrt_dt, int n_terms, void* ws,
+                      size_t ws_bytes, void* stream) {
+  if (!vhs || !w) return fail(TROT_ERR_INVALID, "taylor_apply: null pointer%s%ld");
+  if (W < 1 || N < 1 || n < 1 || n_terms < 0) return fail(TROT_ERR_INVALID, "taylor_apply: bad sizes%s%ld");
+  Carve c{(char*)ws, ws_bytes, 0};
+  double* t0 = c.take<double>((size_t)W * N * n * 2);
+  double* t1 = c.take<double>((size_t)W * N * n * 2);
+  if (!t0 || !t1) return fail(TROT_ERR_WORKSPACE, "taylor_apply: workspace too small%s%ld");
+  hipStream_t s = (hipStream_t)stream;
+  if (n_terms == 0) return TROT_OK;
+  HIPCHK(hipMemcpyAsync(t0, w, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
+  double* prev = t0;
+  double* next = t1;
+  for (int k = 1; k <= n_terms; ++k) {
+    GemmP p{};
+    p.A = vhs; p.lda = N; p.sA = (long)N * N;
+    p.B = prev; p.ldb = n; p.sB = (long)N * n;
+    p.C = next; p.ldc = n; p.sC = (long)N * n;
+    p.C2 = w; p.ldc2 = n; p.sC2 = (long)N * n;
+    p.M = N; p.N = n; p.K = N; p.ar = 0.0; p.ai = sqrt_dt / k;
+    HIPCHK((launch_gemm_f64<64, 64, true, true, true, true, false, EPI_TAYLOR>(p, W, s)));
+    double* tmp = prev;
+    prev = next;
+    next = tmp;
+  }
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// full Trotter factor for spin sets sharing one V, in walker chunks:
+//   w_s <- E . T_K(i sqrt(dt) V(xbar)) . E . w_s
+// walkers_b may be NULL (restricted walkers).
+// ---------------------------------------------------------------------------
+size_t trot_apply_trotter_workspace(int W, int N, int n_max, int chunk) {
+  if (chunk <= 0 || chunk > W) chunk = W;
+  const size_t v = align256((size_t)chunk * N * N * 16);
+  const size_t wb = align256((size_t)chunk * N * n_max * 16);
+  return v + wb + trot_taylor_workspace(chunk, N, n_max);
+}
+
+int trot_apply_trotter(double* walkers_a, int na, double* walkers_b, int nb, const double* xbar,
+                       const double* chol_flat, const double* expm, int W, int N, int G, double sqrt_dt,
+                       int n_terms, int chunk, void* ws, size_t ws_bytes, void* stream) {
+  if (!walkers_a || !xbar || !chol_flat || !expm) return fail(TROT_ERR_INVALID, "apply_trotter: null pointer%s%ld");
+  if (W < 1 || N < 1 || G < 1 || na < 1 || (walkers_b && nb < 1))
+    return fail(TROT_ERR_INVALID, "apply_trotter: bad sizes%s%ld");
+  if (chunk <= 0 || chunk > W) chunk = W;
+  const int n_max = (walkers_b && nb > na) ? nb : na;
+  if (ws_bytes < trot_apply_trotter_workspace(W, N, n_max, chunk))
+    return fail(TROT_ERR_WORKSPACE, "apply_trotter: workspace too small%s%ld");
+  Carve c{(char*)ws, ws_bytes, 0};
+  double* v = c.take<double>((size_t)chunk * N * N * 2);
+  double* wtmp = c.take<double>((size_t)chunk * N * n_max * 2);
+  void* tws = c.base + c.used;
+  const size_t tws_bytes = c.cap - c.used;
+  hipStream_t s = (hipStream_t)stream;
+  (void)s;
+  for (int w0 = 0; w0 < W; w0 += chunk) {
+    const int wc = (W - w0) < chunk ? (W - w0) : chunk;
+    int rc = trot_vhs_build(xbar + 2 * (size_t)w0 * G, chol_flat, wc, G, (long)N * N, v, stream);
+    if (rc) return rc;
+    for (int sp = 0; sp < 2; ++sp) {
+      double* ws_ = sp == 0 ? walkers_a : walkers_b;
+      const int ns = sp == 0 ? na : nb;
+      if (!ws_) continue;
+      // in place per chunk: the chunk is read completely into wtmp before it is overwritten
+      double* wc_ptr = ws_ + 2 * (size_t)w0 * N * ns;
+      if ((rc = trot_one_body_apply(expm, wc_ptr, wc, N, ns, wtmp, stream))) return rc;
+      if ((rc = trot_taylor_apply(v, wtmp, wc, N, ns, sqrt_dt, n_terms, tws, tws_bytes, stream))) return rc;
+      if ((rc = trot_one_body_apply(expm, wtmp, wc, N, ns, wc_ptr, stream))) return rc;
+    }
+  }
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// phaseless weight update.  scalars[0] = pop_control_ene_shift (in/out),
+// scalars[1] = e_estimate (in); nodes[0] accumulates node encounters.
+// ---------------------------------------------------------------------------
+int trot_phaseless_update(double* weights, const double* ov_old, const double* ov_new, const double* shift_term,
+                          const double* fb_term, double* scalars, long long* nodes, int W, double dt, double h0p_re,
+                          double h0p_im, double w_floor, double w_cap, double damping, void* stream) {
+  if (!weights || !ov_old || !ov_new || !shift_term || !fb_term || !scalars || !nodes)
+    return fail(TROT_ERR_INVALID, "phaseless_update: null pointer%s%ld");
+  if (W < 1 || !(dt > 0)) return fail(TROT_ERR_INVALID, "phaseless_update: bad sizes%s%ld");
+  HIPCHK(launch_weight_update(weights, ov_old, ov_new, shift_term, fb_term, scalars, nodes, W, dt, sqrt(dt), h0p_re,
+                              h0p_im, w_floor, w_cap, damping, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// local energy from cached half Green's functions
+// ---------------------------------------------------------------------------
+static const int kEnergyGB = 32;
+
+size_t trot_local_energy_workspace(int W, int G) {
+  const int nblk = (G + kEnergyGB - 1) / kEnergyGB;
+  return align256((size_t)W * (nblk > 0 ? nblk : 1) * 16);
+}
+
+int trot_local_energy(const double* rot_chol_a, const double* rot_chol_b, const double* green_t_a,
+                      const double* green_t_b, const double* rot_h1t_a, const double* rot_h1t_b, int W, int N,
+                      int na, int nb, int G, double h0, double* energy, void* ws, size_t ws_bytes, void* stream) {
+  if (!green_t_a || !rot_h1t_a || !energy || (G > 0 && !rot_chol_a))
+    return fail(TROT_ERR_INVALID, "local_energy: null pointer%s%ld");
+  if (green_t_b && (!rot_h1t_b || (G > 0 && !rot_chol_b)))
+    return fail(TROT_ERR_INVALID, "local_energy: second spin incomplete%s%ld");
+  if (W < 1 || N < 1 || na < 1 || G < 0) return fail(TROT_ERR_INVALID, "local_energy: bad sizes%s%ld");
+  if (na > 64 || (green_t_b && nb > 64)) return fail(TROT_ERR_UNSUPPORTED, "local_energy: nocc > 64%s%ld");
+  if (ws_bytes < trot_local_energy_workspace(W, G)) return fail(TROT_ERR_WORKSPACE, "local_energy: workspace%s%ld");
+  HIPCHK(launch_energy(rot_chol_a, rot_chol_b, green_t_a, green_t_b, rot_h1t_a, rot_h1t_b, na, nb, N, G, W, h0,
+                       kEnergyGB, (double*)ws, energy, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// block energy tail: clamp, weighted mean, EMA of e_estimate (scalars[1])
+// ---------------------------------------------------------------------------
+int trot_block_energy(const double* energy, double* weights, double* scalars, double* out, int W, double dt,
+                      double shift_ema, void* stream) {
+  if (!energy || !weights || !scalars || !out) return fail(TROT_ERR_INVALID, "block_energy: null pointer%s%ld");
+  if (W < 1 || !(dt > 0)) return fail(TROT_ERR_INVALID, "block_energy: bad sizes%s%ld");
+  HIPCHK(launch_block_energy(energy, weights, scalars, out, W, sqrt(2.0 / dt), shift_ema, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// orthonormalisation by two passes of Cholesky-QR: Q = w R^-1 with R the
+// Cholesky factor of w^H w (real positive diagonal, the phase convention of
+// trot/walkers.py:137-148).
+// ---------------------------------------------------------------------------
+size_t trot_orthonormalize_workspace(int W, int N, int n) {
+  return 2 * align256((size_t)W * n * n * 16) + align256((size_t)W * N * n * 16);
+}
+
+int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t ws_bytes, void* stream) {
+  if (!walkers) return fail(TROT_ERR_INVALID, "orthonormalize: null pointer%s%ld");
+  if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "orthonormalize: bad sizes%s%ld");
+  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "orthonormalize: nocc > 64%s%ld");
+  Carve c{(char*)ws, ws_bytes, 0};
+  double* gram = c.take<double>((size_t)W * n * n * 2);
+  double* rinv = c.take<double>((size_t)W * n * n * 2);
+  double* tmp = c.take<double>((size_t)W * N * n * 2);
+  if (!gram || !rinv || !tmp) return fail(TROT_ERR_WORKSPACE, "orthonormalize: workspace too small%s%ld");
+  hipStream_t s = (hipStream_t)stream;
+  double* src = walkers;
+  double* dst = tmp;
+  for (int pass = 0; pass < 2; ++pass) {
+    GemmP p{};
+    p.A = src; p.lda = n; p.sA = (long)N * n;
+    p.B = src; p.ldb = n; p.sB = (long)N * n;
+    p.C = gram; p.ldc = n; p.sC = (long)n * n;
+    p.M = n; p.N = n; p.K = N; p.ar = 1.0;
+    HIPCHK((launch_gemm_f64<64, 64, true, true, false, true, true, EPI_STORE>(p, W, s)));
+    HIPCHK(launch_chol_rinv(gram, rinv, n, W, s));
+    GemmP q{};
+    q.A = src; q.lda = n; q.sA = (long)N * n;
+    q.B = rinv; q.ldb = n; q.sB = (long)n * n;
+    q.C = dst; q.ldc = n; q.sC = (long)N * n;
+    q.M = N; q.N = n; q.K = n; q.ar = 1.0;
+    HIPCHK((launch_gemm_f64<64, 64, true, true, true, true, false, EPI_STORE>(q, W, s)));
+    src = dst;
+    dst = walkers;
+  }
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stochastic reconfiguration (comb) and walker gather
+// ---------------------------------------------------------------------------
+size_t trot_sr_workspace(int W) { return align256((size_t)W * 8); }
+
+int trot_sr_comb(const double* weights, double zeta, int W, int* idx, double* weights_new, void* ws,
+                 size_t ws_bytes, void* stream) {
+  if (!weights || !idx || !weights_new) return fail(TROT_ERR_INVALID, "sr_comb: null pointer%s%ld");
+  if (W < 1) return fail(TROT_ERR_INVALID, "sr_comb: bad sizes%s%ld");
+  if (ws_bytes < trot_sr_workspace(W)) return fail(TROT_ERR_WORKSPACE, "sr_comb: workspace too small%s%ld");
+  HIPCHK(launch_sr_index(weights, (double*)ws, idx, weights_new, W, zeta, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+int trot_gather_rows(const double* src, double* dst, const int* idx, int W, long row_doubles, void* stream) {
+  if (!src || !dst || !idx || src == dst) return fail(TROT_ERR_INVALID, "gather_rows: bad pointers%s%ld");
+  if (W < 0 || row_doubles < 0) return fail(TROT_ERR_INVALID, "gather_rows: bad sizes%s%ld");
+  HIPCHK(launch_gather_rows(src, dst, idx, W, row_doubles, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context builds (trot/prop/chol_afqmc_ops.py:74-127, trot/meas/rhf.py:308-319)
+// ---------------------------------------------------------------------------
+int trot_build_rot_chol(const double* chol, const double* trial, int G, int N, int n, double* rot_chol,
+                        double* rot_chol_t, void* stream) {
+  if (!chol || !trial) return fail(TROT_ERR_INVALID, "build_rot_chol: null pointer%s%ld");
+  if (G < 1 || N < 1 || n < 1 || G > 65535) return fail(TROT_ERR_INVALID, "build_rot_chol: bad sizes%s%ld");
+  hipStream_t s = (hipStream_t)stream;
+  if (rot_chol) {  // rot_chol[g] = C^T L_g  (n x N)
+    GemmP p{};
+    p.A = trial; p.lda = n; p.sA = 0;
+    p.B = chol; p.ldb = N; p.sB = (long)N * N;
+    p.C = rot_chol; p.ldc = N; p.sC = (long)n * N;
+    p.M = n; p.N = N; p.K = N; p.ar = 1.0;
+    HIPCHK((launch_gemm_f64<64, 64, false, false, false, true, false, EPI_STORE>(p, G, s)));
+  }
+  if (rot_chol_t) {  // rot_chol_t[g] = L_g^T C  (N x n)
+    GemmP p{};
+    p.A = chol; p.lda = N; p.sA = (long)N * N;
+    p.B = trial; p.ldb = n; p.sB = 0;
+    p.C = rot_chol_t; p.ldc = n; p.sC = (long)N * n;
+    p.M = N; p.N = n; p.K = N; p.ar = 1.0;
+    HIPCHK((launch_gemm_f64<64, 64, false, false, false, true, false, EPI_STORE>(p, G, s)));
+  }
+  return TROT_OK;
+}
+
+size_t trot_build_prop_workspace(int G, int N) {
+  const int gc = G < 64 ? G : 64;
+  return align256((size_t)gc * N * N * 8);
+}
+
+// mf (G complex) = i tr(L_g dm); mfl (N x N complex) = sum_g mf_g L_g;
+// v0m (N x N real) = 1/2 sum_g L_g L_g.
+int trot_build_prop_tensors(const double* chol, const double* dm, int G, int N, double* mf, double* mfl,
+                            double* v0m, void* ws, size_t ws_bytes, void* stream) {
+  if (!chol || !dm || !mf || !mfl || !v0m) return fail(TROT_ERR_INVALID, "build_prop_tensors: null pointer%s%ld");
+  if (G < 1 || N < 1) return fail(TROT_ERR_INVALID, "build_prop_tensors: bad sizes%s%ld");
+  if (ws_bytes < trot_build_prop_workspace(G, N))
+    return fail(TROT_ERR_WORKSPACE, "build_prop_tensors: workspace too small%s%ld");
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(launch_mf_trace(chol, dm, G, N, mf, s));
+  {
+    GemmP p{};
+    p.A = mf; p.lda = G;
+    p.B = chol; p.ldb = (long)N * N;
+    p.C = mfl; p.ldc = (long)N * N;
+    p.M = 1; p.N = N * N; p.K = G; p.ar = 1.0;
+    HIPCHK((launch_gemm_f64<64, 64, true, false, true, true, false, EPI_STORE>(p, 1, s)));
+  }
+  const int gc = G < 64 ? G : 64;
+  double* tmp = (double*)ws;
+  for (int g0 = 0; g0 < G; g0 += gc) {
+    const int nb = (G - g0) < gc ? (G - g0) : gc;
+    GemmP p{};
+    p.A = chol + (size_t)g0 * N * N; p.lda = N; p.sA = (long)N * N;
+    p.B = chol + (size_t)g0 * N * N; p.ldb = N; p.sB = (long)N * N;
+    p.C = tmp; p.ldc = N; p.sC = (long)N * N;
+    p.M = N; p.N = N; p.K = N; p.ar = 1.0;
+    HIPCHK((launch_gemm_f64<64, 64, false, false, true, true, false, EPI_STORE>(p, nb, s)));
+    HIPCHK(launch_sum_batches(tmp, v0m, nb, (long)N * N, 0.5, g0 > 0, s));
+  }
+  return TROT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Shared definitions for the trot_amd HIP kernels (gfx950 / CDNA4 only).
+//
+// Complex data in HBM is interleaved (re, im) double pairs, the layout of
+// torch.complex128 and of the reference's complex128 arrays.  Kernels
+// de-interleave tiles into separate re / im planes in LDS and feed the
+// planes to real MFMA instructions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+#define TROT_WAVE 64
+
+struct cplx {
+  double re, im;
+};
+
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return {a.re * s, a.im * s}; }
+__device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
+__device__ __forceinline__ double cabs1(cplx a) { return fabs(a.re) + fabs(a.im); }
+__device__ __forceinline__ cplx cinv(cplx a) {
+  // Smith's algorithm for 1/a
+  if (fabs(a.re) >= fabs(a.im)) {
+    double r = a.im / a.re, d = a.re + a.im * r;
+    return {1.0 / d, -r / d};
+  } else {
+    double r = a.re / a.im, d = a.re * r + a.im;
+    return {r / d, -1.0 / d};
+  }
+}
+__device__ __forceinline__ cplx cload(const double* p, long idx) {
+  const double2 v = *reinterpret_cast<const double2*>(p + 2 * idx);
+  return {v.x, v.y};
+}
+__device__ __forceinline__ void cstore(double* p, long idx, cplx v) {
+  *reinterpret_cast<double2*>(p + 2 * idx) = make_double2(v.re, v.im);
+}
+
+// Deterministic block-wide sum of one double per thread (fixed tree).
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}

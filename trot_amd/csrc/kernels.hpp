@@ -1,0 +1,27 @@
+// Host-side launchers of the trot_amd kernels (definitions next to the kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define TROT_MAX_NOCC 64
+
+hipError_t launch_inv_det(const double* m, double* minv, double* det, int n, int W, hipStream_t s);
+hipError_t launch_overlap_combine(const double* det_a, const double* det_b, double* ovlp, int W, hipStream_t s);
+hipError_t launch_shift(const double* fields, const double* fb, const double* mf, double sqrt_dt, int W, int G,
+                        double* xbar, float* xbar32, double* shift, double* fbt, hipStream_t s);
+hipError_t launch_weight_update(double* weights, const double* ov_old, const double* ov_new, const double* shift,
+                                const double* fbt, double* scal, long long* nodes, int W, double dt, double sqrt_dt,
+                                double h0p_re, double h0p_im, double w_floor, double w_cap, double damping,
+                                hipStream_t s);
+hipError_t launch_chol_rinv(const double* gram, double* rinv, int n, int W, hipStream_t s);
+hipError_t launch_sr_index(const double* weights, double* cw, int* idx, double* w_new, int W, double zeta,
+                           hipStream_t s);
+hipError_t launch_gather_rows(const double* src, double* dst, const int* idx, int W, long row, hipStream_t s);
+hipError_t launch_energy(const double* rc_a, const double* rc_b, const double* gt_a, const double* gt_b,
+                         const double* h1t_a, const double* h1t_b, int na, int nb, int N, int G, int W, double h0,
+                         int gb, double* partial, double* energy, hipStream_t s);
+hipError_t launch_mf_trace(const double* chol, const double* dm, int G, int N, double* mf, hipStream_t s);
+hipError_t launch_sum_batches(const double* src, double* dst, int nb, long len, double scale, int accumulate,
+                              hipStream_t s);
+hipError_t launch_block_energy(const double* energy, double* weights, double* scal, double* out, int W,
+                               double thresh, double alpha, hipStream_t s);

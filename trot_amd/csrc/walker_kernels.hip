@@ -1,0 +1,439 @@
+// Per-walker FP64 kernels of the phaseless AFQMC step (gfx950).
+//
+//   inv_det_kernel        m -> (m^-1, det m): overlap and half Green's function
+//                         (trot/trial/rhf.py:45-47, trot/meas/rhf.py:21-25)
+//   shift_kernel          force bias -> shifted fields, shift and bias terms
+//                         (trot/prop/afqmc.py:99-103)
+//   weight_update_kernel  phaseless importance, masking, population shift
+//                         (trot/prop/afqmc.py:108-134)
+//   chol_rinv_kernel      Gram matrix -> R^-1 of its Cholesky factor (CholQR)
+//                         (trot/walkers.py:137-179)
+//   sr_index_kernel       comb stochastic reconfiguration indices
+//                         (trot/walkers.py:249-293)
+//   gather_rows_kernel    walker / cache gather after SR
+#include "common.hpp"
+#include "kernels.hpp"
+
+// ---------------------------------------------------------------------------
+// Gauss-Jordan inversion with partial pivoting (|re|+|im| pivot, as LAPACK's
+// izamax), in place in LDS.  One workgroup per walker, n <= TROT_MAX_NOCC.
+// det = prod(pivots) * (-1)^swaps.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__ m, double* __restrict__ minv,
+                                                      double* __restrict__ det, int n) {
+  extern __shared__ double sh[];
+  cplx* a = reinterpret_cast<cplx*>(sh);
+  cplx* col = a + n * n;
+  int* perm = reinterpret_cast<int*>(col + n);
+  __shared__ double redv[256];
+  __shared__ int redi[256];
+  const int t = threadIdx.x;
+  const long w = blockIdx.x;
+  const double* mw = m + 2 * w * n * n;
+  for (int e = t; e < n * n; e += 256) a[e] = cload(mw, e);
+  __syncthreads();
+  cplx dacc = {1.0, 0.0};
+  for (int k = 0; k < n; ++k) {
+    double best = -1.0;
+    int bi = n;
+    for (int i = k + t; i < n; i += 256) {
+      const double v = cabs1(a[i * n + k]);
+      if (v > best) { best = v; bi = i; }
+    }
+    redv[t] = best;
+    redi[t] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) {
+        const double v2 = redv[t + s];
+        const int i2 = redi[t + s];
+        if (v2 > redv[t] || (v2 == redv[t] && i2 < redi[t])) { redv[t] = v2; redi[t] = i2; }
+      }
+      __syncthreads();
+    }
+    int pr = redi[0];
+    if (pr >= n) pr = k;  // column of NaNs: keep the diagonal
+    __syncthreads();
+    if (t == 0) perm[k] = pr;
+    if (pr != k) {
+      for (int j = t; j < n; j += 256) {
+        const cplx x = a[k * n + j];
+        a[k * n + j] = a[pr * n + j];
+        a[pr * n + j] = x;
+      }
+    }
+    __syncthreads();
+    const cplx piv = a[k * n + k];
+    if (t == 0) {
+      dacc = cmul(dacc, piv);
+      if (pr != k) dacc = {-dacc.re, -dacc.im};
+    }
+    const cplx ipiv = cinv(piv);
+    for (int i = t; i < n; i += 256) col[i] = a[i * n + k];
+    __syncthreads();
+    for (int j = t; j < n; j += 256) {
+      const cplx x = (j == k) ? cplx{1.0, 0.0} : a[k * n + j];
+      a[k * n + j] = cmul(x, ipiv);
+    }
+    __syncthreads();
+    for (int e = t; e < n * n; e += 256) {
+      const int i = e / n, j = e - (e / n) * n;
+      if (i == k) continue;
+      const cplx x = (j == k) ? cplx{0.0, 0.0} : a[e];
+      a[e] = csub(x, cmul(col[i], a[k * n + j]));
+    }
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    const int pr = perm[k];
+    if (pr != k) {
+      for (int i = t; i < n; i += 256) {
+        const cplx x = a[i * n + k];
+        a[i * n + k] = a[i * n + pr];
+        a[i * n + pr] = x;
+      }
+    }
+    __syncthreads();
+  }
+  double* ow = minv + 2 * w * n * n;
+  for (int e = t; e < n * n; e += 256) cstore(ow, e, a[e]);
+  if (t == 0) cstore(det, w, dacc);
+}
+
+// ---------------------------------------------------------------------------
+// overlap from determinants: RHF restricted walkers det^2, unrestricted det_a*det_b
+// ---------------------------------------------------------------------------
+__global__ void overlap_combine_kernel(const double* __restrict__ det_a, const double* __restrict__ det_b,
+                                       double* __restrict__ ovlp, int W) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const cplx a = cload(det_a, w);
+  const cplx b = det_b ? cload(det_b, w) : a;
+  cstore(ovlp, w, cmul(a, b));
+}
+
+// ---------------------------------------------------------------------------
+// shifted fields.  fs = -sqrt(dt) (i fb - mf); xbar = x - fs;
+// shift = sum_g xbar_g mf_g; fbt = sum_g (x_g fs_g - fs_g^2 / 2).
+// One workgroup per walker, deterministic tree reduction.
+// xbar32 (optional): complex64 copy for the FP32 / split-BF16 VHS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void shift_kernel(const double* __restrict__ fields, const double* __restrict__ fb,
+                                                    const double* __restrict__ mf, double sqrt_dt, int G,
+                                                    double* __restrict__ xbar, float* __restrict__ xbar32,
+                                                    double* __restrict__ shift, double* __restrict__ fbt) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  const long w = blockIdx.x;
+  double s_re = 0, s_im = 0, f_re = 0, f_im = 0;
+  for (int g = t; g < G; g += 256) {
+    const double x = fields[w * G + g];
+    const cplx f = cload(fb, w * G + g);
+    const cplx m = cload(mf, g);
+    // i*fb - mf = (-f.im - m.re) + i (f.re - m.im)
+    const cplx ifm = {-f.im - m.re, f.re - m.im};
+    const cplx fsv = {-sqrt_dt * ifm.re, -sqrt_dt * ifm.im};
+    const cplx xb = {x - fsv.re, -fsv.im};
+    cstore(xbar, w * G + g, xb);
+    if (xbar32) {
+      *reinterpret_cast<float2*>(xbar32 + 2 * (w * G + g)) = make_float2((float)xb.re, (float)xb.im);
+    }
+    const cplx sm = cmul(xb, m);
+    s_re += sm.re;
+    s_im += sm.im;
+    const cplx fs2 = cmul(fsv, fsv);
+    f_re += x * fsv.re - 0.5 * fs2.re;
+    f_im += x * fsv.im - 0.5 * fs2.im;
+  }
+  s_re = block_sum<256>(s_re, red);
+  s_im = block_sum<256>(s_im, red);
+  f_re = block_sum<256>(f_re, red);
+  f_im = block_sum<256>(f_im, red);
+  if (t == 0) {
+    cstore(shift, w, {s_re, s_im});
+    cstore(fbt, w, {f_re, f_im});
+  }
+}
+
+// ---------------------------------------------------------------------------
+// phaseless weight update (single workgroup, deterministic).
+// scal[0] = pop_control_ene_shift (in/out), scal[1] = e_estimate (in).
+// nodes[0] += number of walkers with importance <= 0 after flooring.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void weight_update_kernel(
+    double* __restrict__ weights, const double* __restrict__ ov_old, const double* __restrict__ ov_new,
+    const double* __restrict__ shift, const double* __restrict__ fbt, double* __restrict__ scal,
+    long long* __restrict__ nodes, int W, double dt, double sqrt_dt, double h0p_re, double h0p_im, double w_floor,
+    double w_cap, double damping) {
+  __shared__ double red[1024];
+  const int t = threadIdx.x;
+  const double pop = scal[0];
+  const double e_est = scal[1];
+  double wsum = 0.0;
+  double ncount = 0.0;
+  for (int w = t; w < W; w += 1024) {
+    const cplx on = cload(ov_new, w), oo = cload(ov_old, w);
+    const cplx ratio = cmul(on, cinv(oo));
+    const cplx st = cload(shift, w), ft = cload(fbt, w);
+    const cplx ex = {-sqrt_dt * st.re + ft.re + dt * (pop + h0p_re), -sqrt_dt * st.im + ft.im + dt * h0p_im};
+    const double mag = exp(ex.re);
+    const cplx e1 = {mag * cos(ex.im), mag * sin(ex.im)};
+    const cplx imp = cmul(e1, ratio);
+    const double mag2 = exp(-sqrt_dt * st.re);
+    const double ph2 = -sqrt_dt * st.im;
+    const cplx e2 = {mag2 * cos(ph2), mag2 * sin(ph2)};
+    const cplx z = cmul(e2, ratio);
+    const double theta = atan2(z.im, z.re);
+    double p = hypot(imp.re, imp.im) * cos(theta);
+    if (!isfinite(p) || p < w_floor) p = 0.0;
+    if (p <= 0.0) ncount += 1.0;
+    if (p > w_cap) p = 0.0;
+    double wn = weights[w] * p;
+    if (wn > w_cap) wn = 0.0;
+    weights[w] = wn;
+    wsum += wn;
+  }
+  wsum = block_sum<1024>(wsum, red);
+  ncount = block_sum<1024>(ncount, red);
+  if (t == 0) {
+    double avg = wsum / (double)W;
+    if (avg < 1.0e-300) avg = 1.0e-300;
+    scal[0] = e_est - damping * (log(avg) / dt);
+    nodes[0] += (long long)ncount;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky of the Hermitian Gram matrix S = R^H R (R upper, real positive
+// diagonal) and R^-1.  One workgroup per walker.  rinv gets R^-1 (n x n).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict__ s, double* __restrict__ rinv,
+                                                        int n) {
+  extern __shared__ double sh[];
+  cplx* a = reinterpret_cast<cplx*>(sh);
+  cplx* x = a + n * n;
+  const int t = threadIdx.x;
+  const long w = blockIdx.x;
+  const double* sw = s + 2 * w * n * n;
+  for (int e = t; e < n * n; e += 256) a[e] = cload(sw, e);
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const double d = sqrt(a[k * n + k].re);
+    __syncthreads();
+    if (t == 0) a[k * n + k] = {d, 0.0};
+    const double id = 1.0 / d;
+    for (int j = k + 1 + t; j < n; j += 256) a[k * n + j] = cscale(a[k * n + j], id);
+    __syncthreads();
+    const int r = n - k - 1;
+    for (int e = t; e < r * r; e += 256) {
+      const int i = k + 1 + e / r, j = k + 1 + e % r;
+      if (j < i) continue;
+      a[i * n + j] = csub(a[i * n + j], cmul(cconj(a[k * n + i]), a[k * n + j]));
+    }
+    __syncthreads();
+  }
+  // R X = I, X upper: column j by thread j (back substitution)
+  for (int e = t; e < n * n; e += 256) x[e] = {0.0, 0.0};
+  __syncthreads();
+  for (int j = t; j < n; j += 256) {
+    x[j * n + j] = {1.0 / a[j * n + j].re, 0.0};
+    for (int i = j - 1; i >= 0; --i) {
+      cplx acc = {0.0, 0.0};
+      for (int l = i + 1; l <= j; ++l) acc = cadd(acc, cmul(a[i * n + l], x[l * n + j]));
+      x[i * n + j] = cscale(acc, -1.0 / a[i * n + i].re);
+    }
+  }
+  __syncthreads();
+  double* ow = rinv + 2 * w * n * n;
+  for (int e = t; e < n * n; e += 256) cstore(ow, e, x[e]);
+}
+
+// ---------------------------------------------------------------------------
+// comb SR: cw = cumsum(|w|) (sequential, the order of numpy/XLA's scan),
+// z_i = cw[-1] (i + zeta) / W, idx_i = searchsorted(cw, z_i, 'left') clamped to
+// W-1 (gather semantics), new weights = cw[-1] / W.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void sr_index_kernel(const double* __restrict__ weights, double* __restrict__ cw,
+                                                        int* __restrict__ idx, double* __restrict__ w_new, int W,
+                                                        double zeta) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    double acc = 0.0;
+    for (int i = 0; i < W; ++i) {
+      acc += fabs(weights[i]);
+      cw[i] = acc;
+    }
+  }
+  __syncthreads();
+  const double tot = cw[W - 1];
+  for (int i = t; i < W; i += 1024) {
+    const double z = tot * ((double)i + zeta) / (double)W;
+    int lo = 0, hi = W;  // first index with cw[idx] >= z
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cw[mid] < z) lo = mid + 1; else hi = mid;
+    }
+    idx[i] = lo < W ? lo : W - 1;
+  }
+  __syncthreads();
+  for (int i = t; i < W; i += 1024) w_new[i] = tot / (double)W;
+}
+
+// dst[i, :] = src[idx[i], :], rows of `row` doubles.  Grid (W).
+__global__ __launch_bounds__(256) void gather_rows_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                                          const int* __restrict__ idx, long row) {
+  const long i = blockIdx.x;
+  const long j = idx[i];
+  const double2* s = reinterpret_cast<const double2*>(src + j * row);
+  double2* d = reinterpret_cast<double2*>(dst + i * row);
+  const long n2 = row / 2;
+  for (long e = threadIdx.x; e < n2; e += 256) d[e] = s[e];
+  if ((row & 1) && threadIdx.x == 0) dst[i * row + row - 1] = src[j * row + row - 1];
+}
+
+// mf_g = i * sum_ij L_g[i,j] dm[j,i]  (trot/prop/chol_afqmc_ops.py:74-76), real L and dm.
+__global__ __launch_bounds__(256) void mf_trace_kernel(const double* __restrict__ chol, const double* __restrict__ dm,
+                                                       int N, double* __restrict__ mf) {
+  __shared__ double red[256];
+  const long g = blockIdx.x;
+  const double* lg = chol + g * (long)N * N;
+  double s = 0.0;
+  for (long e = threadIdx.x; e < (long)N * N; e += 256) {
+    const long i = e / N, j = e - (e / N) * N;
+    s += lg[e] * dm[j * N + i];
+  }
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) cstore(mf, g, {0.0, s});
+}
+
+// dst[e] (+)= scale * sum_b src[b * len + e], b ascending.
+__global__ void sum_batches_kernel(const double* __restrict__ src, double* __restrict__ dst, int nb, long len,
+                                   double scale, int accumulate) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += src[(long)b * len + e];
+  dst[e] = (accumulate ? dst[e] : 0.0) + scale * s;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static hipError_t set_lds(const void* f, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_inv_det(const double* m, double* minv, double* det, int n, int W, hipStream_t s) {
+  if (W <= 0) return hipSuccess;
+  if (n < 1 || n > TROT_MAX_NOCC) return hipErrorInvalidValue;
+  const size_t lds = (size_t)n * n * 16 + (size_t)n * 16 + (size_t)n * 4;
+  hipError_t e = set_lds((const void*)inv_det_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(inv_det_kernel, dim3(W), dim3(256), lds, s, m, minv, det, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_overlap_combine(const double* det_a, const double* det_b, double* ovlp, int W, hipStream_t s) {
+  if (W <= 0) return hipSuccess;
+  hipLaunchKernelGGL(overlap_combine_kernel, dim3((W + 255) / 256), dim3(256), 0, s, det_a, det_b, ovlp, W);
+  return hipGetLastError();
+}
+
+hipError_t launch_shift(const double* fields, const double* fb, const double* mf, double sqrt_dt, int W, int G,
+                        double* xbar, float* xbar32, double* shift, double* fbt, hipStream_t s) {
+  if (W <= 0) return hipSuccess;
+  hipLaunchKernelGGL(shift_kernel, dim3(W), dim3(256), 0, s, fields, fb, mf, sqrt_dt, G, xbar, xbar32, shift, fbt);
+  return hipGetLastError();
+}
+
+hipError_t launch_weight_update(double* weights, const double* ov_old, const double* ov_new, const double* shift,
+                                const double* fbt, double* scal, long long* nodes, int W, double dt, double sqrt_dt,
+                                double h0p_re, double h0p_im, double w_floor, double w_cap, double damping,
+                                hipStream_t s) {
+  if (W <= 0) return hipSuccess;
+  hipLaunchKernelGGL(weight_update_kernel, dim3(1), dim3(1024), 0, s, weights, ov_old, ov_new, shift, fbt, scal, nodes,
+                     W, dt, sqrt_dt, h0p_re, h0p_im, w_floor, w_cap, damping);
+  return hipGetLastError();
+}
+
+hipError_t launch_chol_rinv(const double* gram, double* rinv, int n, int W, hipStream_t s) {
+  if (W <= 0) return hipSuccess;
+  if (n < 1 || n > TROT_MAX_NOCC) return hipErrorInvalidValue;
+  const size_t lds = (size_t)2 * n * n * 16;
+  hipError_t e = set_lds((const void*)chol_rinv_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(chol_rinv_kernel, dim3(W), dim3(256), lds, s, gram, rinv, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_sr_index(const double* weights, double* cw, int* idx, double* w_new, int W, double zeta,
+                           hipStream_t s) {
+  if (W <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sr_index_kernel, dim3(1), dim3(1024), 0, s, weights, cw, idx, w_new, W, zeta);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const double* src, double* dst, const int* idx, int W, long row, hipStream_t s) {
+  if (W <= 0 || row <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_rows_kernel, dim3(W), dim3(256), 0, s, src, dst, idx, row);
+  return hipGetLastError();
+}
+
+hipError_t launch_mf_trace(const double* chol, const double* dm, int G, int N, double* mf, hipStream_t s) {
+  if (G <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mf_trace_kernel, dim3(G), dim3(256), 0, s, chol, dm, N, mf);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_batches(const double* src, double* dst, int nb, long len, double scale, int accumulate,
+                              hipStream_t s) {
+  if (len <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sum_batches_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, src, dst, nb, len,
+                     scale, accumulate);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// block measurement tail (trot/prop/blocks.py:236-257), single workgroup:
+//   e = Re E_L; non-finite or |e - E_est| > sqrt(2/dt) -> E_est; non-finite
+//   walkers get weight 0; e_block = sum w e / sum w (E_est if sum w == 0);
+//   E_est <- (1 - alpha) E_est + alpha e_block.
+// scal[1] = e_estimate (in/out); out[0] = e_block, out[1] = sum w.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void block_energy_kernel(const double* __restrict__ energy,
+                                                            double* __restrict__ weights, double* __restrict__ scal,
+                                                            double* __restrict__ out, int W, double thresh,
+                                                            double alpha) {
+  __shared__ double red[1024];
+  const int t = threadIdx.x;
+  const double e_ref = scal[1];
+  double sw = 0.0, swe = 0.0;
+  for (int w = t; w < W; w += 1024) {
+    double e = energy[2 * w];
+    const bool bad = !isfinite(e);
+    if (bad || fabs(e - e_ref) > thresh) e = e_ref;
+    double wt = weights[w];
+    if (bad) {
+      wt = 0.0;
+      weights[w] = 0.0;
+    }
+    sw += wt;
+    swe += wt * e;
+  }
+  sw = block_sum<1024>(sw, red);
+  swe = block_sum<1024>(swe, red);
+  if (t == 0) {
+    const double e_block = (sw == 0.0) ? e_ref : swe / sw;
+    scal[1] = (1.0 - alpha) * e_ref + alpha * e_block;
+    out[0] = e_block;
+    out[1] = sw;
+  }
+}
+
+hipError_t launch_block_energy(const double* energy, double* weights, double* scal, double* out, int W,
+                               double thresh, double alpha, hipStream_t s) {
+  if (W <= 0) return hipSuccess;
+  hipLaunchKernelGGL(block_energy_kernel, dim3(1), dim3(1024), 0, s, energy, weights, scal, out, W, thresh, alpha);
+  return hipGetLastError();
+}

@@ -1,0 +1,219 @@
+"""Half-rotated measurement context and batched GPU force-bias / energy kernels.
+
+Mirrors trot/meas/rhf.py (make_rhf_meas_ops, build_meas_ctx, force_bias and
+energy kernels for restricted and unrestricted walkers) and trot/meas/uhf.py
+(UHF trial, unrestricted walkers).  Every kernel takes a batch of walkers;
+a single walker (no leading axis) is promoted to a batch of one.
+
+Layouts (device, float64): rot_chol (G, n, N) = C^T L_g as in the reference;
+rot_chol_t (G, N, n) = its transpose per g, which is the force-bias operand
+matching the transposed Green's function GT = w (C^T w)^-1; rot_h1t (N, n).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _lib
+from .core import MeasOps, System, k_energy, k_force_bias
+from .trial import RhfTrial, UhfTrial, _batched, overlap, overlap_green
+
+
+@dataclass(frozen=True)
+class RhfMeasCfg:
+    """trot/meas/rhf.py:107-109.  Both memory modes run the same streaming kernel."""
+
+    memory_mode: str = "high"
+
+
+def _validate_memory_mode(memory_mode: str) -> str:
+    if memory_mode not in ("high", "low"):
+        raise ValueError(f"Unsupported RHF measurement memory_mode: {memory_mode!r}")
+    return memory_mode
+
+
+@dataclass
+class SpinCtx:
+    rot_h1: torch.Tensor  # (n, N)
+    rot_h1t: torch.Tensor  # (N, n)
+    rot_chol: torch.Tensor  # (G, n, N)
+    rot_chol_t: torch.Tensor  # (G, N, n)
+    low: dict = field(default_factory=dict)  # reduced-precision copies, filled on demand
+
+    @property
+    def rot_chol_flat(self) -> torch.Tensor:
+        """(G, n*N), the reference's flattened layout (trot/meas/rhf.py:318)."""
+        return self.rot_chol.reshape(self.rot_chol.shape[0], -1)
+
+
+@dataclass
+class RhfMeasCtx:
+    """trot/meas/rhf.py:282-305."""
+
+    spin: SpinCtx
+    cfg: RhfMeasCfg = RhfMeasCfg()
+
+    @property
+    def rot_h1(self):
+        return self.spin.rot_h1
+
+    @property
+    def rot_chol(self):
+        return self.spin.rot_chol
+
+    @property
+    def rot_chol_flat(self):
+        return self.spin.rot_chol_flat
+
+
+@dataclass
+class UhfMeasCtx:
+    """trot/meas/uhf.py:277-315."""
+
+    spin_a: SpinCtx
+    spin_b: SpinCtx
+
+    @property
+    def rot_h1_a(self):
+        return self.spin_a.rot_h1
+
+    @property
+    def rot_h1_b(self):
+        return self.spin_b.rot_h1
+
+    @property
+    def rot_chol_a(self):
+        return self.spin_a.rot_chol
+
+    @property
+    def rot_chol_b(self):
+        return self.spin_b.rot_chol
+
+
+def _spin_ctx(h1: torch.Tensor, chol: torch.Tensor, c: torch.Tensor) -> SpinCtx:
+    rc, rct = _lib.build_rot_chol(chol, c)
+    rot_h1 = (c.T @ h1).contiguous()
+    return SpinCtx(rot_h1=rot_h1, rot_h1t=rot_h1.T.contiguous(), rot_chol=rc, rot_chol_t=rct)
+
+
+def _require_chol(ham_data):
+    if ham_data.basis != "restricted":
+        raise ValueError("RHF/UHF MeasOps assume HamChol.basis == 'restricted'.")
+    if ham_data.chol is None:
+        raise ValueError("build_meas_ctx needs ham_data.chol")
+    h1 = ham_data.h1
+    chol = ham_data.chol
+    if not isinstance(h1, torch.Tensor) or h1.device.type != "cuda" or not isinstance(chol, torch.Tensor):
+        ham_data = ham_data.to_device()
+        h1, chol = ham_data.h1, ham_data.chol
+    return h1, chol
+
+
+def build_meas_ctx(ham_data, trial_data: RhfTrial, cfg: RhfMeasCfg = RhfMeasCfg()) -> RhfMeasCtx:
+    """trot/meas/rhf.py:308-319 (C^T h1 and C^T L_g on the GPU)."""
+    h1, chol = _require_chol(ham_data)
+    return RhfMeasCtx(spin=_spin_ctx(h1, chol, trial_data.mo_coeff), cfg=cfg)
+
+
+def build_uhf_meas_ctx(ham_data, trial_data: UhfTrial) -> UhfMeasCtx:
+    """trot/meas/uhf.py:318-336."""
+    h1, chol = _require_chol(ham_data)
+    return UhfMeasCtx(spin_a=_spin_ctx(h1, chol, trial_data.mo_coeff_a),
+                      spin_b=_spin_ctx(h1, chol, trial_data.mo_coeff_b))
+
+
+class Engine:
+    """Batched GPU evaluation shared by the kernels and the propagator.
+
+    kind: "rr" (RHF trial, restricted walkers), "ru" (RHF trial, unrestricted
+    walkers) or "uu" (UHF trial, unrestricted walkers).
+    """
+
+    def __init__(self, kind: str):
+        self.kind = kind
+
+    def spins(self, meas_ctx):
+        if isinstance(meas_ctx, UhfMeasCtx):
+            return meas_ctx.spin_a, meas_ctx.spin_b
+        return meas_ctx.spin, meas_ctx.spin
+
+    def overlap_green(self, walkers, trial_data):
+        return overlap_green(walkers, trial_data, want_green=True)
+
+    def force_bias_from_green(self, greens, meas_ctx) -> torch.Tensor:
+        sa, sb = self.spins(meas_ctx)
+        if self.kind == "rr":
+            return _lib.force_bias(greens[0], sa.rot_chol_t, 2.0)  # trot/meas/rhf.py:124-125
+        fb = _lib.force_bias(greens[0], sa.rot_chol_t, 1.0)  # trot/meas/rhf.py:150-153; uhf.py:55-57
+        return _lib.force_bias(greens[1], sb.rot_chol_t, 1.0, out=fb, accumulate=True)
+
+    def energy_from_green(self, greens, h0: float, meas_ctx) -> torch.Tensor:
+        sa, sb = self.spins(meas_ctx)
+        if self.kind == "rr":
+            return _lib.local_energy(sa.rot_chol, greens[0], sa.rot_h1t, h0)
+        return _lib.local_energy(sa.rot_chol, greens[0], sa.rot_h1t, h0, sb.rot_chol, greens[1], sb.rot_h1t)
+
+
+def _h0(ham_data) -> float:
+    h0 = ham_data.h0
+    if isinstance(h0, torch.Tensor):
+        h0 = h0.detach().cpu().item()
+    return float(complex(h0).real)
+
+
+def _make_kernels(engine: Engine):
+    def force_bias_kernel(walker, ham_data, meas_ctx, trial_data):
+        w, squeeze = _batched(walker)
+        _, greens = engine.overlap_green(w, trial_data)
+        fb = engine.force_bias_from_green(greens, meas_ctx)
+        return fb[0] if squeeze else fb
+
+    def energy_kernel(walker, ham_data, meas_ctx, trial_data):
+        w, squeeze = _batched(walker)
+        _, greens = engine.overlap_green(w, trial_data)
+        e = engine.energy_from_green(greens, _h0(ham_data), meas_ctx)
+        return e[0] if squeeze else e
+
+    return force_bias_kernel, energy_kernel
+
+
+def make_rhf_meas_ops(sys: System, memory_mode: str = "high") -> MeasOps:
+    """trot/meas/rhf.py:322-367 for restricted and unrestricted walkers."""
+    cfg = RhfMeasCfg(memory_mode=_validate_memory_mode(memory_mode))
+    wk = sys.walker_kind
+    if wk == "restricted":
+        engine = Engine("rr")
+    elif wk == "unrestricted":
+        engine = Engine("ru")
+    elif wk == "generalized":
+        raise NotImplementedError("generalized walkers are not part of the trot_amd GPU path")
+    else:
+        raise ValueError(f"unknown walker_kind: {sys.walker_kind}")
+    fb, en = _make_kernels(engine)
+    return MeasOps(
+        overlap=overlap,
+        build_meas_ctx=lambda ham_data, trial_data: build_meas_ctx(ham_data, trial_data, cfg),
+        kernels={k_force_bias: fb, k_energy: en},
+        observables={},
+        engine=engine,
+    )
+
+
+def make_uhf_meas_ops(sys: System) -> MeasOps:
+    """trot/meas/uhf.py:339-379 (unrestricted walkers)."""
+    wk = sys.walker_kind
+    if wk in ("restricted", "generalized"):
+        raise NotImplementedError(f"UHF trial with {wk} walkers is not part of the trot_amd GPU path")
+    if wk != "unrestricted":
+        raise ValueError(f"unknown walker_kind: {sys.walker_kind}")
+    engine = Engine("uu")
+    fb, en = _make_kernels(engine)
+    return MeasOps(
+        overlap=overlap,
+        build_meas_ctx=build_uhf_meas_ctx,
+        kernels={k_force_bias: fb, k_energy: en},
+        observables={},
+        engine=engine,
+    )

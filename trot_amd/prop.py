@@ -1,0 +1,312 @@
+"""Phaseless AFQMC propagation on the GPU: a drop-in for trot/prop/afqmc.py.
+
+``make_prop_ops(ham_basis, walker_kind, mixed_precision=False)`` returns
+``PropOps(init_prop_state, build_prop_ctx, step)`` with the reference's
+signatures (trot/prop/types.py:65-103).  Differences, all documented:
+
+* auxiliary fields come from a field provider on the propagation context
+  instead of JAX's threefry stream.  ``PropState.rng_key`` is a
+  ``(seed, counter)`` pair; each step consumes one counter value.  The
+  default provider draws N(0,1) on the GPU (torch Philox generator);
+  ``NumpyFields`` draws the same values as ``trot_amd.synthetic.fields``
+  so runs can be compared with a CPU restatement;
+* the step returns a new state and never writes into the input state, so
+  the reference's functional semantics hold;
+* ``PropState.green`` caches the half Green's functions of ``walkers``
+  (computed together with the overlaps); the next force bias reuses them
+  instead of recomputing C^T w and its inverse from the same walkers.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Any, Callable, NamedTuple
+
+import numpy as np
+import scipy.linalg
+import torch
+
+from . import _lib
+from .core import MeasOps, System, TrialOps, k_energy
+from . import walkers as wk
+
+
+def _random_seed() -> int:
+    return int(np.random.randint(0, int(1e6)))
+
+
+@dataclass(frozen=True)
+class QmcParamsBase:
+    """trot/prop/types.py:30-38."""
+
+    dt: float = 0.005
+    n_chunks: int = 1
+    n_exp_terms: int = 6
+    n_prop_steps: int = 50
+    n_blocks: int = 200
+    n_walkers: int = 200
+    seed: int = field(default_factory=_random_seed)
+
+
+@dataclass(frozen=True)
+class QmcParams(QmcParamsBase):
+    """trot/prop/types.py:41-47."""
+
+    pop_control_damping: float = 0.1
+    weight_floor: float = 1.0e-3
+    weight_cap: float = 100.0
+    shift_ema: float = 0.1
+    n_eql_blocks: int = 20
+
+
+class PropState(NamedTuple):
+    """trot/prop/types.py:20-27, plus the Green's-function cache ``green``."""
+
+    walkers: Any
+    weights: torch.Tensor
+    overlaps: torch.Tensor
+    rng_key: Any
+    pop_control_ene_shift: torch.Tensor
+    e_estimate: torch.Tensor
+    node_encounters: torch.Tensor
+    green: Any = None
+
+
+# ---------------------------------------------------------------------------
+# auxiliary-field providers
+# ---------------------------------------------------------------------------
+
+
+class DeviceFields:
+    """N(0,1) fields drawn on the GPU, one torch Philox stream per (seed, counter)."""
+
+    def __call__(self, key, n_walkers: int, n_fields: int, device):
+        seed, ctr = key
+        g = torch.Generator(device=device)
+        g.manual_seed((int(seed) * 1_000_003 + int(ctr)) & 0x7FFFFFFFFFFFFFFF)
+        x = torch.randn((n_walkers, n_fields), dtype=torch.float64, device=device, generator=g)
+        return x, (seed, ctr + 1)
+
+    def uniform(self, key):
+        seed, ctr = key
+        rng = np.random.default_rng([int(seed), 104729, int(ctr)])
+        return float(rng.uniform()), (seed, ctr + 1)
+
+
+class NumpyFields:
+    """Fields drawn on the host with numpy PCG64 as trot_amd.synthetic.fields(seed, counter, W, G)."""
+
+    def __call__(self, key, n_walkers: int, n_fields: int, device):
+        from .synthetic import fields
+
+        seed, ctr = key
+        x = torch.from_numpy(fields(int(seed), int(ctr), n_walkers, n_fields)).to(device)
+        return x, (seed, ctr + 1)
+
+    def uniform(self, key):
+        seed, ctr = key
+        rng = np.random.default_rng([int(seed), 104729, int(ctr)])
+        return float(rng.uniform()), (seed, ctr + 1)
+
+
+# ---------------------------------------------------------------------------
+# propagation context
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class CholAfqmcCtx:
+    """trot/prop/chol_afqmc_ops.py:16-50 plus the field provider and precision."""
+
+    dt: float
+    sqrt_dt: float
+    exp_h1_half: torch.Tensor  # (N, N) float64
+    mf_shifts: torch.Tensor  # (G,) complex128
+    h0_prop: complex
+    chol_flat: torch.Tensor  # (G, N*N) float64
+    norb: int
+    fields: Callable = field(default_factory=DeviceFields)
+    precision: str = "f64"
+    walker_chunk: int = 0
+    low: dict = field(default_factory=dict)  # reduced-precision operand copies
+
+
+def _total_dm(rdm1) -> torch.Tensor:
+    dm = rdm1 if isinstance(rdm1, torch.Tensor) else torch.as_tensor(np.asarray(rdm1))
+    if dm.dim() == 3 and dm.shape[0] == 2:
+        dm = dm[0] + dm[1]  # trot/prop/chol_afqmc_ops.py:57-60
+    if dm.is_complex():
+        if torch.count_nonzero(dm.imag).item():
+            raise ValueError("trot_amd supports real trial density matrices only")
+        dm = dm.real
+    return dm.to(device="cuda", dtype=torch.float64).contiguous()
+
+
+def build_prop_ctx(ham_data, rdm1, dt: float, precision: str = "f64", walker_chunk: int = 0) -> CholAfqmcCtx:
+    """trot/prop/chol_afqmc_ops.py:103-127 (and the host twin trot/runtime_layout.py:125-197).
+
+    mf, sum_g mf_g L_g and 1/2 sum_g L_g L_g run on the GPU; the N x N
+    exponential exp(-dt/2 h1_eff) is evaluated on the host (scipy Pade),
+    as the reference's host runtime layout does (runtime_layout.py:171).
+    """
+    if ham_data.basis != "restricted":
+        raise NotImplementedError("trot_amd propagates restricted-basis Hamiltonians")
+    if not isinstance(ham_data.chol, torch.Tensor) or ham_data.chol.device.type != "cuda":
+        ham_data = ham_data.to_device()
+    chol = ham_data.chol
+    G, N, _ = chol.shape
+    dm = _total_dm(rdm1)
+    mf, mfl, v0m = _lib.build_prop_tensors(chol, dm)
+    h0 = float(complex(ham_data.h0 if not isinstance(ham_data.h0, torch.Tensor) else ham_data.h0.item()).real)
+    mf_h = mf.cpu().numpy()
+    h0_prop = complex(-h0 - 0.5 * np.sum(mf_h**2))
+    h1_eff = ham_data.h1 - v0m - (1j * mfl).real  # trot/prop/chol_afqmc_ops.py:90-100
+    e = scipy.linalg.expm(-0.5 * dt * h1_eff.cpu().numpy())
+    ctx = CholAfqmcCtx(
+        dt=float(dt),
+        sqrt_dt=float(np.sqrt(dt)),
+        exp_h1_half=torch.from_numpy(np.ascontiguousarray(e)).to("cuda"),
+        mf_shifts=mf,
+        h0_prop=h0_prop,
+        chol_flat=chol.reshape(G, N * N),
+        norb=N,
+        precision=precision,
+        walker_chunk=walker_chunk,
+    )
+    return ctx
+
+
+# ---------------------------------------------------------------------------
+# state init (trot/prop/afqmc.py:19-76)
+# ---------------------------------------------------------------------------
+
+
+def _scalars(pop, e_est) -> torch.Tensor:
+    return torch.tensor([float(pop), float(e_est)], dtype=torch.float64, device="cuda")
+
+
+def init_prop_state(*, sys: System, ham_data, trial_ops: TrialOps, trial_data, meas_ops: MeasOps, params,
+                    initial_walkers=None, initial_e_estimate=None, rdm1=None, mesh=None) -> PropState:
+    """trot/prop/afqmc.py:19-76.  ``mesh`` must be None (one GPU)."""
+    if mesh is not None:
+        raise NotImplementedError("trot_amd runs on one GPU; mesh sharding is not supported")
+    if initial_walkers is None:
+        if rdm1 is None:
+            rdm1 = trial_ops.get_rdm1(trial_data)
+        initial_walkers = wk.init_walkers(sys=sys, rdm1=rdm1, n_walkers=params.n_walkers)
+    initial_walkers = wk.as_device_walkers(initial_walkers)
+    engine = meas_ops.engine
+    overlaps, greens = engine.overlap_green(initial_walkers, trial_data)
+    if initial_e_estimate is None:
+        meas_ctx = meas_ops.build_meas_ctx(ham_data, trial_data)
+        w0 = wk.take_walkers(initial_walkers, [0])
+        _, g0 = engine.overlap_green(w0, trial_data)
+        from .meas import _h0
+
+        e = engine.energy_from_green(g0, _h0(ham_data), meas_ctx)
+        e_est = float(e.real.mean().item())
+    else:
+        e_est = float(np.real(np.asarray(initial_e_estimate.cpu() if isinstance(initial_e_estimate, torch.Tensor)
+                                         else initial_e_estimate)))
+    scal = _scalars(e_est, e_est)
+    return PropState(
+        walkers=initial_walkers,
+        weights=torch.ones((params.n_walkers,), dtype=torch.float64, device="cuda"),
+        overlaps=overlaps,
+        rng_key=(int(params.seed), 0),
+        pop_control_ene_shift=scal[0],
+        e_estimate=scal[1],
+        node_encounters=torch.zeros((1,), dtype=torch.int64, device="cuda"),
+        green=greens,
+    )
+
+
+# ---------------------------------------------------------------------------
+# the step (trot/prop/afqmc.py:79-144)
+# ---------------------------------------------------------------------------
+
+
+def apply_trotter(walkers, xbar: torch.Tensor, ctx: CholAfqmcCtx, n_terms: int):
+    """trot/prop/chol_afqmc_ops.py:198-222: E T_K(i sqrt(dt) V(xbar)) E w, V shared by both spins."""
+    if isinstance(walkers, tuple):
+        a, b = _lib.apply_trotter(walkers[0], walkers[1], xbar, ctx.chol_flat, ctx.exp_h1_half, ctx.sqrt_dt,
+                                  n_terms, ctx.walker_chunk)
+        return (a, b)
+    a, _ = _lib.apply_trotter(walkers, None, xbar, ctx.chol_flat, ctx.exp_h1_half, ctx.sqrt_dt, n_terms,
+                              ctx.walker_chunk)
+    return a
+
+
+def afqmc_step(state: PropState, *, params, ham_data, trial_data, meas_ops: MeasOps, prop_ctx: CholAfqmcCtx,
+               meas_ctx, fields: torch.Tensor | None = None) -> PropState:
+    """One phaseless step for all walkers.  ``fields`` (W, G) overrides the provider."""
+    engine = meas_ops.engine
+    nw = wk.n_walkers(state.walkers)
+    G = prop_ctx.chol_flat.shape[0]
+    greens = state.green
+    if greens is None:
+        _, greens = engine.overlap_green(state.walkers, trial_data)
+    key = state.rng_key
+    if fields is None:
+        fields, key = prop_ctx.fields(key, nw, G, state.weights.device)
+    else:
+        key = (key[0], key[1] + 1)
+    fb = engine.force_bias_from_green(greens, meas_ctx)  # afqmc.py:95-98
+    xbar, shift_term, fb_term = _lib.field_shift(fields, fb, prop_ctx.mf_shifts, prop_ctx.sqrt_dt)  # :99-103
+    walkers_new = apply_trotter(state.walkers, xbar, prop_ctx, params.n_exp_terms)  # :104-106
+    overlaps_new, greens_new = engine.overlap_green(walkers_new, trial_data)  # :108-110
+    weights = state.weights.clone()
+    scal = torch.stack([state.pop_control_ene_shift, state.e_estimate]).to(torch.float64).contiguous()
+    nodes = state.node_encounters.clone()
+    _lib.phaseless_update(weights, state.overlaps, overlaps_new, shift_term, fb_term, scal, nodes, prop_ctx.dt,
+                          prop_ctx.h0_prop, getattr(params, "weight_floor", 1.0e-3),
+                          getattr(params, "weight_cap", 100.0), getattr(params, "pop_control_damping", 0.1))
+    return PropState(
+        walkers=walkers_new,
+        weights=weights,
+        overlaps=overlaps_new,
+        rng_key=key,
+        pop_control_ene_shift=scal[0],
+        e_estimate=scal[1],
+        node_encounters=nodes,
+        green=greens_new,
+    )
+
+
+@dataclass(frozen=True)
+class PropOps:
+    """trot/prop/types.py:99-103."""
+
+    init_prop_state: Callable
+    build_prop_ctx: Callable
+    step: Callable
+
+
+def make_prop_ops(ham_basis: str, walker_kind: str, mixed_precision: bool = False, *,
+                  precision: str | None = None, walker_chunk: int = 0) -> PropOps:
+    """trot/prop/afqmc.py:147-180.
+
+    ``mixed_precision=True`` selects the FP32 Cholesky GEMMs (the reference's
+    mixed mode, trot/prop/chol_afqmc_ops.py:247-257); ``precision`` may name
+    "f64", "f32" or "bf16x3" explicitly.
+    """
+    if ham_basis not in ("restricted", "generalized"):
+        raise ValueError(f"unknown ham_basis: {ham_basis}")
+    walker_kind = walker_kind.lower()
+    if walker_kind not in ("restricted", "unrestricted", "generalized"):
+        raise ValueError(f"unknown walker_kind: {walker_kind}")
+    if ham_basis != "restricted" or walker_kind == "generalized":
+        raise NotImplementedError(
+            f"Not implemented for ham_basis={ham_basis} and walker_kind={walker_kind}")
+    prec = precision or ("f32" if mixed_precision else "f64")
+    if prec not in ("f64", "f32", "bf16x3"):
+        raise ValueError(f"unknown precision: {prec}")
+
+    def step(state, *, params, ham_data, trial_data, trial_ops, meas_ops, meas_ctx, prop_ctx, fields=None):
+        return afqmc_step(state, params=params, ham_data=ham_data, trial_data=trial_data, meas_ops=meas_ops,
+                          prop_ctx=prop_ctx, meas_ctx=meas_ctx, fields=fields)
+
+    def _build(ham_data, rdm1, params):
+        return build_prop_ctx(ham_data, rdm1, params.dt, precision=prec, walker_chunk=walker_chunk)
+
+    return PropOps(init_prop_state=init_prop_state, build_prop_ctx=_build, step=step)

@@ -103,14 +103,14 @@ def total_rdm1(rdm1: np.ndarray) -> np.ndarray:
 def mf_shifts(chol: np.ndarray, rdm1: np.ndarray) -> np.ndarray:
     """trot/prop/chol_afqmc_ops.py:74-76: mf_g = i * sum_ij L_g,ij dm_ji."""
     dm = total_rdm1(rdm1)
-    return 1.0j * np.einsum("gij,ji->g", chol, dm)
+    return 1.0j * np.einsum("gij,ji->g", chol, dm, optimize=True)
 
 
 def h1_eff(h1: np.ndarray, chol: np.ndarray, mf: np.ndarray) -> np.ndarray:
     """trot/prop/chol_afqmc_ops.py:90-100: h1 - 1/2 sum_g L_g L_g - sum_g Re(i mf_g) L_g."""
-    v0m = 0.5 * np.einsum("gik,gkj->ij", chol, chol)
+    v0m = 0.5 * np.einsum("gik,gkj->ij", chol, chol, optimize=True)
     mf_r = (1.0j * mf).real
-    v1m = np.einsum("g,gik->ik", mf_r, chol)
+    v1m = np.einsum("g,gik->ik", mf_r, chol, optimize=True)
     return h1 - v0m - v1m
 
 
@@ -150,7 +150,7 @@ def uhf_rdm1(ca: np.ndarray, cb: np.ndarray) -> np.ndarray:
 
 def overlap_r(w: np.ndarray, c: np.ndarray) -> np.ndarray:
     """trot/trial/rhf.py:45-47: det(C^H w)^2, batched over walkers."""
-    m = np.einsum("ip,wiq->wpq", c.conj(), w)
+    m = np.einsum("ip,wiq->wpq", c.conj(), w, optimize=True)
     return np.linalg.det(m) ** 2
 
 
@@ -158,8 +158,8 @@ def overlap_u(w: tuple, ca: np.ndarray, cb: np.ndarray | None = None) -> np.ndar
     """trot/trial/rhf.py:50-54 (cb=None: RHF trial) and trot/trial/uhf.py:54-58."""
     wu, wd = w
     cb = ca if cb is None else cb
-    mu = np.einsum("ip,wiq->wpq", ca.conj(), wu)
-    md = np.einsum("ip,wiq->wpq", cb.conj(), wd)
+    mu = np.einsum("ip,wiq->wpq", ca.conj(), wu, optimize=True)
+    md = np.einsum("ip,wiq->wpq", cb.conj(), wd, optimize=True)
     return np.linalg.det(mu) * np.linalg.det(md)
 
 
@@ -192,7 +192,7 @@ def build_meas_ctx_rhf(h1: np.ndarray, chol: np.ndarray, c: np.ndarray, memory_m
     """trot/meas/rhf.py:308-319."""
     ch = c.conj().T
     rot_h1 = ch @ h1
-    rot_chol = np.einsum("pi,gij->gpj", ch, chol)
+    rot_chol = np.einsum("pi,gij->gpj", ch, chol, optimize=True)
     return RhfMeasCtx(rot_h1, rot_chol, rot_chol.reshape(rot_chol.shape[0], -1), memory_mode)
 
 
@@ -202,21 +202,21 @@ def build_meas_ctx_uhf(h1: np.ndarray, chol: np.ndarray, ca: np.ndarray, cb: np.
     return UhfMeasCtx(
         rot_h1_a=caH @ h1,
         rot_h1_b=cbH @ h1,
-        rot_chol_a=np.einsum("pi,gij->gpj", caH, chol),
-        rot_chol_b=np.einsum("pi,gij->gpj", cbH, chol),
+        rot_chol_a=np.einsum("pi,gij->gpj", caH, chol, optimize=True),
+        rot_chol_b=np.einsum("pi,gij->gpj", cbH, chol, optimize=True),
     )
 
 
 def half_green(w: np.ndarray, c: np.ndarray) -> np.ndarray:
     """trot/meas/rhf.py:21-25 with m = C^H w (rhf.py:121): G_half = solve(m^T, w^T), (W, n, N)."""
-    m = np.einsum("ip,wiq->wpq", c.conj(), w)
+    m = np.einsum("ip,wiq->wpq", c.conj(), w, optimize=True)
     return np.linalg.solve(np.swapaxes(m, 1, 2), np.swapaxes(w, 1, 2))
 
 
 def force_bias_rw(w: np.ndarray, ctx: RhfMeasCtx, c: np.ndarray) -> np.ndarray:
     """trot/meas/rhf.py:119-125: 2 * rot_chol_flat @ vec(G_half)."""
     g = half_green(w, c)
-    return 2.0 * np.einsum("gk,wk->wg", ctx.rot_chol_flat, g.reshape(g.shape[0], -1))
+    return 2.0 * np.einsum("gk,wk->wg", ctx.rot_chol_flat, g.reshape(g.shape[0], -1), optimize=True)
 
 
 def force_bias_uw(w: tuple, ctx: RhfMeasCtx, c: np.ndarray) -> np.ndarray:
@@ -224,36 +224,37 @@ def force_bias_uw(w: tuple, ctx: RhfMeasCtx, c: np.ndarray) -> np.ndarray:
     gu = half_green(w[0], c)
     gd = half_green(w[1], c)
     g = gu + gd
-    return np.einsum("gk,wk->wg", ctx.rot_chol_flat, g.reshape(g.shape[0], -1))
+    return np.einsum("gk,wk->wg", ctx.rot_chol_flat, g.reshape(g.shape[0], -1), optimize=True)
 
 
 def force_bias_uhf(w: tuple, ctx: UhfMeasCtx, ca: np.ndarray, cb: np.ndarray) -> np.ndarray:
     """trot/meas/uhf.py:43-57 (UHF trial, unrestricted walkers)."""
     gu = half_green(w[0], ca)
     gd = half_green(w[1], cb)
-    return np.einsum("gij,wij->wg", ctx.rot_chol_a, gu) + np.einsum("gij,wij->wg", ctx.rot_chol_b, gd)
+    return (np.einsum("gij,wij->wg", ctx.rot_chol_a, gu, optimize=True)
+            + np.einsum("gij,wij->wg", ctx.rot_chol_b, gd, optimize=True))
 
 
 def _exchange_restricted_high(g: np.ndarray, rot_chol: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     """trot/meas/rhf.py:57-60: f_g = rot_chol[g] @ G^T; returns (sum_g c_g^2, sum_g sum f*f^T)."""
-    f = np.einsum("gij,wkj->wgik", rot_chol, g)  # (W, G, n, n)
-    c = np.einsum("wgii->wg", f)
-    exc = np.einsum("wgij,wgji->w", f, f)
+    f = np.einsum("gij,wkj->wgik", rot_chol, g, optimize=True)  # (W, G, n, n)
+    c = np.einsum("wgii->wg", f, optimize=True)
+    exc = np.einsum("wgij,wgji->w", f, f, optimize=True)
     return np.sum(c * c, axis=1), exc
 
 
 def energy_rw(w: np.ndarray, h0: float, ctx: RhfMeasCtx, c: np.ndarray) -> np.ndarray:
     """trot/meas/rhf.py:128-138 with _two_body_energy_restricted (rhf.py:41-60)."""
     g = half_green(w, c)
-    e1 = 2.0 * np.einsum("wij,ij->w", g, ctx.rot_h1)
+    e1 = 2.0 * np.einsum("wij,ij->w", g, ctx.rot_h1, optimize=True)
     if ctx.memory_mode == "low":
         c2 = np.zeros(g.shape[0], dtype=complex)
         exc = np.zeros(g.shape[0], dtype=complex)
         for rc in ctx.rot_chol:  # rhf.py:43-55 scan over Cholesky vectors
-            f = np.einsum("ij,wkj->wik", rc, g)
-            cg = np.einsum("wii->w", f)
+            f = np.einsum("ij,wkj->wik", rc, g, optimize=True)
+            cg = np.einsum("wii->w", f, optimize=True)
             c2 = c2 + cg * cg
-            exc = exc + np.einsum("wij,wji->w", f, f)
+            exc = exc + np.einsum("wij,wji->w", f, f, optimize=True)
     else:
         c2, exc = _exchange_restricted_high(g, ctx.rot_chol)
     return h0 + e1 + 2.0 * c2 - exc
@@ -263,13 +264,13 @@ def energy_uw(w: tuple, h0: float, ctx: RhfMeasCtx, c: np.ndarray) -> np.ndarray
     """trot/meas/rhf.py:156-172 with _two_body_energy_unrestricted (rhf.py:63-98)."""
     gu = half_green(w[0], c)
     gd = half_green(w[1], c)
-    e1 = np.einsum("wij,ij->w", gu + gd, ctx.rot_h1)
-    fu = np.einsum("gij,wkj->wgik", ctx.rot_chol, gu)
-    fd = np.einsum("gij,wkj->wgik", ctx.rot_chol, gd)
-    cu = np.einsum("wgii->wg", fu)
-    cd = np.einsum("wgii->wg", fd)
-    exu = np.einsum("wgij,wgji->w", fu, fu)
-    exd = np.einsum("wgij,wgji->w", fd, fd)
+    e1 = np.einsum("wij,ij->w", gu + gd, ctx.rot_h1, optimize=True)
+    fu = np.einsum("gij,wkj->wgik", ctx.rot_chol, gu, optimize=True)
+    fd = np.einsum("gij,wkj->wgik", ctx.rot_chol, gd, optimize=True)
+    cu = np.einsum("wgii->wg", fu, optimize=True)
+    cd = np.einsum("wgii->wg", fd, optimize=True)
+    exu = np.einsum("wgij,wgji->w", fu, fu, optimize=True)
+    exd = np.einsum("wgij,wgji->w", fd, fd, optimize=True)
     e2 = (np.sum(cu * cu, 1) + np.sum(cd * cd, 1) + 2.0 * np.sum(cu * cd, 1) - exu - exd) / 2.0
     return h0 + e1 + e2
 
@@ -278,13 +279,14 @@ def energy_uhf(w: tuple, h0: float, ctx: UhfMeasCtx, ca: np.ndarray, cb: np.ndar
     """trot/meas/uhf.py:205-231 (UHF trial, unrestricted walkers)."""
     gu = half_green(w[0], ca)
     gd = half_green(w[1], cb)
-    e1 = np.einsum("wij,ij->w", gu, ctx.rot_h1_a) + np.einsum("wij,ij->w", gd, ctx.rot_h1_b)
-    fu = np.einsum("gij,wkj->wgik", ctx.rot_chol_a, gu)
-    fd = np.einsum("gij,wkj->wgik", ctx.rot_chol_b, gd)
-    cu = np.einsum("wgii->wg", fu)
-    cd = np.einsum("wgii->wg", fd)
-    exu = np.einsum("wgij,wgji->w", fu, fu)
-    exd = np.einsum("wgij,wgji->w", fd, fd)
+    e1 = (np.einsum("wij,ij->w", gu, ctx.rot_h1_a, optimize=True)
+          + np.einsum("wij,ij->w", gd, ctx.rot_h1_b, optimize=True))
+    fu = np.einsum("gij,wkj->wgik", ctx.rot_chol_a, gu, optimize=True)
+    fd = np.einsum("gij,wkj->wgik", ctx.rot_chol_b, gd, optimize=True)
+    cu = np.einsum("wgii->wg", fu, optimize=True)
+    cd = np.einsum("wgii->wg", fd, optimize=True)
+    exu = np.einsum("wgij,wgji->w", fu, fu, optimize=True)
+    exd = np.einsum("wgij,wgji->w", fd, fd, optimize=True)
     e2 = (np.sum(cu * cu, 1) + np.sum(cd * cd, 1) + 2.0 * np.sum(cu * cd, 1) - exu - exd) / 2.0
     return h0 + e1 + e2
 
@@ -587,7 +589,7 @@ def energy_auto(w: np.ndarray, h0: float, h1: np.ndarray, chol: np.ndarray, c: n
     Here h1_eff = h1 - 1/2 sum_g L_g L_g (the one-body part of H in Cholesky form).
     Both derivatives by central differences.
     """
-    he = h1 - 0.5 * np.einsum("gik,gkj->ij", chol, chol)
+    he = h1 - 0.5 * np.einsum("gik,gkj->ij", chol, chol, optimize=True)
     ov0 = overlap_r(w, c)
     hw = he @ w
     d1 = (overlap_r(w + eps * hw, c) - overlap_r(w - eps * hw, c)) / (2 * eps)

@@ -1,0 +1,80 @@
+"""The C-ABI library: loads, exports every declared symbol, rejects bad input (no GPU needed).
+
+The error paths return before any HIP call, so they run on a CPU-only host.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "trot_amd.h"
+
+
+def _declared():
+    txt = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:const char\*|int|size_t)\s+(trot_\w+)\(", txt, re.M)))
+
+
+def test_header_declares_the_entry_points():
+    names = _declared()
+    for must in ("trot_force_bias", "trot_vhs_build", "trot_taylor_apply", "trot_one_body_apply",
+                 "trot_overlap_green", "trot_phaseless_update", "trot_local_energy", "trot_orthonormalize",
+                 "trot_sr_comb", "trot_apply_trotter", "trot_field_shift", "trot_block_energy"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from trot_amd import _lib
+
+    h = _lib.lib()
+    missing = [n for n in _declared() if not hasattr(h, n)]
+    assert not missing, missing
+    assert sorted(_lib.exported_symbols()) == _declared()
+    assert b"gfx950" in h.trot_version()
+
+
+def test_error_codes_without_gpu():
+    from trot_amd import _lib
+
+    h = _lib.lib()
+    # null pointers
+    assert h.trot_overlap_green(None, None, 4, 8, 2, None, None, None, 0, None) == -1
+    assert b"null" in h.trot_last_error()
+    # nocc above the kernels' limit
+    dummy = ctypes.c_void_p(16)
+    assert h.trot_overlap_green(dummy, dummy, 4, 100, 65, dummy, None, dummy, 1 << 30, None) == -2
+    # workspace smaller than the query
+    need = h.trot_overlap_green_workspace(4, 8, 2)
+    assert need > 0
+    assert h.trot_overlap_green(dummy, dummy, 4, 8, 2, dummy, None, dummy, need - 1, None) == -4
+    assert b"workspace" in h.trot_last_error()
+    assert h.trot_local_energy_workspace(8, 100) >= 8 * 4 * 16
+    assert h.trot_force_bias(None, None, 1, 1, 1, 2.0, 0, None, None) == -1
+    assert h.trot_sr_comb(dummy, 0.5, 0, dummy, dummy, dummy, 4096, None) == -1
+    assert h.trot_one_body_apply(dummy, dummy, 1, 4, 2, dummy, None) == -1  # in place is rejected
+    assert h.trot_orthonormalize(dummy, 2, 8, 70, dummy, 1 << 30, None) == -1  # n > N
+    assert h.trot_orthonormalize(dummy, 2, 100, 70, dummy, 1 << 30, None) == -2  # n > 64
+
+
+def test_workspace_queries_are_sizes():
+    from trot_amd import _lib
+
+    h = _lib.lib()
+    W, N, n = 16, 32, 4
+    assert h.trot_taylor_workspace(W, N, n) >= 2 * W * N * n * 16
+    assert h.trot_orthonormalize_workspace(W, N, n) >= W * N * n * 16 + 2 * W * n * n * 16
+    assert h.trot_apply_trotter_workspace(W, N, n, 4) < h.trot_apply_trotter_workspace(W, N, n, 0)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from trot_amd import _lib
+
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "absent.so")
+    with pytest.raises(ImportError):
+        _lib.lib()

@@ -197,12 +197,13 @@ def one_body_apply(expm: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
     return out
 
 
-def taylor_apply(vhs: torch.Tensor, w: torch.Tensor, sqrt_dt: float, n_terms: int) -> torch.Tensor:
-    """Returns a new tensor: sum_k (i sqrt(dt) V)^k/k! w."""
+def taylor_apply(vhs: torch.Tensor, w: torch.Tensor, sqrt_dt: float, n_terms: int,
+                 inplace: bool = False) -> torch.Tensor:
+    """sum_k (i sqrt(dt) V)^k/k! w; a new tensor unless ``inplace``."""
     W, N, n = w.shape
     _req(w, "walkers", C128, (W, N, n))
     _req(vhs, "vhs", C128, (W, N, N))
-    out = w.clone()
+    out = w if inplace else w.clone()
     L = lib()
     ws = _ws(L.trot_taylor_workspace(W, N, n), w.device)
     _check(L.trot_taylor_apply(_p(vhs), _p(out), W, N, n, float(sqrt_dt), int(n_terms), _p(ws), ws.numel(),

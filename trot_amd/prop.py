@@ -226,15 +226,61 @@ def init_prop_state(*, sys: System, ham_data, trial_ops: TrialOps, trial_data, m
 # ---------------------------------------------------------------------------
 
 
+class KernelTimer:
+    """Collects (start, end) torch.cuda events around named launches on the current stream."""
+
+    def __init__(self):
+        self.events: dict[str, list] = {}
+
+    def wrap(self, name: str, fn, *args):
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        out = fn(*args)
+        e.record()
+        self.events.setdefault(name, []).append((s, e))
+        return out
+
+    def mean_ms(self, name: str) -> float:
+        ev = self.events.get(name, [])
+        return float(np.mean([s.elapsed_time(e) for s, e in ev])) if ev else float("nan")
+
+    def reset(self):
+        self.events.clear()
+
+
+def _vhs(ctx: CholAfqmcCtx, xbar: torch.Tensor) -> torch.Tensor:
+    timer = ctx.low.get("timer")
+    if timer is not None:
+        return timer.wrap("vhs", _lib.vhs_build, xbar, ctx.chol_flat)
+    return _lib.vhs_build(xbar, ctx.chol_flat)
+
+
 def apply_trotter(walkers, xbar: torch.Tensor, ctx: CholAfqmcCtx, n_terms: int):
-    """trot/prop/chol_afqmc_ops.py:198-222: E T_K(i sqrt(dt) V(xbar)) E w, V shared by both spins."""
-    if isinstance(walkers, tuple):
-        a, b = _lib.apply_trotter(walkers[0], walkers[1], xbar, ctx.chol_flat, ctx.exp_h1_half, ctx.sqrt_dt,
-                                  n_terms, ctx.walker_chunk)
-        return (a, b)
-    a, _ = _lib.apply_trotter(walkers, None, xbar, ctx.chol_flat, ctx.exp_h1_half, ctx.sqrt_dt, n_terms,
-                              ctx.walker_chunk)
-    return a
+    """trot/prop/chol_afqmc_ops.py:198-222: E T_K(i sqrt(dt) V(xbar)) E w, V shared by both spins.
+
+    One VHS build, then per spin: one-body half step, Taylor series, one-body
+    half step (chol_afqmc_ops.py:130-131, 152-162, utils.py:20-29), in walker
+    chunks of ``ctx.walker_chunk`` (0: all walkers at once).
+    """
+    spins = walkers if isinstance(walkers, tuple) else (walkers,)
+    W = spins[0].shape[0]
+    N = ctx.norb
+    chunk = ctx.walker_chunk if 0 < ctx.walker_chunk < W else W
+    outs = [None] * len(spins) if chunk == W else [torch.empty_like(s) for s in spins]
+    for w0 in range(0, W, chunk):
+        w1 = min(W, w0 + chunk)
+        v = _vhs(ctx, xbar[w0:w1]).view(w1 - w0, N, N)
+        for i, s in enumerate(spins):
+            t = _lib.one_body_apply(ctx.exp_h1_half, s[w0:w1])
+            t = _lib.taylor_apply(v, t, ctx.sqrt_dt, n_terms, inplace=True)
+            r = _lib.one_body_apply(ctx.exp_h1_half, t)
+            if chunk == W:
+                outs[i] = r
+            else:
+                outs[i][w0:w1] = r
+        del v
+    return tuple(outs) if isinstance(walkers, tuple) else outs[0]
 
 
 def afqmc_step(state: PropState, *, params, ham_data, trial_data, meas_ops: MeasOps, prop_ctx: CholAfqmcCtx,

@@ -35,6 +35,11 @@ extern "C" {
 #define TROT_ERR_HIP (-3)         /* HIP runtime error (launch or configuration) */
 #define TROT_ERR_WORKSPACE (-4)   /* workspace smaller than the *_workspace() query */
 
+/* precision of the Cholesky-side MFMA GEMMs in the *_lp entry points */
+#define TROT_PREC_F64 0    /* v_mfma_f64_16x16x4_f64 (the plain entry points) */
+#define TROT_PREC_F32 1    /* v_mfma_f32_16x16x4_f32, FP32 accumulation */
+#define TROT_PREC_BF16X3 2 /* 3-pass split BF16 (hi.hi + hi.lo + lo.hi) on v_mfma_f32_16x16x32_bf16 */
+
 const char* trot_version(void);
 const char* trot_last_error(void);
 
@@ -134,6 +139,50 @@ int trot_build_rot_chol(const double* chol, const double* trial, int G, int N, i
 size_t trot_build_prop_workspace(int G, int N);
 int trot_build_prop_tensors(const double* chol, const double* dm, int G, int N, double* mf, double* mfl,
                             double* v0m, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- symmetric Cholesky vectors (every L_g == L_g^T, as for real-orbital ERIs) ----
+ * V = sum_g xbar_g L_g is then symmetric.  "sym-block" storage keeps only the 8 x 8 blocks (bi <= bj) of a
+ * symmetric N x N matrix, nb = ceil(N/8) blocks per side, block-row major, each block row-major:
+ * trot_symblk_cols(N) = 64 nb (nb + 1) / 2 numbers per matrix.  trot_vhs_build(_lp) with chol_flat in this
+ * storage (N2 = trot_symblk_cols(N)) produces V in this storage at ~half the flops and bytes, and the
+ * *_symblk Taylor entry points read it.  Same result as the dense path up to summation order.
+ * Replaces: _make_vhs_split_flat + taylor_expm_action (trot/prop/chol_afqmc_ops.py:83-87, 152-162;
+ * trot/prop/utils.py:20-29) for symmetric chol. */
+long trot_symblk_cols(int N);
+int trot_pack_symblk(const double* chol, int G, int N, void* out, int out_f32, void* stream);
+int trot_taylor_apply_symblk(const double* vhs_symblk, double* w, int W, int N, int n, double sqrt_dt, int n_terms,
+                             void* ws, size_t ws_bytes, void* stream);
+int trot_taylor_apply_lp_symblk(const float* vhs_symblk, double* w, int W, int N, int n, double sqrt_dt,
+                                int n_terms, int prec, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- reduced-precision forms (prec = TROT_PREC_F32 or TROT_PREC_BF16X3) ----
+ * Cholesky-side operands are FP32 copies made at context build; FP64 walker-side operands are rounded
+ * (and split) as they are staged into LDS; accumulation is FP32 on MFMA.  The reference's own mixed mode
+ * (trot/prop/chol_afqmc_ops.py:247-257; runtime_layout.py:167) casts chol_flat to float32 and xbar to
+ * complex64 for the VHS GEMM. */
+
+/* As trot_vhs_build with chol_flat FP32; vhs is complex64 (vhs_f64 = 0) or complex128 (vhs_f64 = 1).
+ * Replaces: _make_vhs_split_flat in mixed precision (trot/prop/chol_afqmc_ops.py:83-87, 247-257). */
+int trot_vhs_build_lp(const double* xbar, const float* chol_flat, int W, int G, long N2, void* vhs, int vhs_f64,
+                      int prec, void* stream);
+
+/* As trot_force_bias with R' FP32; fb stays complex128.
+ * Replaces: force_bias_kernel_rw_rh / _uw_rh (trot/meas/rhf.py:119-153; trot/meas/uhf.py:43-57). */
+int trot_force_bias_lp(const double* green_t, const float* rot_chol_t, int W, int G, long Nn, double alpha,
+                       int accumulate, double* fb, int prec, void* stream);
+
+/* As trot_taylor_apply with V complex64 and complex64 terms; the running sum w stays complex128.
+ * Replaces: taylor_expm_action (trot/prop/utils.py:20-29). */
+size_t trot_taylor_lp_workspace(int W, int N, int n);
+int trot_taylor_apply_lp(const float* vhs, double* w, int W, int N, int n, double sqrt_dt, int n_terms, int prec,
+                         void* ws, size_t ws_bytes, void* stream);
+
+/* As trot_local_energy with rot_chol FP32 (exchange GEMM on MFMA FP32 / split BF16; sums in FP64).
+ * Replaces: trot/meas/rhf.py:41-172; trot/meas/uhf.py:205-231. */
+int trot_local_energy_lp(const float* rot_chol_a, const float* rot_chol_b, const double* green_t_a,
+                         const double* green_t_b, const double* rot_h1t_a, const double* rot_h1t_b, int W, int N,
+                         int na, int nb, int G, double h0, double* energy, int prec, void* ws, size_t ws_bytes,
+                         void* stream);
 
 #ifdef __cplusplus
 }

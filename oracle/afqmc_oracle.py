@@ -298,8 +298,9 @@ def energy_uhf(w: tuple, h0: float, ctx: UhfMeasCtx, ca: np.ndarray, cb: np.ndar
 
 def make_vhs(xbar: np.ndarray, chol_flat: np.ndarray, norb: int) -> np.ndarray:
     """trot/prop/chol_afqmc_ops.py:83-87: V = (Re x @ Lflat + i Im x @ Lflat).reshape(N, N)."""
-    v_re = np.real(xbar) @ chol_flat
-    v_im = np.imag(xbar) @ chol_flat
+    # contiguous copies keep numpy on its BLAS path (a strided view of a complex array is not)
+    v_re = np.ascontiguousarray(np.real(xbar)) @ chol_flat
+    v_im = np.ascontiguousarray(np.imag(xbar)) @ chol_flat
     return (v_re + 1.0j * v_im).reshape(xbar.shape[0], norb, norb)
 
 

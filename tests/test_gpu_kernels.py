@@ -253,3 +253,41 @@ def test_bad_sizes_raise(gpu):
         _lib.overlap_green(c, w)
     with pytest.raises(ValueError):
         _lib.overlap_green(c[:, :3].contiguous(), w)
+
+
+@pytest.mark.parametrize("N", [64, 37, 9])
+def test_symblk_vhs_and_taylor(gpu, N):
+    """Sym-block V (upper 8x8 blocks only) gives the dense VHS / Taylor results."""
+    from trot_amd import _lib
+
+    n, G, W = 5, 21, 6
+    p = problem(N, n, G, W, seed=41)
+    rng = np.random.default_rng(2)
+    xbar = rng.standard_normal((W, G)) + 1j * rng.standard_normal((W, G))
+    chol_flat = p["chol"].reshape(G, -1)
+    blk = _lib.pack_symblk(_t(p["chol"]))
+    nb = (N + 7) // 8
+    assert blk.shape == (G, 64 * nb * (nb + 1) // 2)
+    vpk = _lib.vhs_build(_t(xbar), blk)
+    v_ref = O.make_vhs(xbar, chol_flat, N)
+    # unpack on the host and compare with the dense VHS
+    vp = vpk.cpu().numpy()
+    dense = np.zeros((W, nb * 8, nb * 8), dtype=complex)
+    pidx = 0
+    for bi in range(nb):
+        for bj in range(bi, nb):
+            b = vp[:, pidx * 64:(pidx + 1) * 64].reshape(W, 8, 8)
+            dense[:, bi * 8:bi * 8 + 8, bj * 8:bj * 8 + 8] = b
+            dense[:, bj * 8:bj * 8 + 8, bi * 8:bi * 8 + 8] = np.swapaxes(b, 1, 2)
+            pidx += 1
+    assert rel(dense[:, :N, :N], v_ref) < TOL
+    assert np.all(dense[:, N:, :] == 0) and np.all(dense[:, :, N:] == 0)
+    sdt = 0.1
+    ref = O.taylor_expm_action(1j * sdt, v_ref, p["wa"], 6)
+    out = _lib.taylor_apply_symblk(vpk, _t(p["wa"]), sdt, 6)
+    assert rel(out.cpu().numpy(), ref) < TOL
+    for prec, tol in (("f32", 2e-6), ("bf16x3", 2e-4)):
+        blk32 = _lib.pack_symblk(_t(p["chol"]), out_f32=True)
+        v32 = _lib.vhs_build_lp(_t(xbar), blk32, prec)
+        out = _lib.taylor_apply_lp_symblk(v32, _t(p["wa"]), sdt, 6, prec)
+        assert rel(out.cpu().numpy() - p["wa"], ref - p["wa"]) < 4 * tol

@@ -1,0 +1,116 @@
+"""Reduced-precision forms (FP32 and 3-pass split-BF16 MFMA) against the FP64 oracle.
+
+Tolerances (max abs error / max abs value): FP32 GEMMs 2e-6; split-BF16
+2e-4.  A single-pass BF16 product would miss these by about 10x (relative
+error ~ 2^-8), so the BF16 bound also checks that all three passes are used.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import afqmc_oracle as O
+from tests.cases import problem, rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f32": 2e-6, "bf16x3": 2e-4}
+
+
+def _t(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+@pytest.mark.parametrize("shape", [(64, 16, 96, 64), (37, 6, 70, 5), (24, 13, 33, 130)])
+def test_vhs_and_taylor_lp(gpu, prec, shape):
+    from trot_amd import _lib
+
+    N, n, G, W = shape
+    p = problem(N, n, G, W, seed=31)
+    rng = np.random.default_rng(1)
+    xbar = rng.standard_normal((W, G)) + 1j * rng.standard_normal((W, G))
+    chol_flat = p["chol"].reshape(G, -1)
+    v_ref = O.make_vhs(xbar, chol_flat, N)
+    c32 = _t(chol_flat).float()
+    v = _lib.vhs_build_lp(_t(xbar), c32, prec).reshape(W, N, N)
+    assert v.dtype == torch.complex64
+    assert rel(v.cpu().numpy(), v_ref) < TOL[prec]
+    v64 = _lib.vhs_build_lp(_t(xbar), c32, prec, out_f64=True).reshape(W, N, N)
+    assert v64.dtype == torch.complex128 and rel(v64.cpu().numpy(), v_ref) < TOL[prec]
+    sdt = 0.1
+    ref = O.taylor_expm_action(1j * sdt, v_ref, p["wa"], 6)
+    out = _lib.taylor_apply_lp(v.contiguous(), _t(p["wa"]), sdt, 6, prec)
+    # the identity term is exact; compare the propagated change against its own size
+    d_ref = ref - p["wa"]
+    assert rel(out.cpu().numpy() - p["wa"], d_ref) < 4 * TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+@pytest.mark.parametrize("kind", ["rr", "ru", "uu"])
+def test_force_bias_and_energy_lp(gpu, prec, kind):
+    from trot_amd import meas, trial
+    from trot_amd.core import HamChol, System
+
+    N, n, G, W = 40, 8, 50, 12
+    nb = 7 if kind == "uu" else n
+    p = problem(N, n, G, W, seed=32, nocc_b=nb)
+    ham = HamChol(p["h0"], p["h1"], p["chol"]).to_device()
+    if kind == "uu":
+        sys_ = System(N, (n, nb), "unrestricted")
+        tr = trial.UhfTrial(p["ca"], p["cb"])
+        ops = meas.make_uhf_meas_ops(sys_, precision=prec)
+        ref = O.make_uhf_ops(p["h0"], p["h1"], p["chol"], p["ca"], p["cb"])
+        w, wn = (_t(p["wa"]), _t(p["wb"])), (p["wa"], p["wb"])
+    elif kind == "ru":
+        sys_ = System(N, (n, n), "unrestricted")
+        tr = trial.RhfTrial(p["ca"])
+        ops = meas.make_rhf_meas_ops(sys_, precision=prec)
+        ref = O.make_rhf_ops(p["h0"], p["h1"], p["chol"], p["ca"], "unrestricted")
+        w, wn = (_t(p["wa"]), _t(p["wb"])), (p["wa"], p["wb"])
+    else:
+        sys_ = System(N, (n, n), "restricted")
+        tr = trial.RhfTrial(p["ca"])
+        ops = meas.make_rhf_meas_ops(sys_, precision=prec)
+        ref = O.make_rhf_ops(p["h0"], p["h1"], p["chol"], p["ca"])
+        w, wn = _t(p["wa"]), p["wa"]
+    ctx = ops.build_meas_ctx(ham, tr)
+    fb = ops.require_kernel("force_bias")(w, ham, ctx, tr).cpu().numpy()
+    e = ops.require_kernel("energy")(w, ham, ctx, tr).cpu().numpy()
+    assert rel(fb, ref.force_bias(wn)) < TOL[prec]
+    e_ref = ref.energy(wn)
+    assert rel(e - p["h0"], e_ref - p["h0"]) < 4 * TOL[prec]
+
+
+@pytest.mark.parametrize("prec,tprec", [("f32", "f32"), ("bf16x3", "f32"), ("bf16x3", "bf16x3"), ("f32", "f64")])
+def test_mixed_steps_track_fp64(gpu, prec, tprec):
+    """A few mixed-precision steps stay close to the FP64 oracle with identical fields."""
+    from trot_amd import meas, prop, trial
+    from trot_amd.core import HamChol, System
+
+    N, n, G, W = 48, 8, 60, 32
+    p = problem(N, n, G, W, seed=33)
+    sys_ = System(N, (n, n), "restricted")
+    ham = HamChol(p["h0"], p["h1"], p["chol"]).to_device()
+    tr = trial.RhfTrial(p["ca"])
+    mops = meas.make_rhf_meas_ops(sys_, precision=prec)
+    tops = trial.make_rhf_trial_ops(sys_)
+    pops = prop.make_prop_ops("restricted", "restricted", precision=prec, taylor_precision=tprec)
+    params = prop.QmcParams(dt=0.01, n_walkers=W, seed=1)
+    pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
+    mctx = mops.build_meas_ctx(ham, tr)
+    st = pops.init_prop_state(sys=sys_, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops, params=params,
+                              initial_walkers=_t(p["wa"]))
+    ref_ops = O.make_rhf_ops(p["h0"], p["h1"], p["chol"], p["ca"])
+    ref_ctx = O.build_prop_ctx(p["h0"], p["h1"], p["chol"], O.rhf_rdm1(p["ca"]), 0.01)
+    ref = O.init_prop_state(p["wa"], ref_ops, W)
+    for s in range(4):
+        f = p["fields"][s]
+        st = pops.step(st, params=params, ham_data=ham, trial_data=tr, trial_ops=tops, meas_ops=mops, meas_ctx=mctx,
+                       prop_ctx=pctx, fields=_t(f))
+        ref = O.afqmc_step(ref, f, ref_ctx, ref_ops, O.QmcParams(dt=0.01, n_walkers=W))
+    tol = 50 * TOL[prec if tprec != "bf16x3" else "bf16x3"]
+    assert rel(st.walkers.cpu().numpy(), ref.walkers) < tol
+    assert rel(st.weights.cpu().numpy(), ref.weights) < tol

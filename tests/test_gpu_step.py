@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def _setup(p, kind: str, dt: float, uhf: bool = False):
+def _setup(p, kind: str, dt: float, uhf: bool = False, layout: str = "auto"):
     from trot_amd import meas, prop, trial
     from trot_amd.core import HamChol, System
 
@@ -40,7 +40,7 @@ def _setup(p, kind: str, dt: float, uhf: bool = False):
         tops = trial.make_rhf_trial_ops(sys)
         ref_ops = O.make_rhf_ops(p["h0"], p["h1"], p["chol"], p["ca"], kind)
         rdm1 = O.rhf_rdm1(p["ca"])
-    pops = prop.make_prop_ops("restricted", kind)
+    pops = prop.make_prop_ops("restricted", kind, vhs_layout=layout)
     params = prop.QmcParams(dt=dt, n_walkers=p["wa"].shape[0], seed=3, n_prop_steps=4)
     pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
     mctx = mops.build_meas_ctx(ham, tr)
@@ -66,16 +66,23 @@ def _cmp_walkers(wg, wr):
     return rel(wg.cpu().numpy(), wr)
 
 
+@pytest.mark.parametrize("layout", ["auto", "dense"])
 @pytest.mark.parametrize("case", [
     ("restricted", (24, 6, 40, 16), False),
     ("restricted", (37, 5, 23, 70), False),
     ("unrestricted", (30, 5, 28, 9), False),
     ("unrestricted", (33, 7, 30, 11), True),
+    ("nonsym", (21, 4, 13, 10), False),
 ])
-def test_steps_match_oracle(gpu, case):
+def test_steps_match_oracle(gpu, case, layout):
     kind, (N, n, G, W), uhf = case
     p = problem(N, n, G, W, seed=21, nocc_b=(n - 1 if uhf else (n if kind == "unrestricted" else None)))
-    sys, ham, tr, mops, tops, pops, params, pctx, mctx, ref_ops, ref_ctx = _setup(p, kind, 0.01, uhf)
+    if kind == "nonsym":  # non-symmetric L_g (trot tests/test_prop_afqmc_step.py:85-162): dense VHS path
+        kind = "restricted"
+        p["chol"] = p["chol"] + 0.01 * np.random.default_rng(5).standard_normal(p["chol"].shape)
+    sys, ham, tr, mops, tops, pops, params, pctx, mctx, ref_ops, ref_ctx = _setup(p, kind, 0.01, uhf, layout)
+    assert ("chol_symblk" in pctx.low) == (layout == "auto" and bool(np.array_equal(
+        p["chol"], np.swapaxes(p["chol"], 1, 2))))
     w0 = _walkers(p, kind)
     state = pops.init_prop_state(sys=sys, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops,
                                  params=params, initial_walkers=_to_dev(w0))

@@ -17,7 +17,7 @@ HEADER = ROOT / "include" / "trot_amd.h"
 
 def _declared():
     txt = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:const char\*|int|size_t)\s+(trot_\w+)\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const char\*|int|long|size_t)\s+(trot_\w+)\(", txt, re.M)))
 
 
 def test_header_declares_the_entry_points():

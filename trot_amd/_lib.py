@@ -57,7 +57,20 @@ _SIGS = {
     "trot_build_rot_chol": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
     "trot_build_prop_workspace": (_sz, [_i, _i]),
     "trot_build_prop_tensors": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "trot_vhs_build_lp": (_i, [_vp, _vp, _i, _i, _l, _vp, _i, _i, _vp]),
+    "trot_force_bias_lp": (_i, [_vp, _vp, _i, _i, _l, _d, _i, _vp, _i, _vp]),
+    "trot_taylor_lp_workspace": (_sz, [_i, _i, _i]),
+    "trot_taylor_apply_lp": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _i, _vp, _sz, _vp]),
+    "trot_local_energy_lp": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _d, _vp, _i, _vp, _sz, _vp]),
+    "trot_symblk_cols": (_l, [_i]),
+    "trot_pack_symblk": (_i, [_vp, _i, _i, _vp, _i, _vp]),
+    "trot_taylor_apply_symblk": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
+    "trot_taylor_apply_lp_symblk": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _i, _vp, _sz, _vp]),
 }
+
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
+F32 = torch.float32
+C64 = torch.complex64
 
 
 def lib():
@@ -345,3 +358,124 @@ def build_prop_tensors(chol: torch.Tensor, dm: torch.Tensor):
     _check(L.trot_build_prop_tensors(_p(chol), _p(dm), G, N, _p(mf), _p(mfl), _p(v0m), _p(ws), ws.numel(), _stream()),
            "trot_build_prop_tensors")
     return mf, mfl, v0m
+
+
+# ---------------------------------------------------------------------------
+# reduced-precision forms (FP32 / split-BF16 MFMA); FP32 Cholesky-side operands
+# ---------------------------------------------------------------------------
+
+
+def _prec(p: str) -> int:
+    if p not in ("f32", "bf16x3"):
+        raise ValueError(f"reduced precision must be 'f32' or 'bf16x3', got {p!r}")
+    return PREC[p]
+
+
+def vhs_build_lp(xbar: torch.Tensor, chol_flat32: torch.Tensor, prec: str, out_f64: bool = False) -> torch.Tensor:
+    W, G = xbar.shape
+    N2 = chol_flat32.shape[1]
+    _req(xbar, "xbar", C128, (W, G))
+    _req(chol_flat32, "chol_flat (f32)", F32, (G, N2))
+    out = torch.empty((W, N2), dtype=C128 if out_f64 else C64, device=xbar.device)
+    _check(lib().trot_vhs_build_lp(_p(xbar), _p(chol_flat32), W, G, N2, _p(out), int(out_f64), _prec(prec),
+                                   _stream()), "trot_vhs_build_lp")
+    return out
+
+
+def force_bias_lp(green_t: torch.Tensor, rot_chol_t32: torch.Tensor, alpha: float, prec: str,
+                  out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
+    W, N, n = green_t.shape
+    G = rot_chol_t32.shape[0]
+    _req(green_t, "green_t", C128, (W, N, n))
+    _req(rot_chol_t32, "rot_chol_t (f32)", F32, (G, N, n))
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs an output tensor")
+        out = torch.empty((W, G), dtype=C128, device=green_t.device)
+    _req(out, "fb", C128, (W, G))
+    _check(lib().trot_force_bias_lp(_p(green_t), _p(rot_chol_t32), W, G, N * n, float(alpha), int(accumulate),
+                                    _p(out), _prec(prec), _stream()), "trot_force_bias_lp")
+    return out
+
+
+def taylor_apply_lp(vhs32: torch.Tensor, w: torch.Tensor, sqrt_dt: float, n_terms: int, prec: str,
+                    inplace: bool = False) -> torch.Tensor:
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    _req(vhs32, "vhs (complex64)", C64, (W, N, N))
+    out = w if inplace else w.clone()
+    L = lib()
+    ws = _ws(L.trot_taylor_lp_workspace(W, N, n), w.device)
+    _check(L.trot_taylor_apply_lp(_p(vhs32), _p(out), W, N, n, float(sqrt_dt), int(n_terms), _prec(prec), _p(ws),
+                                  ws.numel(), _stream()), "trot_taylor_apply_lp")
+    return out
+
+
+def local_energy_lp(rot_chol_a32, green_t_a, rot_h1t_a, h0: float, prec: str, rot_chol_b32=None, green_t_b=None,
+                    rot_h1t_b=None):
+    W, N, na = green_t_a.shape
+    G = rot_chol_a32.shape[0]
+    _req(green_t_a, "green_t_a", C128, (W, N, na))
+    _req(rot_chol_a32, "rot_chol_a (f32)", F32, (G, na, N))
+    _req(rot_h1t_a, "rot_h1t_a", F64, (N, na))
+    nb = 0
+    if green_t_b is not None:
+        nb = green_t_b.shape[2]
+        _req(green_t_b, "green_t_b", C128, (W, N, nb))
+        _req(rot_chol_b32, "rot_chol_b (f32)", F32, (G, nb, N))
+        _req(rot_h1t_b, "rot_h1t_b", F64, (N, nb))
+    if max(na, nb) > MAX_NOCC:
+        raise ValueError(f"nocc > {MAX_NOCC} is not supported by the GPU energy kernel")
+    L = lib()
+    out = torch.empty((W,), dtype=C128, device=green_t_a.device)
+    ws = _ws(L.trot_local_energy_workspace(W, G), green_t_a.device)
+    _check(L.trot_local_energy_lp(_p(rot_chol_a32), _p(rot_chol_b32), _p(green_t_a), _p(green_t_b), _p(rot_h1t_a),
+                                  _p(rot_h1t_b), W, N, na, nb, G, float(h0), _p(out), _prec(prec), _p(ws),
+                                  ws.numel(), _stream()), "trot_local_energy_lp")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# symmetric Cholesky vectors: sym-block storage of L_g and V
+# ---------------------------------------------------------------------------
+
+
+def symblk_cols(N: int) -> int:
+    """Numbers per N x N symmetric matrix in sym-block storage (upper 8x8 blocks)."""
+    return int(lib().trot_symblk_cols(int(N)))
+
+
+def pack_symblk(chol: torch.Tensor, out_f32: bool = False) -> torch.Tensor:
+    G, N, _ = chol.shape
+    _req(chol, "chol", F64, (G, N, N))
+    if G > 65535:
+        raise ValueError("nchol > 65535 is not supported")
+    out = torch.empty((G, symblk_cols(N)), dtype=F32 if out_f32 else F64, device=chol.device)
+    _check(lib().trot_pack_symblk(_p(chol), G, N, _p(out), int(out_f32), _stream()), "trot_pack_symblk")
+    return out
+
+
+def taylor_apply_symblk(vpk: torch.Tensor, w: torch.Tensor, sqrt_dt: float, n_terms: int,
+                        inplace: bool = False) -> torch.Tensor:
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    _req(vpk, "vhs (sym-block)", C128, (W, symblk_cols(N)))
+    out = w if inplace else w.clone()
+    L = lib()
+    ws = _ws(L.trot_taylor_workspace(W, N, n), w.device)
+    _check(L.trot_taylor_apply_symblk(_p(vpk), _p(out), W, N, n, float(sqrt_dt), int(n_terms), _p(ws), ws.numel(),
+                                      _stream()), "trot_taylor_apply_symblk")
+    return out
+
+
+def taylor_apply_lp_symblk(vpk32: torch.Tensor, w: torch.Tensor, sqrt_dt: float, n_terms: int, prec: str,
+                           inplace: bool = False) -> torch.Tensor:
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    _req(vpk32, "vhs (sym-block, complex64)", C64, (W, symblk_cols(N)))
+    out = w if inplace else w.clone()
+    L = lib()
+    ws = _ws(L.trot_taylor_lp_workspace(W, N, n), w.device)
+    _check(L.trot_taylor_apply_lp_symblk(_p(vpk32), _p(out), W, N, n, float(sqrt_dt), int(n_terms), _prec(prec),
+                                         _p(ws), ws.numel(), _stream()), "trot_taylor_apply_lp_symblk")
+    return out

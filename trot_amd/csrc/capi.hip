@@ -15,45 +15,14 @@
 #include "common.hpp"
 #include "gemm.hpp"
 #include "kernels.hpp"
-#include "../../include/trot_amd.h"
+#include "capi_util.hpp"
 
-static thread_local char g_err[512];
-
-static int fail(int code, const char* fmt, const char* a = "", long b = 0) {
-  snprintf(g_err, sizeof(g_err), fmt, a, b);
-  return code;
-}
-static int hip_status(hipError_t e, const char* where) {
-  if (e == hipSuccess) return TROT_OK;
-  snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
-  return TROT_ERR_HIP;
-}
-#define HIPCHK(expr)                                    \
-  do {                                                  \
-    hipError_t _e = (expr);                             \
-    if (_e != hipSuccess) return hip_status(_e, #expr); \
-  } while (0)
-
-static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-// bump allocator over the caller's workspace
-struct Carve {
-  char* base;
-  size_t cap, used;
-  template <typename T>
-  T* take(size_t count) {
-    size_t bytes = align256(count * sizeof(T));
-    if (used + bytes > cap) return nullptr;
-    T* p = reinterpret_cast<T*>(base + used);
-    used += bytes;
-    return p;
-  }
-};
+thread_local char g_trot_err[512];
 
 extern "C" {
 
 const char* trot_version(void) { return "trot_amd 0.1 (gfx950, FP64 MFMA)"; }
-const char* trot_last_error(void) { return g_err; }
+const char* trot_last_error(void) { return g_trot_err; }
 
 // ---------------------------------------------------------------------------
 // overlap + half Green's function
@@ -65,13 +34,13 @@ size_t trot_overlap_green_workspace(int W, int N, int n) {
 
 int trot_overlap_green(const double* trial, const double* walkers, int W, int N, int n, double* det,
                        double* green_t, void* ws, size_t ws_bytes, void* stream) {
-  if (!trial || !walkers || !det) return fail(TROT_ERR_INVALID, "overlap_green: null pointer%s%ld");
-  if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "overlap_green: bad sizes%s%ld");
-  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "overlap_green: nocc > 64 unsupported%s%ld", "", n);
+  if (!trial || !walkers || !det) return fail(TROT_ERR_INVALID, "overlap_green: null pointer");
+  if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "overlap_green: bad sizes");
+  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "overlap_green: nocc > 64 unsupported");
   Carve c{(char*)ws, ws_bytes, 0};
   double* m = c.take<double>((size_t)W * n * n * 2);
   double* minv = c.take<double>((size_t)W * n * n * 2);
-  if (!m || !minv) return fail(TROT_ERR_WORKSPACE, "overlap_green: workspace too small%s%ld");
+  if (!m || !minv) return fail(TROT_ERR_WORKSPACE, "overlap_green: workspace too small");
   hipStream_t s = (hipStream_t)stream;
   // m[b] = C^T w[b]  (n x n)
   GemmP p{};
@@ -94,7 +63,7 @@ int trot_overlap_green(const double* trial, const double* walkers, int W, int N,
 }
 
 int trot_overlap_combine(const double* det_a, const double* det_b, double* overlap, int W, void* stream) {
-  if (!det_a || !overlap || W < 0) return fail(TROT_ERR_INVALID, "overlap_combine: bad arguments%s%ld");
+  if (!det_a || !overlap || W < 0) return fail(TROT_ERR_INVALID, "overlap_combine: bad arguments");
   HIPCHK(launch_overlap_combine(det_a, det_b, overlap, W, (hipStream_t)stream));
   return TROT_OK;
 }
@@ -104,8 +73,8 @@ int trot_overlap_combine(const double* det_a, const double* det_b, double* overl
 // ---------------------------------------------------------------------------
 int trot_force_bias(const double* green_t, const double* rot_chol_t, int W, int G, long Nn, double alpha,
                     int accumulate, double* fb, void* stream) {
-  if (!green_t || !rot_chol_t || !fb) return fail(TROT_ERR_INVALID, "force_bias: null pointer%s%ld");
-  if (W < 1 || G < 1 || Nn < 1) return fail(TROT_ERR_INVALID, "force_bias: bad sizes%s%ld");
+  if (!green_t || !rot_chol_t || !fb) return fail(TROT_ERR_INVALID, "force_bias: null pointer");
+  if (W < 1 || G < 1 || Nn < 1) return fail(TROT_ERR_INVALID, "force_bias: bad sizes");
   GemmP p{};
   p.A = green_t; p.lda = Nn; p.sA = 0;
   p.B = rot_chol_t; p.ldb = Nn; p.sB = 0;
@@ -121,8 +90,8 @@ int trot_force_bias(const double* green_t, const double* rot_chol_t, int W, int 
 int trot_field_shift(const double* fields, const double* fb, const double* mf, double sqrt_dt, int W, int G,
                      double* xbar, double* shift_term, double* fb_term, void* stream) {
   if (!fields || !fb || !mf || !xbar || !shift_term || !fb_term)
-    return fail(TROT_ERR_INVALID, "field_shift: null pointer%s%ld");
-  if (W < 1 || G < 1) return fail(TROT_ERR_INVALID, "field_shift: bad sizes%s%ld");
+    return fail(TROT_ERR_INVALID, "field_shift: null pointer");
+  if (W < 1 || G < 1) return fail(TROT_ERR_INVALID, "field_shift: bad sizes");
   HIPCHK(launch_shift(fields, fb, mf, sqrt_dt, W, G, xbar, nullptr, shift_term, fb_term, (hipStream_t)stream));
   return TROT_OK;
 }
@@ -131,8 +100,8 @@ int trot_field_shift(const double* fields, const double* fb, const double* mf, d
 // VHS: V (W x N^2 complex) = xbar (W x G complex) . Lflat (G x N^2 real)
 // ---------------------------------------------------------------------------
 int trot_vhs_build(const double* xbar, const double* chol_flat, int W, int G, long N2, double* vhs, void* stream) {
-  if (!xbar || !chol_flat || !vhs) return fail(TROT_ERR_INVALID, "vhs_build: null pointer%s%ld");
-  if (W < 1 || G < 1 || N2 < 1) return fail(TROT_ERR_INVALID, "vhs_build: bad sizes%s%ld");
+  if (!xbar || !chol_flat || !vhs) return fail(TROT_ERR_INVALID, "vhs_build: null pointer");
+  if (W < 1 || G < 1 || N2 < 1) return fail(TROT_ERR_INVALID, "vhs_build: bad sizes");
   GemmP p{};
   p.A = xbar; p.lda = G;
   p.B = chol_flat; p.ldb = N2;
@@ -146,9 +115,9 @@ int trot_vhs_build(const double* xbar, const double* chol_flat, int W, int G, lo
 // one-body half step: w_out[b] = E (N x N real) . w_in[b]
 // ---------------------------------------------------------------------------
 int trot_one_body_apply(const double* expm, const double* w_in, int W, int N, int n, double* w_out, void* stream) {
-  if (!expm || !w_in || !w_out) return fail(TROT_ERR_INVALID, "one_body_apply: null pointer%s%ld");
-  if (w_in == w_out) return fail(TROT_ERR_INVALID, "one_body_apply: in-place not supported%s%ld");
-  if (W < 1 || N < 1 || n < 1) return fail(TROT_ERR_INVALID, "one_body_apply: bad sizes%s%ld");
+  if (!expm || !w_in || !w_out) return fail(TROT_ERR_INVALID, "one_body_apply: null pointer");
+  if (w_in == w_out) return fail(TROT_ERR_INVALID, "one_body_apply: in-place not supported");
+  if (W < 1 || N < 1 || n < 1) return fail(TROT_ERR_INVALID, "one_body_apply: bad sizes");
   GemmP p{};
   p.A = expm; p.lda = N; p.sA = 0;
   p.B = w_in; p.ldb = n; p.sB = (long)N * n;
@@ -165,12 +134,12 @@ size_t trot_taylor_workspace(int W, int N, int n) { return 2 * align256((size_t)
 
 int trot_taylor_apply(const double* vhs, double* w, int W, int N, int n, double sqrt_dt, int n_terms, void* ws,
                       size_t ws_bytes, void* stream) {
-  if (!vhs || !w) return fail(TROT_ERR_INVALID, "taylor_apply: null pointer%s%ld");
-  if (W < 1 || N < 1 || n < 1 || n_terms < 0) return fail(TROT_ERR_INVALID, "taylor_apply: bad sizes%s%ld");
+  if (!vhs || !w) return fail(TROT_ERR_INVALID, "taylor_apply: null pointer");
+  if (W < 1 || N < 1 || n < 1 || n_terms < 0) return fail(TROT_ERR_INVALID, "taylor_apply: bad sizes");
   Carve c{(char*)ws, ws_bytes, 0};
   double* t0 = c.take<double>((size_t)W * N * n * 2);
   double* t1 = c.take<double>((size_t)W * N * n * 2);
-  if (!t0 || !t1) return fail(TROT_ERR_WORKSPACE, "taylor_apply: workspace too small%s%ld");
+  if (!t0 || !t1) return fail(TROT_ERR_WORKSPACE, "taylor_apply: workspace too small");
   hipStream_t s = (hipStream_t)stream;
   if (n_terms == 0) return TROT_OK;
   HIPCHK(hipMemcpyAsync(t0, w, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
@@ -184,6 +153,50 @@ int trot_taylor_apply(const double* vhs, double* w, int W, int N, int n, double 
     p.C2 = w; p.ldc2 = n; p.sC2 = (long)N * n;
     p.M = N; p.N = n; p.K = N; p.ar = 0.0; p.ai = sqrt_dt / k;
     HIPCHK((launch_gemm_f64<64, 64, true, true, true, true, false, EPI_TAYLOR>(p, W, s)));
+    double* tmp = prev;
+    prev = next;
+    next = tmp;
+  }
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// symmetric Cholesky vectors: sym-block storage of L_g and of V
+// ---------------------------------------------------------------------------
+long trot_symblk_cols(int N) {
+  const int nb = (N + 7) / 8;
+  return symblk_count(nb) * 64;
+}
+
+int trot_pack_symblk(const double* chol, int G, int N, void* out, int out_f32, void* stream) {
+  if (!chol || !out) return fail(TROT_ERR_INVALID, "pack_symblk: null pointer");
+  if (G < 1 || N < 1 || G > 65535) return fail(TROT_ERR_INVALID, "pack_symblk: bad sizes");
+  HIPCHK(launch_pack_symblk(chol, out, out_f32, G, N, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+int trot_taylor_apply_symblk(const double* vhs_symblk, double* w, int W, int N, int n, double sqrt_dt, int n_terms,
+                             void* ws, size_t ws_bytes, void* stream) {
+  if (!vhs_symblk || !w) return fail(TROT_ERR_INVALID, "taylor_apply_symblk: null pointer");
+  if (W < 1 || N < 1 || n < 1 || n_terms < 0) return fail(TROT_ERR_INVALID, "taylor_apply_symblk: bad sizes");
+  Carve c{(char*)ws, ws_bytes, 0};
+  double* t0 = c.take<double>((size_t)W * N * n * 2);
+  double* t1 = c.take<double>((size_t)W * N * n * 2);
+  if (!t0 || !t1) return fail(TROT_ERR_WORKSPACE, "taylor_apply_symblk: workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  if (n_terms == 0) return TROT_OK;
+  HIPCHK(hipMemcpyAsync(t0, w, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
+  const int nb = (N + 7) / 8;
+  double* prev = t0;
+  double* next = t1;
+  for (int k = 1; k <= n_terms; ++k) {
+    GemmP p{};
+    p.A = vhs_symblk; p.lda = nb; p.sA = trot_symblk_cols(N);
+    p.B = prev; p.ldb = n; p.sB = (long)N * n;
+    p.C = next; p.ldc = n; p.sC = (long)N * n;
+    p.C2 = w; p.ldc2 = n; p.sC2 = (long)N * n;
+    p.M = N; p.N = n; p.K = N; p.ar = 0.0; p.ai = sqrt_dt / k;
+    HIPCHK((launch_gemm_f64<64, 64, true, true, true, true, false, EPI_TAYLOR, true>(p, W, s)));
     double* tmp = prev;
     prev = next;
     next = tmp;
@@ -206,13 +219,13 @@ size_t trot_apply_trotter_workspace(int W, int N, int n_max, int chunk) {
 int trot_apply_trotter(double* walkers_a, int na, double* walkers_b, int nb, const double* xbar,
                        const double* chol_flat, const double* expm, int W, int N, int G, double sqrt_dt,
                        int n_terms, int chunk, void* ws, size_t ws_bytes, void* stream) {
-  if (!walkers_a || !xbar || !chol_flat || !expm) return fail(TROT_ERR_INVALID, "apply_trotter: null pointer%s%ld");
+  if (!walkers_a || !xbar || !chol_flat || !expm) return fail(TROT_ERR_INVALID, "apply_trotter: null pointer");
   if (W < 1 || N < 1 || G < 1 || na < 1 || (walkers_b && nb < 1))
-    return fail(TROT_ERR_INVALID, "apply_trotter: bad sizes%s%ld");
+    return fail(TROT_ERR_INVALID, "apply_trotter: bad sizes");
   if (chunk <= 0 || chunk > W) chunk = W;
   const int n_max = (walkers_b && nb > na) ? nb : na;
   if (ws_bytes < trot_apply_trotter_workspace(W, N, n_max, chunk))
-    return fail(TROT_ERR_WORKSPACE, "apply_trotter: workspace too small%s%ld");
+    return fail(TROT_ERR_WORKSPACE, "apply_trotter: workspace too small");
   Carve c{(char*)ws, ws_bytes, 0};
   double* v = c.take<double>((size_t)chunk * N * N * 2);
   double* wtmp = c.take<double>((size_t)chunk * N * n_max * 2);
@@ -246,8 +259,8 @@ int trot_phaseless_update(double* weights, const double* ov_old, const double* o
                           const double* fb_term, double* scalars, long long* nodes, int W, double dt, double h0p_re,
                           double h0p_im, double w_floor, double w_cap, double damping, void* stream) {
   if (!weights || !ov_old || !ov_new || !shift_term || !fb_term || !scalars || !nodes)
-    return fail(TROT_ERR_INVALID, "phaseless_update: null pointer%s%ld");
-  if (W < 1 || !(dt > 0)) return fail(TROT_ERR_INVALID, "phaseless_update: bad sizes%s%ld");
+    return fail(TROT_ERR_INVALID, "phaseless_update: null pointer");
+  if (W < 1 || !(dt > 0)) return fail(TROT_ERR_INVALID, "phaseless_update: bad sizes");
   HIPCHK(launch_weight_update(weights, ov_old, ov_new, shift_term, fb_term, scalars, nodes, W, dt, sqrt(dt), h0p_re,
                               h0p_im, w_floor, w_cap, damping, (hipStream_t)stream));
   return TROT_OK;
@@ -267,12 +280,12 @@ int trot_local_energy(const double* rot_chol_a, const double* rot_chol_b, const 
                       const double* green_t_b, const double* rot_h1t_a, const double* rot_h1t_b, int W, int N,
                       int na, int nb, int G, double h0, double* energy, void* ws, size_t ws_bytes, void* stream) {
   if (!green_t_a || !rot_h1t_a || !energy || (G > 0 && !rot_chol_a))
-    return fail(TROT_ERR_INVALID, "local_energy: null pointer%s%ld");
+    return fail(TROT_ERR_INVALID, "local_energy: null pointer");
   if (green_t_b && (!rot_h1t_b || (G > 0 && !rot_chol_b)))
-    return fail(TROT_ERR_INVALID, "local_energy: second spin incomplete%s%ld");
-  if (W < 1 || N < 1 || na < 1 || G < 0) return fail(TROT_ERR_INVALID, "local_energy: bad sizes%s%ld");
-  if (na > 64 || (green_t_b && nb > 64)) return fail(TROT_ERR_UNSUPPORTED, "local_energy: nocc > 64%s%ld");
-  if (ws_bytes < trot_local_energy_workspace(W, G)) return fail(TROT_ERR_WORKSPACE, "local_energy: workspace%s%ld");
+    return fail(TROT_ERR_INVALID, "local_energy: second spin incomplete");
+  if (W < 1 || N < 1 || na < 1 || G < 0) return fail(TROT_ERR_INVALID, "local_energy: bad sizes");
+  if (na > 64 || (green_t_b && nb > 64)) return fail(TROT_ERR_UNSUPPORTED, "local_energy: nocc > 64");
+  if (ws_bytes < trot_local_energy_workspace(W, G)) return fail(TROT_ERR_WORKSPACE, "local_energy: workspace");
   HIPCHK(launch_energy(rot_chol_a, rot_chol_b, green_t_a, green_t_b, rot_h1t_a, rot_h1t_b, na, nb, N, G, W, h0,
                        kEnergyGB, (double*)ws, energy, (hipStream_t)stream));
   return TROT_OK;
@@ -283,8 +296,8 @@ int trot_local_energy(const double* rot_chol_a, const double* rot_chol_b, const 
 // ---------------------------------------------------------------------------
 int trot_block_energy(const double* energy, double* weights, double* scalars, double* out, int W, double dt,
                       double shift_ema, void* stream) {
-  if (!energy || !weights || !scalars || !out) return fail(TROT_ERR_INVALID, "block_energy: null pointer%s%ld");
-  if (W < 1 || !(dt > 0)) return fail(TROT_ERR_INVALID, "block_energy: bad sizes%s%ld");
+  if (!energy || !weights || !scalars || !out) return fail(TROT_ERR_INVALID, "block_energy: null pointer");
+  if (W < 1 || !(dt > 0)) return fail(TROT_ERR_INVALID, "block_energy: bad sizes");
   HIPCHK(launch_block_energy(energy, weights, scalars, out, W, sqrt(2.0 / dt), shift_ema, (hipStream_t)stream));
   return TROT_OK;
 }
@@ -299,14 +312,14 @@ size_t trot_orthonormalize_workspace(int W, int N, int n) {
 }
 
 int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t ws_bytes, void* stream) {
-  if (!walkers) return fail(TROT_ERR_INVALID, "orthonormalize: null pointer%s%ld");
-  if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "orthonormalize: bad sizes%s%ld");
-  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "orthonormalize: nocc > 64%s%ld");
+  if (!walkers) return fail(TROT_ERR_INVALID, "orthonormalize: null pointer");
+  if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "orthonormalize: bad sizes");
+  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "orthonormalize: nocc > 64");
   Carve c{(char*)ws, ws_bytes, 0};
   double* gram = c.take<double>((size_t)W * n * n * 2);
   double* rinv = c.take<double>((size_t)W * n * n * 2);
   double* tmp = c.take<double>((size_t)W * N * n * 2);
-  if (!gram || !rinv || !tmp) return fail(TROT_ERR_WORKSPACE, "orthonormalize: workspace too small%s%ld");
+  if (!gram || !rinv || !tmp) return fail(TROT_ERR_WORKSPACE, "orthonormalize: workspace too small");
   hipStream_t s = (hipStream_t)stream;
   double* src = walkers;
   double* dst = tmp;
@@ -337,16 +350,16 @@ size_t trot_sr_workspace(int W) { return align256((size_t)W * 8); }
 
 int trot_sr_comb(const double* weights, double zeta, int W, int* idx, double* weights_new, void* ws,
                  size_t ws_bytes, void* stream) {
-  if (!weights || !idx || !weights_new) return fail(TROT_ERR_INVALID, "sr_comb: null pointer%s%ld");
-  if (W < 1) return fail(TROT_ERR_INVALID, "sr_comb: bad sizes%s%ld");
-  if (ws_bytes < trot_sr_workspace(W)) return fail(TROT_ERR_WORKSPACE, "sr_comb: workspace too small%s%ld");
+  if (!weights || !idx || !weights_new) return fail(TROT_ERR_INVALID, "sr_comb: null pointer");
+  if (W < 1) return fail(TROT_ERR_INVALID, "sr_comb: bad sizes");
+  if (ws_bytes < trot_sr_workspace(W)) return fail(TROT_ERR_WORKSPACE, "sr_comb: workspace too small");
   HIPCHK(launch_sr_index(weights, (double*)ws, idx, weights_new, W, zeta, (hipStream_t)stream));
   return TROT_OK;
 }
 
 int trot_gather_rows(const double* src, double* dst, const int* idx, int W, long row_doubles, void* stream) {
-  if (!src || !dst || !idx || src == dst) return fail(TROT_ERR_INVALID, "gather_rows: bad pointers%s%ld");
-  if (W < 0 || row_doubles < 0) return fail(TROT_ERR_INVALID, "gather_rows: bad sizes%s%ld");
+  if (!src || !dst || !idx || src == dst) return fail(TROT_ERR_INVALID, "gather_rows: bad pointers");
+  if (W < 0 || row_doubles < 0) return fail(TROT_ERR_INVALID, "gather_rows: bad sizes");
   HIPCHK(launch_gather_rows(src, dst, idx, W, row_doubles, (hipStream_t)stream));
   return TROT_OK;
 }
@@ -356,8 +369,8 @@ int trot_gather_rows(const double* src, double* dst, const int* idx, int W, long
 // ---------------------------------------------------------------------------
 int trot_build_rot_chol(const double* chol, const double* trial, int G, int N, int n, double* rot_chol,
                         double* rot_chol_t, void* stream) {
-  if (!chol || !trial) return fail(TROT_ERR_INVALID, "build_rot_chol: null pointer%s%ld");
-  if (G < 1 || N < 1 || n < 1 || G > 65535) return fail(TROT_ERR_INVALID, "build_rot_chol: bad sizes%s%ld");
+  if (!chol || !trial) return fail(TROT_ERR_INVALID, "build_rot_chol: null pointer");
+  if (G < 1 || N < 1 || n < 1 || G > 65535) return fail(TROT_ERR_INVALID, "build_rot_chol: bad sizes");
   hipStream_t s = (hipStream_t)stream;
   if (rot_chol) {  // rot_chol[g] = C^T L_g  (n x N)
     GemmP p{};
@@ -387,10 +400,10 @@ size_t trot_build_prop_workspace(int G, int N) {
 // v0m (N x N real) = 1/2 sum_g L_g L_g.
 int trot_build_prop_tensors(const double* chol, const double* dm, int G, int N, double* mf, double* mfl,
                             double* v0m, void* ws, size_t ws_bytes, void* stream) {
-  if (!chol || !dm || !mf || !mfl || !v0m) return fail(TROT_ERR_INVALID, "build_prop_tensors: null pointer%s%ld");
-  if (G < 1 || N < 1) return fail(TROT_ERR_INVALID, "build_prop_tensors: bad sizes%s%ld");
+  if (!chol || !dm || !mf || !mfl || !v0m) return fail(TROT_ERR_INVALID, "build_prop_tensors: null pointer");
+  if (G < 1 || N < 1) return fail(TROT_ERR_INVALID, "build_prop_tensors: bad sizes");
   if (ws_bytes < trot_build_prop_workspace(G, N))
-    return fail(TROT_ERR_WORKSPACE, "build_prop_tensors: workspace too small%s%ld");
+    return fail(TROT_ERR_WORKSPACE, "build_prop_tensors: workspace too small");
   hipStream_t s = (hipStream_t)stream;
   HIPCHK(launch_mf_trace(chol, dm, G, N, mf, s));
   {

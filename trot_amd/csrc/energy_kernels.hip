@@ -1,4 +1,4 @@
-// Local energy (trot/meas/rhf.py:41-60, 128-172; trot/meas/uhf.py:205-231), FP64.
+// Local energy (trot/meas/rhf.py:41-60, 128-172; trot/meas/uhf.py:205-231).
 //
 // E_L = h0 + e1 + 1/2 sum_g [ (c_g^a + c_g^b)^2 - K_g^a - K_g^b ],
 //   f_g^s = rot_chol_s[g] (n_s x N) . GT_s (N x n_s)       (GT = G_half^T)
@@ -7,29 +7,49 @@
 // c = 2 c^a, K = 2 K^a, which is 2 sum c^2 - sum K of the reference.
 //
 // exchange_kernel: grid (ceil(G / GB), W).  Each workgroup takes one walker
-// and GB Cholesky vectors; every f_g tile (<= 64 x 64) is produced on MFMA,
-// parked in LDS for the transpose, reduced, and dropped: f is never written
-// to HBM (the reference's "high" mode materialises all of it, "low" scans g;
-// both give the same sums).
+// and GB Cholesky vectors; every f_g tile (<= 64 x 64) is produced on MFMA
+// (FP64, FP32 or split-BF16), parked in LDS for the transpose, reduced in
+// FP64, and dropped: f is never written to HBM (the reference's "high" mode
+// materialises all of it, "low" scans g; both give the same sums).
 #include "common.hpp"
 #include "gemm.hpp"
+#include "gemm_lp.hpp"
 #include "kernels.hpp"
 
-typedef TileF64<64, 64, false, true, true, true, false> ExTile;
+typedef TileF64<64, 64, false, true, true, true, false> ExTile64;
+typedef TileLP<64, 64, MODE_F32, false, true, float, double, true, true, false> ExTile32;
+typedef TileLP<64, 64, MODE_BF16X3, false, true, float, double, true, true, false> ExTileB3;
 
-__device__ __forceinline__ void exchange_one(const double* __restrict__ rc, const double* __restrict__ gt, int n,
-                                             int N, ExTile::Smem& sm, cplx* F, double* red, cplx& c, cplx& k) {
-  ExTile::Acc acc;
-  ExTile::run(rc, N, gt, n, n, n, N, 0, 0, sm, acc);
+template <typename Tile>
+struct ExOps;
+template <>
+struct ExOps<ExTile64> {
+  typedef double TA;
+};
+template <>
+struct ExOps<ExTile32> {
+  typedef float TA;
+};
+template <>
+struct ExOps<ExTileB3> {
+  typedef float TA;
+};
+
+template <typename Tile>
+__device__ __forceinline__ void exchange_one(const typename ExOps<Tile>::TA* __restrict__ rc,
+                                             const double* __restrict__ gt, int n, int N, typename Tile::Smem& sm,
+                                             cplx* F, double* red, cplx& c, cplx& k) {
+  typename Tile::Acc acc;
+  Tile::run(rc, N, gt, n, n, n, N, 0, 0, sm, acc);
   __syncthreads();  // sm is aliased by F
 #pragma unroll
-  for (int i = 0; i < ExTile::TM; ++i)
+  for (int i = 0; i < Tile::TM; ++i)
 #pragma unroll
-    for (int j = 0; j < ExTile::TN; ++j)
+    for (int j = 0; j < Tile::TN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int p = ExTile::row_of(i, r), q = ExTile::col_of(j);
-        F[p * 64 + q] = {acc[0][i][j][r], acc[1][i][j][r]};
+        const int p = Tile::row_of(i, r), q = Tile::col_of(j);
+        F[p * 64 + q] = {(double)acc[0][i][j][r], (double)acc[1][i][j][r]};
       }
   __syncthreads();
   const int t = threadIdx.x;
@@ -52,12 +72,14 @@ __device__ __forceinline__ void exchange_one(const double* __restrict__ rc, cons
   k = {kr, ki};
 }
 
-__global__ __launch_bounds__(256) void exchange_kernel(const double* __restrict__ rc_a, const double* __restrict__ rc_b,
+template <typename Tile>
+__global__ __launch_bounds__(256) void exchange_kernel(const typename ExOps<Tile>::TA* __restrict__ rc_a,
+                                                       const typename ExOps<Tile>::TA* __restrict__ rc_b,
                                                        const double* __restrict__ gt_a, const double* __restrict__ gt_b,
                                                        int na, int nb, int N, int G, int GB,
                                                        double* __restrict__ partial) {
   __shared__ union {
-    ExTile::Smem sm;
+    typename Tile::Smem sm;
     cplx F[64 * 64];
   } u;
   __shared__ double red[256];
@@ -70,9 +92,9 @@ __global__ __launch_bounds__(256) void exchange_kernel(const double* __restrict_
   double sr = 0, si = 0;
   for (int g = g0; g < g1; ++g) {
     cplx ca, ka, cb, kb;
-    exchange_one(rc_a + (long)g * na * N, ga, na, N, u.sm, u.F, red, ca, ka);
+    exchange_one<Tile>(rc_a + (long)g * na * N, ga, na, N, u.sm, u.F, red, ca, ka);
     if (gb) {
-      exchange_one(rc_b + (long)g * nb * N, gb, nb, N, u.sm, u.F, red, cb, kb);
+      exchange_one<Tile>(rc_b + (long)g * nb * N, gb, nb, N, u.sm, u.F, red, cb, kb);
     } else {
       cb = ca;
       kb = ka;
@@ -124,17 +146,37 @@ __global__ __launch_bounds__(256) void energy_finalize_kernel(const double* __re
   }
 }
 
-hipError_t launch_energy(const double* rc_a, const double* rc_b, const double* gt_a, const double* gt_b,
-                         const double* h1t_a, const double* h1t_b, int na, int nb, int N, int G, int W, double h0,
-                         int gb, double* partial, double* energy, hipStream_t s) {
+template <typename Tile>
+static hipError_t launch_energy_t(const typename ExOps<Tile>::TA* rc_a, const typename ExOps<Tile>::TA* rc_b,
+                                  const double* gt_a, const double* gt_b, const double* h1t_a, const double* h1t_b,
+                                  int na, int nb, int N, int G, int W, double h0, int gb, double* partial,
+                                  double* energy, hipStream_t s) {
   if (W <= 0) return hipSuccess;
   if (na > 64 || nb > 64 || na < 1 || (gt_b && nb < 1)) return hipErrorInvalidValue;
   const int nblk = (G + gb - 1) / gb;
   if (G > 0) {
-    hipLaunchKernelGGL(exchange_kernel, dim3(nblk, W), dim3(256), 0, s, rc_a, rc_b, gt_a, gt_b, na, nb, N, G, gb,
-                       partial);
+    hipLaunchKernelGGL(exchange_kernel<Tile>, dim3(nblk, W), dim3(256), 0, s, rc_a, rc_b, gt_a, gt_b, na, nb, N, G,
+                       gb, partial);
   }
   hipLaunchKernelGGL(energy_finalize_kernel, dim3(W), dim3(256), 0, s, gt_a, gt_b, h1t_a, h1t_b, na, nb, N, h0,
                      partial, G > 0 ? nblk : 0, energy);
   return hipGetLastError();
+}
+
+hipError_t launch_energy(const double* rc_a, const double* rc_b, const double* gt_a, const double* gt_b,
+                         const double* h1t_a, const double* h1t_b, int na, int nb, int N, int G, int W, double h0,
+                         int gb, double* partial, double* energy, hipStream_t s) {
+  return launch_energy_t<ExTile64>(rc_a, rc_b, gt_a, gt_b, h1t_a, h1t_b, na, nb, N, G, W, h0, gb, partial, energy, s);
+}
+
+hipError_t launch_energy_lp(const float* rc_a, const float* rc_b, const double* gt_a, const double* gt_b,
+                            const double* h1t_a, const double* h1t_b, int na, int nb, int N, int G, int W, double h0,
+                            int gb, double* partial, double* energy, int mode, hipStream_t s) {
+  if (mode == MODE_F32)
+    return launch_energy_t<ExTile32>(rc_a, rc_b, gt_a, gt_b, h1t_a, h1t_b, na, nb, N, G, W, h0, gb, partial, energy,
+                                     s);
+  if (mode == MODE_BF16X3)
+    return launch_energy_t<ExTileB3>(rc_a, rc_b, gt_a, gt_b, h1t_a, h1t_b, na, nb, N, G, W, h0, gb, partial, energy,
+                                     s);
+  return hipErrorInvalidValue;
 }

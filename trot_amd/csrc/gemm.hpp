@@ -38,7 +38,8 @@ struct GemmP {
 
 // AKC: A[i][k] at i*lda + k (K contiguous), else at k*lda + i.
 // BNC: B[k][j] at k*ldb + j (N contiguous), else at j*ldb + k.
-template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA>
+// ASYM: A is a symmetric matrix in sym-block storage (common.hpp); lda = nb.
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, bool ASYM = false>
 struct TileF64 {
   static constexpr int BK = 16;
   static constexpr int PA = AC ? 2 : 1;
@@ -66,16 +67,24 @@ struct TileF64 {
     const int wm = wave >> 1, wn = wave & 1;
     double ra[EA][PA];
     double rb[EB][PB];
+    int cia[EA];  // ASYM: tile-local (i << 8 | k) of each staged A element
 
     auto load_chunk = [&](int k0) {
 #pragma unroll
       for (int r = 0; r < EA; ++r) {
         const int e = t + 256 * r;
         int i, k;
-        if (AKC) { k = e % BK; i = e / BK; } else { i = e % BM; k = e / BM; }
-        const int gi = m0 + i, gk = k0 + k;
-        const bool ok = gi < M && gk < K;
-        const long off = AKC ? ((long)gi * lda + gk) : ((long)gk * lda + gi);
+        long off;
+        bool ok;
+        if constexpr (ASYM) {
+          ok = symblk_coords<BK>(e, m0, k0, (int)lda, i, k, off);
+          cia[r] = (i << 8) | k;
+        } else {
+          if (AKC) { k = e % BK; i = e / BK; } else { i = e % BM; k = e / BM; }
+          const int gi = m0 + i, gk = k0 + k;
+          ok = gi < M && gk < K;
+          off = AKC ? ((long)gi * lda + gk) : ((long)gk * lda + gi);
+        }
         if (AC) {
           double2 v = ok ? *reinterpret_cast<const double2*>(A + 2 * off) : make_double2(0.0, 0.0);
           ra[r][0] = v.x;
@@ -106,7 +115,12 @@ struct TileF64 {
       for (int r = 0; r < EA; ++r) {
         const int e = t + 256 * r;
         int i, k;
-        if (AKC) { k = e % BK; i = e / BK; } else { i = e % BM; k = e / BM; }
+        if constexpr (ASYM) {
+          i = cia[r] >> 8;
+          k = cia[r] & 255;
+        } else {
+          if (AKC) { k = e % BK; i = e / BK; } else { i = e % BM; k = e / BM; }
+        }
 #pragma unroll
         for (int q = 0; q < PA; ++q) sm.As[q][k][i] = ra[r][q];
       }
@@ -180,9 +194,9 @@ struct TileF64 {
   }
 };
 
-template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI>
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
-  typedef TileF64<BM, BN, AC, BC, AKC, BNC, CONJA> T;
+  typedef TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM> T;
   __shared__ typename T::Smem sm;
   const int b = blockIdx.z;
   const int m0 = blockIdx.y * BM;
@@ -231,11 +245,11 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
       }
 }
 
-template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI>
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false>
 static hipError_t launch_gemm_f64(const GemmP& p, int batch, hipStream_t s) {
   if (p.M <= 0 || p.N <= 0 || batch <= 0) return hipSuccess;
   dim3 grid((p.N + BN - 1) / BN, (p.M + BM - 1) / BM, batch);
   if (grid.y > 65535 || grid.z > 65535) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AC, BC, AKC, BNC, CONJA, EPI>), grid, dim3(256), 0, s, p);
+  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM>), grid, dim3(256), 0, s, p);
   return hipGetLastError();
 }

@@ -20,8 +20,13 @@ hipError_t launch_gather_rows(const double* src, double* dst, const int* idx, in
 hipError_t launch_energy(const double* rc_a, const double* rc_b, const double* gt_a, const double* gt_b,
                          const double* h1t_a, const double* h1t_b, int na, int nb, int N, int G, int W, double h0,
                          int gb, double* partial, double* energy, hipStream_t s);
+hipError_t launch_energy_lp(const float* rc_a, const float* rc_b, const double* gt_a, const double* gt_b,
+                            const double* h1t_a, const double* h1t_b, int na, int nb, int N, int G, int W, double h0,
+                            int gb, double* partial, double* energy, int mode, hipStream_t s);
 hipError_t launch_mf_trace(const double* chol, const double* dm, int G, int N, double* mf, hipStream_t s);
 hipError_t launch_sum_batches(const double* src, double* dst, int nb, long len, double scale, int accumulate,
                               hipStream_t s);
 hipError_t launch_block_energy(const double* energy, double* weights, double* scal, double* out, int W,
                                double thresh, double alpha, hipStream_t s);
+hipError_t launch_pack_symblk(const double* chol, void* out, int out_f32, int G, int N, hipStream_t s);
+hipError_t launch_c128_to_c64(const double* src, float* dst, long n, hipStream_t s);

@@ -437,3 +437,46 @@ hipError_t launch_block_energy(const double* energy, double* weights, double* sc
   hipLaunchKernelGGL(block_energy_kernel, dim3(1), dim3(1024), 0, s, energy, weights, scal, out, W, thresh, alpha);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// chol (G, N, N) -> sym-block storage (G, P*64) of the upper-triangle 8 x 8
+// blocks (common.hpp), FP64 or FP32.  Grid (nb, nb, G), 64 threads; blocks
+// with bi > bj exit.  Entries outside N x N are zero.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void pack_symblk_kernel(const double* __restrict__ chol, T* __restrict__ out, int N,
+                                                         int nb) {
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  if (bi > bj) return;
+  const long g = blockIdx.z;
+  const int t = threadIdx.x;
+  const int i = bi * 8 + (t >> 3), j = bj * 8 + (t & 7);
+  const double v = (i < N && j < N) ? chol[g * (long)N * N + (long)i * N + j] : 0.0;
+  out[g * symblk_count(nb) * 64 + symblk_index(bi, bj, nb) * 64 + t] = (T)v;
+}
+
+hipError_t launch_pack_symblk(const double* chol, void* out, int out_f32, int G, int N, hipStream_t s) {
+  if (G <= 0 || N <= 0) return hipSuccess;
+  const int nb = (N + 7) / 8;
+  dim3 grid(nb, nb, G);
+  if (G > 65535) return hipErrorInvalidConfiguration;
+  if (out_f32)
+    hipLaunchKernelGGL(pack_symblk_kernel<float>, grid, dim3(64), 0, s, chol, (float*)out, N, nb);
+  else
+    hipLaunchKernelGGL(pack_symblk_kernel<double>, grid, dim3(64), 0, s, chol, (double*)out, N, nb);
+  return hipGetLastError();
+}
+
+// complex128 -> complex64 (n complex elements)
+__global__ void c128_to_c64_kernel(const double* __restrict__ src, float* __restrict__ dst, long n) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double2 v = reinterpret_cast<const double2*>(src)[e];
+  reinterpret_cast<float2*>(dst)[e] = make_float2((float)v.x, (float)v.y);
+}
+
+hipError_t launch_c128_to_c64(const double* src, float* dst, long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(c128_to_c64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}

@@ -131,29 +131,52 @@ class Engine:
     walkers) or "uu" (UHF trial, unrestricted walkers).
     """
 
-    def __init__(self, kind: str):
+    def __init__(self, kind: str, precision: str = "f64"):
+        if precision not in ("f64", "f32", "bf16x3"):
+            raise ValueError(f"unknown precision: {precision}")
         self.kind = kind
+        self.precision = precision
 
     def spins(self, meas_ctx):
         if isinstance(meas_ctx, UhfMeasCtx):
             return meas_ctx.spin_a, meas_ctx.spin_b
         return meas_ctx.spin, meas_ctx.spin
 
+    @staticmethod
+    def _f32(spin: SpinCtx, name: str) -> torch.Tensor:
+        """FP32 copy of a Cholesky-side operand, made once per context."""
+        if name not in spin.low:
+            spin.low[name] = getattr(spin, name).to(torch.float32).contiguous()
+        return spin.low[name]
+
     def overlap_green(self, walkers, trial_data):
         return overlap_green(walkers, trial_data, want_green=True)
 
     def force_bias_from_green(self, greens, meas_ctx) -> torch.Tensor:
         sa, sb = self.spins(meas_ctx)
-        if self.kind == "rr":
-            return _lib.force_bias(greens[0], sa.rot_chol_t, 2.0)  # trot/meas/rhf.py:124-125
-        fb = _lib.force_bias(greens[0], sa.rot_chol_t, 1.0)  # trot/meas/rhf.py:150-153; uhf.py:55-57
-        return _lib.force_bias(greens[1], sb.rot_chol_t, 1.0, out=fb, accumulate=True)
+        alpha = 2.0 if self.kind == "rr" else 1.0  # trot/meas/rhf.py:124-125 (both spins); 150-153; uhf.py:55-57
+        if self.precision == "f64":
+            fb = _lib.force_bias(greens[0], sa.rot_chol_t, alpha)
+            if self.kind != "rr":
+                fb = _lib.force_bias(greens[1], sb.rot_chol_t, 1.0, out=fb, accumulate=True)
+            return fb
+        p = self.precision
+        fb = _lib.force_bias_lp(greens[0], self._f32(sa, "rot_chol_t"), alpha, p)
+        if self.kind != "rr":
+            fb = _lib.force_bias_lp(greens[1], self._f32(sb, "rot_chol_t"), 1.0, p, out=fb, accumulate=True)
+        return fb
 
     def energy_from_green(self, greens, h0: float, meas_ctx) -> torch.Tensor:
         sa, sb = self.spins(meas_ctx)
+        if self.precision == "f64":
+            if self.kind == "rr":
+                return _lib.local_energy(sa.rot_chol, greens[0], sa.rot_h1t, h0)
+            return _lib.local_energy(sa.rot_chol, greens[0], sa.rot_h1t, h0, sb.rot_chol, greens[1], sb.rot_h1t)
+        p = self.precision
         if self.kind == "rr":
-            return _lib.local_energy(sa.rot_chol, greens[0], sa.rot_h1t, h0)
-        return _lib.local_energy(sa.rot_chol, greens[0], sa.rot_h1t, h0, sb.rot_chol, greens[1], sb.rot_h1t)
+            return _lib.local_energy_lp(self._f32(sa, "rot_chol"), greens[0], sa.rot_h1t, h0, p)
+        return _lib.local_energy_lp(self._f32(sa, "rot_chol"), greens[0], sa.rot_h1t, h0, p,
+                                    self._f32(sb, "rot_chol"), greens[1], sb.rot_h1t)
 
 
 def _h0(ham_data) -> float:
@@ -179,14 +202,19 @@ def _make_kernels(engine: Engine):
     return force_bias_kernel, energy_kernel
 
 
-def make_rhf_meas_ops(sys: System, memory_mode: str = "high") -> MeasOps:
-    """trot/meas/rhf.py:322-367 for restricted and unrestricted walkers."""
+def make_rhf_meas_ops(sys: System, memory_mode: str = "high", *, precision: str = "f64") -> MeasOps:
+    """trot/meas/rhf.py:322-367 for restricted and unrestricted walkers.
+
+    ``precision`` ("f64", "f32", "bf16x3") selects the MFMA precision of the
+    force-bias and exchange GEMMs; Green's functions, determinants and all
+    sums stay FP64.
+    """
     cfg = RhfMeasCfg(memory_mode=_validate_memory_mode(memory_mode))
     wk = sys.walker_kind
     if wk == "restricted":
-        engine = Engine("rr")
+        engine = Engine("rr", precision)
     elif wk == "unrestricted":
-        engine = Engine("ru")
+        engine = Engine("ru", precision)
     elif wk == "generalized":
         raise NotImplementedError("generalized walkers are not part of the trot_amd GPU path")
     else:
@@ -201,14 +229,14 @@ def make_rhf_meas_ops(sys: System, memory_mode: str = "high") -> MeasOps:
     )
 
 
-def make_uhf_meas_ops(sys: System) -> MeasOps:
+def make_uhf_meas_ops(sys: System, *, precision: str = "f64") -> MeasOps:
     """trot/meas/uhf.py:339-379 (unrestricted walkers)."""
     wk = sys.walker_kind
     if wk in ("restricted", "generalized"):
         raise NotImplementedError(f"UHF trial with {wk} walkers is not part of the trot_amd GPU path")
     if wk != "unrestricted":
         raise ValueError(f"unknown walker_kind: {sys.walker_kind}")
-    engine = Engine("uu")
+    engine = Engine("uu", precision)
     fb, en = _make_kernels(engine)
     return MeasOps(
         overlap=overlap,

@@ -126,9 +126,10 @@ class CholAfqmcCtx:
     chol_flat: torch.Tensor  # (G, N*N) float64
     norb: int
     fields: Callable = field(default_factory=DeviceFields)
-    precision: str = "f64"
+    precision: str = "f64"  # VHS GEMM precision
+    taylor_precision: str = "f64"
     walker_chunk: int = 0
-    low: dict = field(default_factory=dict)  # reduced-precision operand copies
+    low: dict = field(default_factory=dict)  # FP32 operand copies, kernel timer
 
 
 def _total_dm(rdm1) -> torch.Tensor:
@@ -142,7 +143,8 @@ def _total_dm(rdm1) -> torch.Tensor:
     return dm.to(device="cuda", dtype=torch.float64).contiguous()
 
 
-def build_prop_ctx(ham_data, rdm1, dt: float, precision: str = "f64", walker_chunk: int = 0) -> CholAfqmcCtx:
+def build_prop_ctx(ham_data, rdm1, dt: float, precision: str = "f64", walker_chunk: int = 0,
+                   taylor_precision: str | None = None, vhs_layout: str = "auto") -> CholAfqmcCtx:
     """trot/prop/chol_afqmc_ops.py:103-127 (and the host twin trot/runtime_layout.py:125-197).
 
     mf, sum_g mf_g L_g and 1/2 sum_g L_g L_g run on the GPU; the N x N
@@ -171,8 +173,20 @@ def build_prop_ctx(ham_data, rdm1, dt: float, precision: str = "f64", walker_chu
         chol_flat=chol.reshape(G, N * N),
         norb=N,
         precision=precision,
+        taylor_precision=taylor_precision or ("f64" if precision == "f64" else "f32"),
         walker_chunk=walker_chunk,
     )
+    if vhs_layout not in ("auto", "dense", "symblk"):
+        raise ValueError(f"unknown vhs_layout: {vhs_layout}")
+    symmetric = bool(torch.equal(chol, chol.transpose(1, 2)))
+    if vhs_layout == "symblk" and not symmetric:
+        raise ValueError("vhs_layout='symblk' needs symmetric Cholesky vectors")
+    if vhs_layout != "dense" and symmetric:
+        # V = sum_g xbar_g L_g is symmetric: build only its upper 8x8 blocks
+        ctx.low["chol_symblk"] = _lib.pack_symblk(chol, out_f32=precision != "f64")
+    elif precision != "f64":
+        # the reference's mixed mode stores chol_flat as float32 (chol_afqmc_ops.py:117)
+        ctx.low["chol_flat32"] = ctx.chol_flat.to(torch.float32).contiguous()
     return ctx
 
 
@@ -250,10 +264,33 @@ class KernelTimer:
 
 
 def _vhs(ctx: CholAfqmcCtx, xbar: torch.Tensor) -> torch.Tensor:
+    """VHS in the context's precision; complex64 when the Taylor series runs in reduced precision."""
+    chol = ctx.low.get("chol_symblk")  # sym-block storage when every L_g is symmetric
+    if ctx.precision == "f64":
+        fn, args = _lib.vhs_build, (xbar, ctx.chol_flat if chol is None else chol)
+    else:
+        chol = ctx.low["chol_flat32"] if chol is None else chol
+        fn, args = _lib.vhs_build_lp, (xbar, chol, ctx.precision, ctx.taylor_precision == "f64")
     timer = ctx.low.get("timer")
     if timer is not None:
-        return timer.wrap("vhs", _lib.vhs_build, xbar, ctx.chol_flat)
-    return _lib.vhs_build(xbar, ctx.chol_flat)
+        return timer.wrap("vhs", fn, *args)
+    return fn(*args)
+
+
+def _taylor(ctx: CholAfqmcCtx, v: torch.Tensor, t: torch.Tensor, n_terms: int) -> torch.Tensor:
+    sym = "chol_symblk" in ctx.low
+    if ctx.taylor_precision == "f64":
+        if v.dtype != torch.complex128:
+            raise ValueError("an FP64 Taylor series needs a complex128 VHS")
+        if sym:
+            return _lib.taylor_apply_symblk(v, t, ctx.sqrt_dt, n_terms, inplace=True)
+        return _lib.taylor_apply(v.view(t.shape[0], ctx.norb, ctx.norb), t, ctx.sqrt_dt, n_terms, inplace=True)
+    if v.dtype != torch.complex64:
+        raise ValueError("a reduced-precision Taylor series needs a complex64 VHS")
+    if sym:
+        return _lib.taylor_apply_lp_symblk(v, t, ctx.sqrt_dt, n_terms, ctx.taylor_precision, inplace=True)
+    return _lib.taylor_apply_lp(v.view(t.shape[0], ctx.norb, ctx.norb), t, ctx.sqrt_dt, n_terms,
+                                ctx.taylor_precision, inplace=True)
 
 
 def apply_trotter(walkers, xbar: torch.Tensor, ctx: CholAfqmcCtx, n_terms: int):
@@ -270,10 +307,10 @@ def apply_trotter(walkers, xbar: torch.Tensor, ctx: CholAfqmcCtx, n_terms: int):
     outs = [None] * len(spins) if chunk == W else [torch.empty_like(s) for s in spins]
     for w0 in range(0, W, chunk):
         w1 = min(W, w0 + chunk)
-        v = _vhs(ctx, xbar[w0:w1]).view(w1 - w0, N, N)
+        v = _vhs(ctx, xbar[w0:w1])
         for i, s in enumerate(spins):
             t = _lib.one_body_apply(ctx.exp_h1_half, s[w0:w1])
-            t = _lib.taylor_apply(v, t, ctx.sqrt_dt, n_terms, inplace=True)
+            t = _taylor(ctx, v, t, n_terms)
             r = _lib.one_body_apply(ctx.exp_h1_half, t)
             if chunk == W:
                 outs[i] = r
@@ -329,13 +366,18 @@ class PropOps:
 
 
 def make_prop_ops(ham_basis: str, walker_kind: str, mixed_precision: bool = False, *,
-                  precision: str | None = None, walker_chunk: int = 0) -> PropOps:
+                  precision: str | None = None, taylor_precision: str | None = None,
+                  walker_chunk: int = 0, vhs_layout: str = "auto") -> PropOps:
     """trot/prop/afqmc.py:147-180.
 
-    ``mixed_precision=True`` selects the FP32 Cholesky GEMMs (the reference's
-    mixed mode, trot/prop/chol_afqmc_ops.py:247-257); ``precision`` may name
-    "f64", "f32" or "bf16x3" explicitly.
+    ``mixed_precision=True`` selects the FP32 VHS GEMM (the reference's mixed
+    mode, trot/prop/chol_afqmc_ops.py:247-257); ``precision`` may name "f64",
+    "f32" or "bf16x3" explicitly.  ``taylor_precision`` (default: "f64" for an
+    FP64 VHS, else "f32") sets the MFMA precision of the Taylor terms; the
+    walkers and the running Taylor sum stay complex128.
     """
+    if taylor_precision not in (None, "f64", "f32", "bf16x3"):
+        raise ValueError(f"unknown taylor_precision: {taylor_precision}")
     if ham_basis not in ("restricted", "generalized"):
         raise ValueError(f"unknown ham_basis: {ham_basis}")
     walker_kind = walker_kind.lower()
@@ -353,6 +395,7 @@ def make_prop_ops(ham_basis: str, walker_kind: str, mixed_precision: bool = Fals
                           prop_ctx=prop_ctx, meas_ctx=meas_ctx, fields=fields)
 
     def _build(ham_data, rdm1, params):
-        return build_prop_ctx(ham_data, rdm1, params.dt, precision=prec, walker_chunk=walker_chunk)
+        return build_prop_ctx(ham_data, rdm1, params.dt, precision=prec, walker_chunk=walker_chunk,
+                              taylor_precision=taylor_precision, vhs_layout=vhs_layout)
 
     return PropOps(init_prop_state=init_prop_state, build_prop_ctx=_build, step=step)

@@ -1,20 +1,28 @@
-"""Throughput of the phaseless AFQMC walker step on one MI355X.
+"""Throughput of the phaseless AFQMC walker step on one MI355X: mixed-precision MFMA vs FP64.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--precision f64|f32|bf16x3]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                    [--precision bf16x3|f32] [--taylor-precision f32|bf16x3|f64] [--no-cpu] [--no-block]
 
 Workload (default): the nbasis=512 synthetic system, N=512 orbitals, n=64
 occupied, G=2560 Cholesky vectors, 2048 complex128 walkers, dt=0.005, RHF
-trial, restricted walkers (random-init synthetic Hamiltonian, see
-trot_amd/synthetic.py).  A step is one phaseless step for all walkers: field
-draw, force bias, shifted fields, VHS build, one-body / Taylor / one-body
-propagation, overlaps and Green's functions, weight update.  All inputs are
-resident in HBM before the timed region.
+trial, restricted walkers (seeded synthetic Hamiltonian, trot_amd/synthetic.py).
+A step is one phaseless step for all walkers: field draw, force bias, shifted
+fields, VHS build, one-body / Taylor / one-body propagation, overlaps and
+Green's functions, weight update.  All inputs are resident in HBM before the
+timed regions.
 
-Prints one JSON line: walker-steps/s (whole job), the roofline of the
-dominant kernel (the VHS GEMM; duration from HIP events recorded around its
-launches inside the timed steps), the block-inclusive rate (energy, QR and
-SR every n_prop_steps=50 steps), and a CPU baseline: the NumPy FP64 oracle
-timed on a bounded sample of the same workload on the host cores.
+The same process times K steps of the FP64 path and K steps of the mixed
+path (reduced-precision Cholesky GEMMs and Taylor terms on MFMA; FP64
+determinants, inverse, weights, energies).  `value` is the mixed rate; the
+FP64 rate and the speedup sit beside it.  The mixed path's accuracy is
+measured against the FP64 path (which the test suite pins to the CPU oracle)
+by running both from one state with identical fields and comparing the
+per-walker local energies and the weighted block energy, in mHa.
+
+`roofline` describes the dominant kernel of the mixed step (the VHS GEMM);
+its duration comes from HIP events recorded around its launches inside the
+timed steps.  `cpu_baseline` is the NumPy FP64 oracle timed on a bounded
+sample of the same workload on the host cores.
 """
 
 from __future__ import annotations
@@ -32,7 +40,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}  # dense MFMA TFLOP/s, MI355X datasheet
-METRIC = "walker-steps/s at nbasis=512 (phaseless AFQMC step)"
+METRIC = "walker-steps/s at nbasis=512: MFMA mixed vs FP64 step"
 
 
 def parse():
@@ -41,10 +49,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="n512_o64_g2560_w2048")
-    ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16x3"])
+    ap.add_argument("--precision", default="bf16x3", choices=["f32", "bf16x3"])
+    ap.add_argument("--taylor-precision", default="f32", choices=["f64", "f32", "bf16x3"])
+    ap.add_argument("--accuracy-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-block", action="store_true", help="skip the block-tail timing")
-    ap.add_argument("--cpu-walkers", type=int, default=128)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (profiling runs)")
+    ap.add_argument("--cpu-walkers", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=2)
     return ap.parse_args()
 
@@ -53,9 +64,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel_tag: str, workload: str):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary, if any."""
-    p = ROOT / "profiles" / f"{kernel_tag}_{workload}_traffic.md"
+def traffic_from_profiles(tag: str):
+    """HBM bytes per launch of the dominant kernel from a committed rocprofv3 --pmc summary, if any."""
+    p = ROOT / "profiles" / f"{tag}_traffic.md"
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
@@ -64,13 +75,60 @@ def traffic_from_profiles(kernel_tag: str, workload: str):
     return None
 
 
+class Runner:
+    """One precision configuration of the step on a shared synthetic system."""
+
+    def __init__(self, env, precision: str, taylor_precision: str):
+        from trot_amd import meas, prop
+
+        self.env = env
+        self.mops = meas.make_rhf_meas_ops(env["sys"], precision=precision)
+        self.pops = prop.make_prop_ops("restricted", "restricted", precision=precision,
+                                       taylor_precision=taylor_precision)
+        self.pctx = self.pops.build_prop_ctx(env["ham"], env["tops"].get_rdm1(env["tr"]), env["params"])
+        self.mctx = self.mops.build_meas_ctx(env["ham"], env["tr"])
+        self.precision = precision
+        self.taylor_precision = taylor_precision
+
+    def init(self):
+        e = self.env
+        return self.pops.init_prop_state(sys=e["sys"], ham_data=e["ham"], trial_ops=e["tops"], trial_data=e["tr"],
+                                         meas_ops=self.mops, params=e["params"], initial_walkers=e["w0"])
+
+    def step(self, st, fields=None):
+        e = self.env
+        return self.pops.step(st, params=e["params"], ham_data=e["ham"], trial_data=e["tr"], trial_ops=e["tops"],
+                              meas_ops=self.mops, meas_ctx=self.mctx, prop_ctx=self.pctx, fields=fields)
+
+    def timed(self, st, warmup: int, steps: int, tag: str):
+        import torch
+
+        from trot_amd import prop
+
+        for i in range(warmup):
+            st = self.step(st)
+            torch.cuda.synchronize()
+            log(f"{tag}: warmup step {i + 1}/{warmup} done")
+        timer = prop.KernelTimer()
+        self.pctx.low["timer"] = timer
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = self.step(st)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        self.pctx.low.pop("timer")
+        log(f"{tag}: {steps} steps in {dt:.3f}s ({1e3 * dt / steps:.2f} ms/step)")
+        return st, dt, timer.mean_ms("vhs")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (--gpus must be 1)")
     import torch
 
-    from trot_amd import _lib, blocks, meas, prop, synthetic, trial
+    from trot_amd import _lib, meas, prop, synthetic, trial
     from trot_amd.core import HamChol, System
 
     _lib.lib()
@@ -80,77 +138,55 @@ def main():
     t0 = time.perf_counter()
     syn = synthetic.make_synthetic(N, n, G, W, cfg["seed"], cfg["tau"], device=dev)
     log(f"synthetic system N={N} n={n} G={G} W={W} in {time.perf_counter() - t0:.1f}s")
-
     sys_ = System(N, (n, n), "restricted")
-    ham = HamChol(syn.h0, torch.from_numpy(syn.h1).to(dev), syn.chol)
-    tr = trial.RhfTrial(syn.trial)
-    tops = trial.make_rhf_trial_ops(sys_)
-    mops = meas.make_rhf_meas_ops(sys_)
-    pops = prop.make_prop_ops("restricted", "restricted", precision=args.precision)
-    params = prop.QmcParams(dt=dt, n_walkers=W, seed=cfg["seed"])
-    pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
-    mctx = mops.build_meas_ctx(ham, tr)
-    state = pops.init_prop_state(sys=sys_, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops,
-                                 params=params, initial_walkers=syn.walkers)
+    env = dict(sys=sys_, ham=HamChol(syn.h0, torch.from_numpy(syn.h1).to(dev), syn.chol),
+               tr=trial.RhfTrial(syn.trial), tops=trial.make_rhf_trial_ops(sys_),
+               params=prop.QmcParams(dt=dt, n_walkers=W, seed=cfg["seed"]), w0=syn.walkers)
+
+    mixed = Runner(env, args.precision, args.taylor_precision)
+    st_mix = mixed.init()
     torch.cuda.synchronize()
-    log(f"contexts and initial state ready at {time.perf_counter() - t0:.1f}s")
+    log(f"mixed contexts ready at {time.perf_counter() - t0:.1f}s (sym-block VHS: {'chol_symblk' in mixed.pctx.low})")
+    st_mix, t_mix, vhs_ms = mixed.timed(st_mix, args.warmup, args.steps, f"mixed {args.precision}")
+    value = W * args.steps / t_mix
 
-    def step(st):
-        return pops.step(st, params=params, ham_data=ham, trial_data=tr, trial_ops=tops, meas_ops=mops,
-                         meas_ctx=mctx, prop_ctx=pctx)
+    fp64 = None
+    rate64 = None
+    if not args.no_fp64:
+        fp64 = Runner(env, "f64", "f64")
+        st64 = fp64.init()
+        st64, t64, vhs64_ms = fp64.timed(st64, args.warmup, args.steps, "fp64")
+        rate64 = W * args.steps / t64
 
-    for i in range(args.warmup):
-        state = step(state)
-        torch.cuda.synchronize()
-        log(f"warmup step {i + 1}/{args.warmup} done")
+    accuracy = None
+    if fp64 is not None and args.accuracy_steps > 0:
+        # same sizes and draws, L scaled by 1/sqrt(N) so that walkers survive (see synthetic.make_synthetic)
+        del st64, st_mix
+        env_acc = dict(env, ham=HamChol(syn.h0, env["ham"].h1, syn.chol * (1.0 / np.sqrt(N))))
+        mixed_acc = Runner(env_acc, args.precision, args.taylor_precision)
+        fp64_acc = Runner(env_acc, "f64", "f64")
+        accuracy = paired_accuracy(env_acc, mixed_acc, fp64_acc, args.accuracy_steps)
+        accuracy["system"] = "same sizes and seed, every L_gamma scaled by 1/sqrt(N)"
+        del mixed_acc, fp64_acc, env_acc
+        st_mix = mixed.init()
 
-    timer = prop.KernelTimer()
-    pctx.low["timer"] = timer
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        state = step(state)
-    torch.cuda.synchronize()
-    t_total = time.perf_counter() - t_start
-    pctx.low.pop("timer")
-    ms_step = 1e3 * t_total / args.steps
-    value = W * args.steps / t_total
-    log(f"timed {args.steps} steps: {ms_step:.2f} ms/step, {value:.1f} walker-steps/s")
-
-    vhs_ms = timer.mean_ms("vhs")
-    vhs_flop = 4.0 * W * G * N * N
-    achieved = vhs_flop / (vhs_ms * 1e-3) / 1e12
+    # roofline of the dominant kernel (VHS GEMM of the mixed step): flops the algorithm run here needs
+    sym = "chol_symblk" in mixed.pctx.low
+    cols = _lib.symblk_cols(N) if sym else N * N
+    passes = 3 if args.precision == "bf16x3" else 1
+    vhs_flop = 4.0 * W * G * cols  # real x complex MAC = 4 flop over the stored V entries
+    achieved = vhs_flop * passes / (vhs_ms * 1e-3) / 1e12
     peak = PEAK[args.precision]
-    traffic = traffic_from_profiles("vhs", args.workload)
-    roofline = {"bound": "mfma", "kernel": "VHS GEMM (vhs_build)", "achieved": round(achieved, 2),
-                "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
-                "launch_ms": round(vhs_ms, 3), "flop_per_launch": vhs_flop}
-    step_flop = 4.0 * G * N * N + 4.0 * G * n * N + 8.0 * params.n_exp_terms * N * N * n + 8.0 * N * N * n \
-        + 16.0 * n * n * N
-    step_tflops = step_flop * W / (ms_step * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": f"VHS GEMM ({args.precision}, {'sym-block' if sym else 'dense'} V)",
+                "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                "traffic": traffic_from_profiles(f"vhs_{args.precision}_{args.workload}"),
+                "launch_ms": round(vhs_ms, 3), "flop_per_launch": vhs_flop * passes}
 
     block_rate = None
     if not args.no_block:
-        torch.cuda.synchronize()
-        tb = time.perf_counter()
-        reps = 2
-        for _ in range(reps):
-            wq = __import__("trot_amd.walkers", fromlist=["x"]).orthonormalize(state.walkers, "restricted")
-            ov, greens = mops.engine.overlap_green(wq, tr)
-            e = mops.engine.energy_from_green(greens, 0.0, mctx)
-            from trot_amd import walkers as wk
+        block_rate = block_inclusive(env, mixed, st_mix, t_mix / args.steps)
 
-            idx, wn = wk.sr_indices(state.weights, 0.5)
-            _ = wk.gather(wq, idx)
-        torch.cuda.synchronize()
-        t_tail = (time.perf_counter() - tb) / reps
-        nps = params.n_prop_steps
-        block_rate = W * nps / (nps * t_total / args.steps + t_tail)
-        log(f"block tail (QR + energy + SR): {1e3 * t_tail:.1f} ms; block-inclusive {block_rate:.1f} walker-steps/s")
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(syn, pctx, mctx, args, dt)
+    cpu = None if args.no_cpu else cpu_baseline(syn, mixed, args, dt)
 
     out = {
         "metric": METRIC,
@@ -159,17 +195,21 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": round(1e3 * t_mix / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f64": "f64 (complex128 walkers, FP64 MFMA)", "f32": "f32 Cholesky GEMMs / f64 det, QR, weights",
-                  "bf16x3": "split-bf16 Cholesky GEMMs / f64 det, QR, weights"}[args.precision],
+        "dtype": f"mixed: {args.precision} MFMA Cholesky GEMMs (VHS, force bias), {args.taylor_precision} Taylor "
+                 f"terms, f64 walkers / det / inverse / weights / energies",
         "data": "synthetic (seeded numpy PCG64 Hamiltonian, trial and walkers; on-device N(0,1) fields)",
         "config": {"workload": args.workload, "nbasis": N, "nocc": n, "nchol": G, "n_walkers": W, "dt": dt,
-                   "n_exp_terms": params.n_exp_terms, "trial": "RHF", "walker_kind": "restricted",
-                   "precision": args.precision},
-        "step_tflops": round(step_tflops, 2),
+                   "n_exp_terms": env["params"].n_exp_terms, "trial": "RHF", "walker_kind": "restricted",
+                   "precision": args.precision, "taylor_precision": args.taylor_precision,
+                   "vhs_layout": "sym-block" if sym else "dense"},
+        "fp64_walker_steps_per_s": None if rate64 is None else round(rate64, 2),
+        "fp64_ms_per_step": None if rate64 is None else round(1e3 * W / rate64, 3),
+        "speedup_vs_fp64": None if rate64 is None else round(value / rate64, 3),
+        "accuracy_vs_fp64": accuracy,
         "block_inclusive_walker_steps_per_s": None if block_rate is None else round(block_rate, 2),
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -177,10 +217,60 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(syn, pctx, mctx, args, dt):
-    """NumPy FP64 oracle (oracle/afqmc_oracle.py) timed on a bounded sample of the same workload."""
+def paired_accuracy(env, mixed: Runner, fp64: Runner, steps: int) -> dict:
+    """Both paths from one initial state with identical fields; E_L of the final walkers in FP64."""
     import torch
 
+    from trot_amd.meas import _h0
+
+    W = env["params"].n_walkers
+    G = mixed.pctx.chol_flat.shape[0]
+    a, b = mixed.init(), fp64.init()
+    for s in range(steps):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(9000 + s)
+        f = torch.randn((W, G), dtype=torch.float64, device="cuda", generator=g)
+        a = mixed.step(a, f)
+        b = fp64.step(b, f)
+    _, ga = fp64.mops.engine.overlap_green(a.walkers, env["tr"])
+    _, gb = fp64.mops.engine.overlap_green(b.walkers, env["tr"])
+    ea = fp64.mops.engine.energy_from_green(ga, _h0(env["ham"]), fp64.mctx).real
+    eb = fp64.mops.engine.energy_from_green(gb, _h0(env["ham"]), fp64.mctx).real
+    blk_a = float((a.weights * ea).sum() / a.weights.sum())
+    blk_b = float((b.weights * eb).sum() / b.weights.sum())
+    out = {"steps": steps, "per_walker_max_abs_dE_mHa": round(1e3 * float((ea - eb).abs().max()), 6),
+           "per_walker_rms_dE_mHa": round(1e3 * float(((ea - eb) ** 2).mean().sqrt()), 6),
+           "block_energy_dE_mHa": round(1e3 * abs(blk_a - blk_b), 6), "block_energy_fp64_Ha": round(blk_b, 8),
+           "max_rel_weight_diff": float(((a.weights - b.weights).abs() / b.weights.abs().clamp_min(1e-300)).max())}
+    log(f"accuracy after {steps} paired steps: {out}")
+    return out
+
+
+def block_inclusive(env, r: Runner, st, t_step: float) -> float:
+    """Rate with the block tail (QR, energy, clamp / mean / EMA, SR) every n_prop_steps steps."""
+    import torch
+
+    from trot_amd import blocks
+
+    params = env["params"]
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    # a block with zero propagation steps is exactly the tail: QR, overlaps, energy, SR
+    import dataclasses
+
+    p0 = dataclasses.replace(params, n_prop_steps=0)
+    st, _ = blocks.block(st, sys=env["sys"], params=p0, ham_data=env["ham"], trial_data=env["tr"],
+                         trial_ops=env["tops"], meas_ops=r.mops, meas_ctx=r.mctx, prop_ops=r.pops, prop_ctx=r.pctx)
+    torch.cuda.synchronize()
+    t_tail = time.perf_counter() - tb
+    nps = params.n_prop_steps
+    rate = params.n_walkers * nps / (nps * t_step + t_tail)
+    log(f"block tail {1e3 * t_tail:.1f} ms; block-inclusive {rate:.1f} walker-steps/s")
+    return rate
+
+
+def cpu_baseline(syn, r: Runner, args, dt):
+    """NumPy FP64 oracle (oracle/afqmc_oracle.py) timed on a bounded sample of the same workload."""
     from oracle import afqmc_oracle as O
 
     try:
@@ -190,14 +280,14 @@ def cpu_baseline(syn, pctx, mctx, args, dt):
     except Exception:  # pragma: no cover
         threads = os.cpu_count() or 1
     W = min(args.cpu_walkers, syn.walkers.shape[0])
-    N = pctx.norb
+    N = r.pctx.norb
     chol_flat = syn.chol.reshape(syn.chol.shape[0], -1).cpu().numpy()
-    ctx = O.PropCtx(dt=dt, sqrt_dt=float(np.sqrt(dt)), exp_h1_half=pctx.exp_h1_half.cpu().numpy(),
-                    mf_shifts=pctx.mf_shifts.cpu().numpy(), h0_prop=pctx.h0_prop, chol_flat=chol_flat, norb=N)
+    ctx = O.PropCtx(dt=dt, sqrt_dt=float(np.sqrt(dt)), exp_h1_half=r.pctx.exp_h1_half.cpu().numpy(),
+                    mf_shifts=r.pctx.mf_shifts.cpu().numpy(), h0_prop=r.pctx.h0_prop, chol_flat=chol_flat, norb=N)
     c = syn.trial
-    rot = O.RhfMeasCtx(rot_h1=mctx.rot_h1.cpu().numpy(), rot_chol=mctx.rot_chol.cpu().numpy(),
-                       rot_chol_flat=mctx.rot_chol.cpu().numpy().reshape(mctx.rot_chol.shape[0], -1))
-    ops = O.RhfOps(overlap=lambda w: O.overlap_r(w, c), force_bias=lambda w: O.force_bias_rw(w, rot, c),
+    rc = r.mctx.rot_chol.cpu().numpy()
+    mctx = O.RhfMeasCtx(rot_h1=r.mctx.rot_h1.cpu().numpy(), rot_chol=rc, rot_chol_flat=rc.reshape(rc.shape[0], -1))
+    ops = O.RhfOps(overlap=lambda w: O.overlap_r(w, c), force_bias=lambda w: O.force_bias_rw(w, mctx, c),
                    energy=None)
     w0 = syn.walkers[:W].cpu().numpy()
     st = O.PropState(walkers=w0, weights=np.ones(W), overlaps=ops.overlap(w0), pop_control_ene_shift=0.0,
@@ -205,18 +295,18 @@ def cpu_baseline(syn, pctx, mctx, args, dt):
     params = O.QmcParams(dt=dt, n_walkers=W)
     rng = np.random.default_rng(0)
     G = chol_flat.shape[0]
-    st = O.afqmc_step(st, rng.standard_normal((W, G)), ctx, ops, params)  # warm-up
+    log(f"cpu baseline: {W} walkers, warm-up step ({threads} BLAS threads)")
+    st = O.afqmc_step(st, rng.standard_normal((W, G)), ctx, ops, params)
     t0 = time.perf_counter()
-    for _ in range(args.cpu_steps):
+    for s in range(args.cpu_steps):
         st = O.afqmc_step(st, rng.standard_normal((W, G)), ctx, ops, params)
+        log(f"cpu baseline: step {s + 1}/{args.cpu_steps} at {time.perf_counter() - t0:.1f}s")
     dt_cpu = time.perf_counter() - t0
     rate = W * args.cpu_steps / dt_cpu
-    log(f"cpu baseline: {W} walkers x {args.cpu_steps} steps in {dt_cpu:.1f}s -> {rate:.2f} walker-steps/s "
-        f"({threads} threads)")
-    del torch
+    log(f"cpu baseline: {rate:.2f} walker-steps/s")
     return {"value": round(rate, 3), "unit": "walker-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{W} of the workload's walkers x {args.cpu_steps} steps (after 1 warm-up step), "
-                      f"numpy FP64 oracle, same Hamiltonian and trial, {dt_cpu:.1f} s"}
+            "sample": f"{W} of the workload's 2048 walkers x {args.cpu_steps} FP64 steps after 1 warm-up step, "
+                      f"numpy oracle (oracle/afqmc_oracle.py), same Hamiltonian and trial, {dt_cpu:.1f} s"}
 
 
 if __name__ == "__main__":

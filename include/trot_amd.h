@@ -155,6 +155,23 @@ int trot_taylor_apply_symblk(const double* vhs_symblk, double* w, int W, int N, 
 int trot_taylor_apply_lp_symblk(const float* vhs_symblk, double* w, int W, int N, int n, double sqrt_dt,
                                 int n_terms, int prec, void* ws, size_t ws_bytes, void* stream);
 
+/* ---- 3-pass split-BF16 NT GEMM on pre-split operands ----
+ * An operand matrix X (rows x K) is stored as bf16 planes x = hi + lo: [re_hi, re_lo, im_hi, im_lo] for
+ * complex, [hi, lo] for real, each plane (trot_b3_rows_pad(rows), trot_b3_k_pad(K)), K contiguous, zero
+ * padded.  trot_gemm_b3 computes C = alpha A B^T (+ beta C) with A (M x K, real or complex) and B
+ * (N x K, real) in that storage on v_mfma_f32_16x16x32_bf16 (lo.hi + hi.lo + hi.hi, FP32 accumulation).
+ * Used for the VHS (A = xbar, B = chol in sym-block or dense layout, transposed) and the force bias
+ * (A = GT, B = R'): the Cholesky GEMMs of trot/prop/chol_afqmc_ops.py:83-87 and trot/meas/rhf.py:125. */
+long trot_b3_rows_pad(int rows);
+long trot_b3_k_pad(int K);
+size_t trot_b3_planes_bytes(int rows, int K, int is_complex);
+int trot_split_bf16(const void* src, int src_f32, int is_complex, int transposed, long ld, int rows, int K,
+                    void* dst, void* stream);
+size_t trot_gemm_b3_workspace(int M, int N, int a_complex, int ksplit);
+int trot_gemm_b3(const void* a_planes, int a_complex, const void* b_planes, int M, int N, int K, void* C, int c_f64,
+                 long ldc, double alpha_re, double alpha_im, double beta, int ksplit, void* ws, size_t ws_bytes,
+                 void* stream);
+
 /* ---- reduced-precision forms (prec = TROT_PREC_F32 or TROT_PREC_BF16X3) ----
  * Cholesky-side operands are FP32 copies made at context build; FP64 walker-side operands are rounded
  * (and split) as they are staged into LDS; accumulation is FP32 on MFMA.  The reference's own mixed mode

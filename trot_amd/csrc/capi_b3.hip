@@ -1,0 +1,113 @@
+// C-ABI of the 3-pass split-BF16 path: operand splitting and the NT GEMM on split planes.
+#include "capi_util.hpp"
+#include "common.hpp"
+#include "gemm_b3.hpp"
+
+static inline long pad_to(long x, long m) { return (x + m - 1) / m * m; }
+
+extern "C" {
+
+long trot_b3_rows_pad(int rows) { return pad_to(rows < 1 ? 1 : rows, 128); }
+long trot_b3_k_pad(int K) { return pad_to(K < 1 ? 1 : K, 32); }
+
+size_t trot_b3_planes_bytes(int rows, int K, int is_complex) {
+  return (size_t)(is_complex ? 4 : 2) * trot_b3_rows_pad(rows) * trot_b3_k_pad(K) * sizeof(bf16raw);
+}
+
+// src: rows x K (row-major with leading dim ld, or column-major if transposed), FP64 or FP32
+// (src_f32), real or interleaved complex.  dst: bf16 planes of (rows_pad, K_pad), zero padded.
+int trot_split_bf16(const void* src, int src_f32, int is_complex, int transposed, long ld, int rows, int K,
+                    void* dst, void* stream) {
+  if (!src || !dst) return fail(TROT_ERR_INVALID, "split_bf16: null pointer");
+  if (rows < 1 || K < 1 || ld < 1) return fail(TROT_ERR_INVALID, "split_bf16: bad sizes");
+  const int rp = (int)trot_b3_rows_pad(rows), kp = (int)trot_b3_k_pad(K);
+  const long total = (long)rp * kp;
+  const unsigned grid = (unsigned)((total + 255) / 256);
+  hipStream_t s = (hipStream_t)stream;
+  bf16raw* d = (bf16raw*)dst;
+#define SPLIT_LAUNCH(CX, TR, T) \
+  hipLaunchKernelGGL((split_planes_kernel<CX, TR, T>), dim3(grid), dim3(256), 0, s, (const T*)src, ld, rows, K, rp, kp, d)
+  if (is_complex) {
+    if (src_f32) {
+      if (transposed) SPLIT_LAUNCH(true, true, float); else SPLIT_LAUNCH(true, false, float);
+    } else {
+      if (transposed) SPLIT_LAUNCH(true, true, double); else SPLIT_LAUNCH(true, false, double);
+    }
+  } else {
+    if (src_f32) {
+      if (transposed) SPLIT_LAUNCH(false, true, float); else SPLIT_LAUNCH(false, false, float);
+    } else {
+      if (transposed) SPLIT_LAUNCH(false, true, double); else SPLIT_LAUNCH(false, false, double);
+    }
+  }
+#undef SPLIT_LAUNCH
+  HIPCHK(hipGetLastError());
+  return TROT_OK;
+}
+
+size_t trot_gemm_b3_workspace(int M, int N, int a_complex, int ksplit) {
+  if (ksplit <= 1) return 0;
+  return align256((size_t)ksplit * M * N * (a_complex ? 8 : 4));
+}
+
+// C (M x N, ldc; complex if a_complex, FP64 if c_f64 else FP32) = alpha * A . B^T (+ beta C)
+// A: split planes (M rows, K), B: split planes (N rows, K), both as made by trot_split_bf16.
+// ksplit > 1 splits K over blockIdx.y with FP32 partials in ws and an FP64 reduction
+// (complex FP64 output only).
+int trot_gemm_b3(const void* a_planes, int a_complex, const void* b_planes, int M, int N, int K, void* C, int c_f64,
+                 long ldc, double alpha_re, double alpha_im, double beta, int ksplit, void* ws, size_t ws_bytes,
+                 void* stream) {
+  if (!a_planes || !b_planes || !C) return fail(TROT_ERR_INVALID, "gemm_b3: null pointer");
+  if (M < 1 || N < 1 || K < 1 || ldc < N) return fail(TROT_ERR_INVALID, "gemm_b3: bad sizes");
+  if (!a_complex && alpha_im != 0.0) return fail(TROT_ERR_INVALID, "gemm_b3: complex alpha needs complex A");
+  const int kp = (int)trot_b3_k_pad(K);
+  if (ksplit < 1) ksplit = 1;
+  int kslice = (int)pad_to((kp + ksplit - 1) / ksplit, 32);
+  ksplit = (kp + kslice - 1) / kslice;
+  if (ksplit > 1 && !(a_complex && c_f64))
+    return fail(TROT_ERR_UNSUPPORTED, "gemm_b3: split K supports complex FP64 output only");
+  if (ws_bytes < trot_gemm_b3_workspace(M, N, a_complex, ksplit))
+    return fail(TROT_ERR_WORKSPACE, "gemm_b3: workspace too small");
+  if (ksplit > 65535) return fail(TROT_ERR_INVALID, "gemm_b3: ksplit too large");
+  B3P p{};
+  p.A = (const bf16raw*)a_planes;
+  p.B = (const bf16raw*)b_planes;
+  p.a_plane = trot_b3_rows_pad(M) * kp;
+  p.b_plane = trot_b3_rows_pad(N) * kp;
+  p.Kp = kp;
+  p.k_slice = kslice;
+  p.M = M;
+  p.N = N;
+  p.C = C;
+  p.ldc = ldc;
+  p.part = ksplit > 1 ? (float*)ws : nullptr;
+  p.ar = (float)alpha_re;
+  p.ai = (float)alpha_im;
+  p.beta = (float)beta;
+  const int n_mt = (M + 127) / 128, n_nt = (N + 127) / 128;
+  const long T = (long)n_mt * n_nt;
+  if (T > 0x7fffffffL / 2) return fail(TROT_ERR_INVALID, "gemm_b3: too many tiles");
+  dim3 grid((unsigned)((T + 7) / 8 * 8), (unsigned)ksplit);
+  hipStream_t s = (hipStream_t)stream;
+  if (a_complex) {
+    if (c_f64)
+      hipLaunchKernelGGL((gemm_b3_nt_kernel<true, double>), grid, dim3(256), 0, s, p, n_mt, n_nt);
+    else
+      hipLaunchKernelGGL((gemm_b3_nt_kernel<true, float>), grid, dim3(256), 0, s, p, n_mt, n_nt);
+  } else {
+    if (c_f64)
+      hipLaunchKernelGGL((gemm_b3_nt_kernel<false, double>), grid, dim3(256), 0, s, p, n_mt, n_nt);
+    else
+      hipLaunchKernelGGL((gemm_b3_nt_kernel<false, float>), grid, dim3(256), 0, s, p, n_mt, n_nt);
+  }
+  HIPCHK(hipGetLastError());
+  if (ksplit > 1) {
+    const long total = (long)M * N;
+    hipLaunchKernelGGL(b3_reduce_c128_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       (const float*)ws, ksplit, M, N, (double*)C, ldc, alpha_re, alpha_im, beta);
+    HIPCHK(hipGetLastError());
+  }
+  return TROT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+// 3-pass split-BF16 "NT" GEMM on v_mfma_f32_16x16x32_bf16 (gfx950), FP32 accumulation:
+//
+//   C[m][n] = alpha * sum_k A[m][k] B[n][k]  (+ beta C)        A real or complex, B real
+//
+// Operands arrive pre-split into bf16 planes, x = hi + lo (hi = bf16(x),
+// lo = bf16(x - hi)), each plane K-contiguous with rows padded to a multiple
+// of 128 and K to a multiple of 32 (zeros), so the main loop has no bounds
+// checks and every global load is a 16-byte vector:
+//   A planes: [re_hi, re_lo, im_hi, im_lo] (complex) or [hi, lo], each (Mp, Kp)
+//   B planes: [hi, lo], each (Np, Kp)
+// a.b ~ lo_a.hi_b + hi_a.lo_b + hi_a.hi_b (the lo.lo term, ~2^-16 relative,
+// is dropped).
+//
+// Workgroup 256 threads, tile 128 x 128 x 32; four waves in 2 x 2, each wave
+// 64 x 64 = 4 x 4 MFMA tiles per plane: 96 MFMAs per K step for complex A.
+// The next K chunk is loaded into registers while the current one is
+// multiplied from LDS.  LDS rows are padded to 40 bf16 (80 B): the 16 lanes
+// of a 16-byte fragment read hit 16 disjoint 4-bank groups.
+//
+// Tile order is XCD-aware: workgroup b runs on XCD b % 8, and XCD x is given
+// a contiguous range of tiles ordered row-tile fastest, so consecutive
+// workgroups on one XCD share a B column tile in that XCD's L2.
+#pragma once
+
+#include "common.hpp"
+
+typedef unsigned short bf16raw;
+
+struct B3P {
+  const bf16raw* A;  // planes of (Mp, Kp)
+  const bf16raw* B;  // planes of (Np, Kp)
+  long a_plane, b_plane;  // plane strides (elements)
+  int Kp;                 // padded K (multiple of 32); also the row stride
+  int k_slice;            // K range per blockIdx.y (multiple of 32)
+  int M, N;               // true sizes (output bounds)
+  void* C;                // final output (when part == nullptr)
+  long ldc;
+  float* part;            // split K: raw FP32 partial sums [slice][M][N] (complex if AC), ld = N
+  float ar, ai;
+  float beta;
+};
+
+__device__ __forceinline__ void split_bf16_raw(float x, bf16raw& hi, bf16raw& lo) {
+  const __bf16 h = (__bf16)x;
+  const __bf16 l = (__bf16)(x - (float)h);
+  hi = __builtin_bit_cast(bf16raw, h);
+  lo = __builtin_bit_cast(bf16raw, l);
+}
+
+template <bool AC, typename TC>
+__global__ __launch_bounds__(256, 1) void gemm_b3_nt_kernel(B3P p, int n_mt, int n_nt) {
+  constexpr int BM = 128, BN = 128, BK = 32, LDR = 40;
+  constexpr int PA = AC ? 4 : 2;  // A planes
+  __shared__ __attribute__((aligned(16))) bf16raw As[PA][BM][LDR];
+  __shared__ __attribute__((aligned(16))) bf16raw Bs[2][BN][LDR];
+
+  // XCD-aware tile index
+  const int T = n_mt * n_nt;
+  const int per = (T + 7) / 8;
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per + (b >> 3);
+  if (tile >= T) return;
+  const int mt = tile % n_mt, nt = tile / n_mt;
+  const int m0 = mt * BM, n0 = nt * BN;
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  // staging: A PA planes x 2 segments, B 2 planes x 2 segments (16 B each)
+  uint4 ra[PA][2], rb[2][2];
+  const bf16raw* Ab = p.A + (long)m0 * p.Kp;
+  const bf16raw* Bb = p.B + (long)n0 * p.Kp;
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int seg = t + 256 * s, row = seg >> 2, kq = (seg & 3) * 8;
+      const long off = (long)row * p.Kp + k0 + kq;
+#pragma unroll
+      for (int q = 0; q < PA; ++q) ra[q][s] = *reinterpret_cast<const uint4*>(Ab + q * p.a_plane + off);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) rb[q][s] = *reinterpret_cast<const uint4*>(Bb + q * p.b_plane + off);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int seg = t + 256 * s, row = seg >> 2, kq = (seg & 3) * 8;
+#pragma unroll
+      for (int q = 0; q < PA; ++q) *reinterpret_cast<uint4*>(&As[q][row][kq]) = ra[q][s];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) *reinterpret_cast<uint4*>(&Bs[q][row][kq]) = rb[q][s];
+    }
+  };
+
+  constexpr int PC = AC ? 2 : 1;
+  f4_t acc[PC][4][4];
+#pragma unroll
+  for (int q = 0; q < PC; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[q][i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+  const int kbeg = blockIdx.y * p.k_slice;
+  const int nk = min(p.k_slice, p.Kp - kbeg) / BK;
+  const int ko = 8 * (lane >> 4);
+  if (nk > 0) load(kbeg);
+  for (int kc = 0; kc < nk; ++kc) {
+    __syncthreads();
+    store();
+    __syncthreads();
+    if (kc + 1 < nk) load(kbeg + (kc + 1) * BK);
+    bf16x8_t bh[4], bl[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = wn * 64 + j * 16 + (lane & 15);
+      bh[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[0][col][ko]);
+      bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[1][col][ko]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = wm * 64 + i * 16 + (lane & 15);
+#pragma unroll
+      for (int q = 0; q < PC; ++q) {
+        const bf16x8_t ah = *reinterpret_cast<const bf16x8_t*>(&As[2 * q][row][ko]);
+        const bf16x8_t al = *reinterpret_cast<const bf16x8_t*>(&As[2 * q + 1][row][ko]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          f4_t c = acc[q][i][j];
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], c, 0, 0, 0);
+          acc[q][i][j] = c;
+        }
+      }
+    }
+  }
+
+  // epilogue: D[i = 4 (l / 16) + r][j = l % 16] in register r
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+        const int gj = n0 + wn * 64 + j * 16 + (lane & 15);
+        if (gi >= p.M || gj >= p.N) continue;
+        if (p.part) {  // split K: raw partial sums, reduced by b3_reduce_kernel
+          const long po = ((long)blockIdx.y * p.M + gi) * p.N + gj;
+          if constexpr (AC) {
+            reinterpret_cast<float2*>(p.part)[po] = make_float2(acc[0][i][j][r], acc[1][i][j][r]);
+          } else {
+            p.part[po] = acc[0][i][j][r];
+          }
+          continue;
+        }
+        const long off = (long)gi * p.ldc + gj;
+        if constexpr (AC) {
+          const float xr = acc[0][i][j][r], xi = acc[1][i][j][r];
+          const float vr = p.ar * xr - p.ai * xi, vi = p.ar * xi + p.ai * xr;
+          if constexpr (sizeof(TC) == 8) {
+            double2* c = reinterpret_cast<double2*>(p.C) + off;
+            double2 o = p.beta != 0.f ? *c : make_double2(0.0, 0.0);
+            *c = make_double2((double)vr + (double)p.beta * o.x, (double)vi + (double)p.beta * o.y);
+          } else {
+            float2* c = reinterpret_cast<float2*>(p.C) + off;
+            float2 o = p.beta != 0.f ? *c : make_float2(0.f, 0.f);
+            *c = make_float2(vr + p.beta * o.x, vi + p.beta * o.y);
+          }
+        } else {
+          const float v = p.ar * acc[0][i][j][r];
+          if constexpr (sizeof(TC) == 8) {
+            double* c = reinterpret_cast<double*>(p.C) + off;
+            *c = (double)v + (p.beta != 0.f ? (double)p.beta * *c : 0.0);
+          } else {
+            float* c = reinterpret_cast<float*>(p.C) + off;
+            *c = v + (p.beta != 0.f ? p.beta * *c : 0.f);
+          }
+        }
+      }
+}
+
+// C (complex128, M x N, ldc) = alpha * sum_s part[s] (+ beta C), slices summed in order in FP64.
+__global__ void b3_reduce_c128_kernel(const float* __restrict__ part, int S, int M, int N, double* __restrict__ C,
+                                      long ldc, double ar, double ai, double beta) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)M * N) return;
+  const int i = (int)(e / N), j = (int)(e - (e / N) * N);
+  double sr = 0.0, si = 0.0;
+  for (int s = 0; s < S; ++s) {
+    const float2 v = reinterpret_cast<const float2*>(part)[(long)s * M * N + e];
+    sr += v.x;
+    si += v.y;
+  }
+  double2* c = reinterpret_cast<double2*>(C) + (long)i * ldc + j;
+  const double2 o = beta != 0.0 ? *c : make_double2(0.0, 0.0);
+  *c = make_double2(ar * sr - ai * si + beta * o.x, ar * si + ai * sr + beta * o.y);
+}
+
+// Split a row-major FP64 matrix (rows x K, leading dim ld; complex if CX) into bf16 planes of
+// (rows_pad, Kp), zero padded.  Planes: [re_hi, re_lo, im_hi, im_lo] or [hi, lo].
+// TRANS: element (r, k) of the source sits at k * ld + r (column-major source).
+template <bool CX, bool TRANS, typename TS>
+__global__ void split_planes_kernel(const TS* __restrict__ src, long ld, int rows, int K, int rows_pad, int Kp,
+                                    bf16raw* __restrict__ dst) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)rows_pad * Kp;
+  if (e >= total) return;
+  const int r = (int)(e / Kp), k = (int)(e - (e / Kp) * Kp);
+  const bool ok = r < rows && k < K;
+  float vr = 0.f, vi = 0.f;
+  if (ok) {
+    const long so = TRANS ? (long)k * ld + r : (long)r * ld + k;
+    if (CX) {
+      vr = (float)src[2 * so];
+      vi = (float)src[2 * so + 1];
+    } else {
+      vr = (float)src[so];
+    }
+  }
+  bf16raw h, l;
+  split_bf16_raw(vr, h, l);
+  dst[e] = h;
+  dst[total + e] = l;
+  if (CX) {
+    split_bf16_raw(vi, h, l);
+    dst[2 * total + e] = h;
+    dst[3 * total + e] = l;
+  }
+}

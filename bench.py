@@ -146,7 +146,7 @@ def main():
     mixed = Runner(env, args.precision, args.taylor_precision)
     st_mix = mixed.init()
     torch.cuda.synchronize()
-    log(f"mixed contexts ready at {time.perf_counter() - t0:.1f}s (sym-block VHS: {'chol_symblk' in mixed.pctx.low})")
+    log(f"mixed contexts ready at {time.perf_counter() - t0:.1f}s (sym-block VHS: {bool(mixed.pctx.low.get('symblk'))})")
     st_mix, t_mix, vhs_ms = mixed.timed(st_mix, args.warmup, args.steps, f"mixed {args.precision}")
     value = W * args.steps / t_mix
 
@@ -171,7 +171,7 @@ def main():
         st_mix = mixed.init()
 
     # roofline of the dominant kernel (VHS GEMM of the mixed step): flops the algorithm run here needs
-    sym = "chol_symblk" in mixed.pctx.low
+    sym = bool(mixed.pctx.low.get("symblk"))
     cols = _lib.symblk_cols(N) if sym else N * N
     passes = 3 if args.precision == "bf16x3" else 1
     vhs_flop = 4.0 * W * G * cols  # real x complex MAC = 4 flop over the stored V entries

@@ -84,6 +84,30 @@ def test_force_bias_and_energy_lp(gpu, prec, kind):
     assert rel(e - p["h0"], e_ref - p["h0"]) < 4 * TOL[prec]
 
 
+@pytest.mark.parametrize("shape", [(128, 128, 64, 1), (130, 70, 33, 1), (5, 300, 1000, 4), (257, 129, 96, 3)])
+@pytest.mark.parametrize("cx", [True, False])
+def test_split_bf16_nt_gemm(gpu, shape, cx):
+    """trot_gemm_b3 against a float64 product, ragged sizes and split K."""
+    from trot_amd import _b3
+
+    M, N, K, ks = shape
+    rng = np.random.default_rng(M + N + K)
+    a = rng.standard_normal((M, K)) + (1j * rng.standard_normal((M, K)) if cx else 0)
+    b = rng.standard_normal((N, K))
+    ref = 0.7 * (a @ b.T)
+    ap, bp = _b3.split(_t(a)), _b3.split(_t(b.T.copy()), transposed=True)
+    for dt in ((torch.complex128, torch.complex64) if cx else (torch.float64, torch.float32)):
+        if ks > 1 and dt != torch.complex128:
+            continue
+        out = torch.zeros((M, N), dtype=dt, device="cuda")
+        _b3.gemm_nt(ap, bp, out, alpha=0.7, ksplit=ks)
+        assert rel(out.cpu().numpy(), ref) < TOL["bf16x3"]
+    if cx:  # beta accumulation and complex alpha
+        out = torch.from_numpy(ref.copy()).cuda()
+        _b3.gemm_nt(ap, bp, out, alpha=0.5j, beta=1.0, ksplit=ks)
+        assert rel(out.cpu().numpy(), ref + 0.5j * (a @ b.T)) < TOL["bf16x3"]
+
+
 @pytest.mark.parametrize("prec,tprec", [("f32", "f32"), ("bf16x3", "f32"), ("bf16x3", "bf16x3"), ("f32", "f64")])
 def test_mixed_steps_track_fp64(gpu, prec, tprec):
     """A few mixed-precision steps stay close to the FP64 oracle with identical fields."""

@@ -66,6 +66,12 @@ _SIGS = {
     "trot_pack_symblk": (_i, [_vp, _i, _i, _vp, _i, _vp]),
     "trot_taylor_apply_symblk": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
     "trot_taylor_apply_lp_symblk": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _i, _vp, _sz, _vp]),
+    "trot_b3_rows_pad": (_l, [_i]),
+    "trot_b3_k_pad": (_l, [_i]),
+    "trot_b3_planes_bytes": (_sz, [_i, _i, _i]),
+    "trot_split_bf16": (_i, [_vp, _i, _i, _i, _l, _i, _i, _vp, _vp]),
+    "trot_gemm_b3_workspace": (_sz, [_i, _i, _i, _i]),
+    "trot_gemm_b3": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _i, _l, _d, _d, _d, _i, _vp, _sz, _vp]),
 }
 
 PREC = {"f64": 0, "f32": 1, "bf16x3": 2}

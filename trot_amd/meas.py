@@ -16,7 +16,7 @@ from dataclasses import dataclass, field
 
 import torch
 
-from . import _lib
+from . import _b3, _lib
 from .core import MeasOps, System, k_energy, k_force_bias
 from .trial import RhfTrial, UhfTrial, _batched, overlap, overlap_green
 
@@ -161,6 +161,20 @@ class Engine:
                 fb = _lib.force_bias(greens[1], sb.rot_chol_t, 1.0, out=fb, accumulate=True)
             return fb
         p = self.precision
+        if p == "bf16x3":
+            # NT GEMM on split-bf16 planes: A = GT (W x N n, complex), B = R' (G x N n), split K
+            W = greens[0].shape[0]
+            G = sa.rot_chol_t.shape[0]
+            fb = torch.empty((W, G), dtype=torch.complex128, device=greens[0].device)
+            for i, (g, s) in enumerate(((greens[0], sa), (greens[1], sb))):
+                if i == 1 and self.kind == "rr":
+                    break
+                if "rot_chol_t_b3" not in s.low:
+                    s.low["rot_chol_t_b3"] = _b3.split(s.rot_chol_t.reshape(G, -1))
+                gp = _b3.split(g.reshape(W, -1))
+                _b3.gemm_nt(gp, s.low["rot_chol_t_b3"], fb, alpha if i == 0 else 1.0, 0.0 if i == 0 else 1.0,
+                            ksplit=8)
+            return fb
         fb = _lib.force_bias_lp(greens[0], self._f32(sa, "rot_chol_t"), alpha, p)
         if self.kind != "rr":
             fb = _lib.force_bias_lp(greens[1], self._f32(sb, "rot_chol_t"), 1.0, p, out=fb, accumulate=True)

@@ -26,7 +26,7 @@ import numpy as np
 import scipy.linalg
 import torch
 
-from . import _lib
+from . import _b3, _lib
 from .core import MeasOps, System, TrialOps, k_energy
 from . import walkers as wk
 
@@ -181,7 +181,13 @@ def build_prop_ctx(ham_data, rdm1, dt: float, precision: str = "f64", walker_chu
     symmetric = bool(torch.equal(chol, chol.transpose(1, 2)))
     if vhs_layout == "symblk" and not symmetric:
         raise ValueError("vhs_layout='symblk' needs symmetric Cholesky vectors")
-    if vhs_layout != "dense" and symmetric:
+    ctx.low["symblk"] = vhs_layout != "dense" and symmetric
+    if precision == "bf16x3":
+        # VHS as an NT GEMM on split-bf16 planes: B = (chol columns) x G, K = G contiguous
+        src = _lib.pack_symblk(chol) if ctx.low["symblk"] else ctx.chol_flat
+        ctx.low["chol_b3"] = _b3.split(src, transposed=True)
+        del src
+    elif ctx.low["symblk"]:
         # V = sum_g xbar_g L_g is symmetric: build only its upper 8x8 blocks
         ctx.low["chol_symblk"] = _lib.pack_symblk(chol, out_f32=precision != "f64")
     elif precision != "f64":
@@ -266,7 +272,16 @@ class KernelTimer:
 def _vhs(ctx: CholAfqmcCtx, xbar: torch.Tensor) -> torch.Tensor:
     """VHS in the context's precision; complex64 when the Taylor series runs in reduced precision."""
     chol = ctx.low.get("chol_symblk")  # sym-block storage when every L_g is symmetric
-    if ctx.precision == "f64":
+    if ctx.precision == "bf16x3":
+        planes = ctx.low["chol_b3"]
+        dtype = torch.complex128 if ctx.taylor_precision == "f64" else torch.complex64
+
+        def fn(x):
+            out = torch.empty((x.shape[0], planes.rows), dtype=dtype, device=x.device)
+            return _b3.gemm_nt(_b3.split(x), planes, out)
+
+        args = (xbar,)
+    elif ctx.precision == "f64":
         fn, args = _lib.vhs_build, (xbar, ctx.chol_flat if chol is None else chol)
     else:
         chol = ctx.low["chol_flat32"] if chol is None else chol
@@ -278,7 +293,7 @@ def _vhs(ctx: CholAfqmcCtx, xbar: torch.Tensor) -> torch.Tensor:
 
 
 def _taylor(ctx: CholAfqmcCtx, v: torch.Tensor, t: torch.Tensor, n_terms: int) -> torch.Tensor:
-    sym = "chol_symblk" in ctx.low
+    sym = ctx.low.get("symblk", False)
     if ctx.taylor_precision == "f64":
         if v.dtype != torch.complex128:
             raise ValueError("an FP64 Taylor series needs a complex128 VHS")

@@ -57,7 +57,21 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (profiling runs)")
     ap.add_argument("--cpu-walkers", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 3 device copies of 1 GiB (scale reference for rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def calibrate_copy() -> None:
+    """Three copies of a 1 GiB float64 tensor: 1 GiB read + 1 GiB written per copy kernel."""
+    import torch
+
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8, "copies": 3}))
 
 
 def log(msg: str) -> None:
@@ -126,6 +140,9 @@ def main():
     args = parse()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (--gpus must be 1)")
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     import torch
 
     from trot_amd import _lib, meas, prop, synthetic, trial

@@ -25,6 +25,7 @@
 #include "common.hpp"
 
 typedef unsigned short bf16raw;
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 struct B3P {
   const bf16raw* A;  // planes of (Mp, Kp)
@@ -66,31 +67,37 @@ __global__ __launch_bounds__(256, 1) void gemm_b3_nt_kernel(B3P p, int n_mt, int
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  // staging: A PA planes x 2 segments, B 2 planes x 2 segments (16 B each)
-  uint4 ra[PA][2], rb[2][2];
+  // staging: A PA planes x 2 segments, B 2 planes x 2 segments (16 B each), flat so that every
+  // index is a compile-time constant after unrolling (the array then lives in VGPRs, not scratch)
+  constexpr int NA = PA * 2, NB = 4;
+  u32x4_t stg[NA + NB];
   const bf16raw* Ab = p.A + (long)m0 * p.Kp;
   const bf16raw* Bb = p.B + (long)n0 * p.Kp;
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int seg = t + 256 * s, row = seg >> 2, kq = (seg & 3) * 8;
-      const long off = (long)row * p.Kp + k0 + kq;
-#pragma unroll
-      for (int q = 0; q < PA; ++q) ra[q][s] = *reinterpret_cast<const uint4*>(Ab + q * p.a_plane + off);
-#pragma unroll
-      for (int q = 0; q < 2; ++q) rb[q][s] = *reinterpret_cast<const uint4*>(Bb + q * p.b_plane + off);
-    }
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int seg = t + 256 * s, row = seg >> 2, kq = (seg & 3) * 8;
-#pragma unroll
-      for (int q = 0; q < PA; ++q) *reinterpret_cast<uint4*>(&As[q][row][kq]) = ra[q][s];
-#pragma unroll
-      for (int q = 0; q < 2; ++q) *reinterpret_cast<uint4*>(&Bs[q][row][kq]) = rb[q][s];
-    }
-  };
+  const int row0 = t >> 2, kq = (t & 3) * 8;  // segment t; segment t + 256 is row0 + 64
+#define B3_LOAD(k0)                                                                              \
+  do {                                                                                           \
+    const long off0 = (long)row0 * p.Kp + (k0) + kq;                                             \
+    const long off1 = off0 + 64L * p.Kp;                                                         \
+    _Pragma("unroll") for (int q = 0; q < PA; ++q) {                                             \
+      stg[2 * q] = *reinterpret_cast<const u32x4_t*>(Ab + q * p.a_plane + off0);                   \
+      stg[2 * q + 1] = *reinterpret_cast<const u32x4_t*>(Ab + q * p.a_plane + off1);               \
+    }                                                                                            \
+    _Pragma("unroll") for (int q = 0; q < 2; ++q) {                                              \
+      stg[NA + 2 * q] = *reinterpret_cast<const u32x4_t*>(Bb + q * p.b_plane + off0);              \
+      stg[NA + 2 * q + 1] = *reinterpret_cast<const u32x4_t*>(Bb + q * p.b_plane + off1);          \
+    }                                                                                            \
+  } while (0)
+#define B3_STORE()                                                                               \
+  do {                                                                                           \
+    _Pragma("unroll") for (int q = 0; q < PA; ++q) {                                             \
+      *reinterpret_cast<u32x4_t*>(&As[q][row0][kq]) = stg[2 * q];                                  \
+      *reinterpret_cast<u32x4_t*>(&As[q][row0 + 64][kq]) = stg[2 * q + 1];                         \
+    }                                                                                            \
+    _Pragma("unroll") for (int q = 0; q < 2; ++q) {                                              \
+      *reinterpret_cast<u32x4_t*>(&Bs[q][row0][kq]) = stg[NA + 2 * q];                             \
+      *reinterpret_cast<u32x4_t*>(&Bs[q][row0 + 64][kq]) = stg[NA + 2 * q + 1];                    \
+    }                                                                                            \
+  } while (0)
 
   constexpr int PC = AC ? 2 : 1;
   f4_t acc[PC][4][4];
@@ -104,12 +111,12 @@ __global__ __launch_bounds__(256, 1) void gemm_b3_nt_kernel(B3P p, int n_mt, int
   const int kbeg = blockIdx.y * p.k_slice;
   const int nk = min(p.k_slice, p.Kp - kbeg) / BK;
   const int ko = 8 * (lane >> 4);
-  if (nk > 0) load(kbeg);
+  if (nk > 0) B3_LOAD(kbeg);
   for (int kc = 0; kc < nk; ++kc) {
     __syncthreads();
-    store();
+    B3_STORE();
     __syncthreads();
-    if (kc + 1 < nk) load(kbeg + (kc + 1) * BK);
+    if (kc + 1 < nk) B3_LOAD(kbeg + (kc + 1) * BK);
     bf16x8_t bh[4], bl[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {

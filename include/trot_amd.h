@@ -172,6 +172,15 @@ int trot_gemm_b3(const void* a_planes, int a_complex, const void* b_planes, int 
                  long ldc, double alpha_re, double alpha_im, double beta, int ksplit, void* ws, size_t ws_bytes,
                  void* stream);
 
+/* Taylor series with every product on split-BF16 MFMA and the running sum in complex128: V (complex64,
+ * sym-block storage, (W, trot_symblk_cols(N))) is expanded once into dense bf16 hi/lo planes, the terms
+ * are kept as split planes of T^T.  Workspace trot_taylor_b3_workspace(W, N, n).
+ * Replaces: taylor_expm_action (trot/prop/utils.py:20-29) in _apply_two_body_array
+ * (trot/prop/chol_afqmc_ops.py:152-162). */
+size_t trot_taylor_b3_workspace(int W, int N, int n);
+int trot_taylor_apply_b3_symblk(const float* vhs_symblk, double* w, int W, int N, int n, double sqrt_dt, int n_terms,
+                                void* ws, size_t ws_bytes, void* stream);
+
 /* ---- reduced-precision forms (prec = TROT_PREC_F32 or TROT_PREC_BF16X3) ----
  * Cholesky-side operands are FP32 copies made at context build; FP64 walker-side operands are rounded
  * (and split) as they are staged into LDS; accumulation is FP32 on MFMA.  The reference's own mixed mode
@@ -187,6 +196,11 @@ int trot_vhs_build_lp(const double* xbar, const float* chol_flat, int W, int G, 
  * Replaces: force_bias_kernel_rw_rh / _uw_rh (trot/meas/rhf.py:119-153; trot/meas/uhf.py:43-57). */
 int trot_force_bias_lp(const double* green_t, const float* rot_chol_t, int W, int G, long Nn, double alpha,
                        int accumulate, double* fb, int prec, void* stream);
+
+/* As trot_one_body_apply with the propagator in FP32 (FP32 or split-BF16 MFMA); walkers stay complex128.
+ * Replaces: _apply_one_body_half_array (trot/prop/chol_afqmc_ops.py:130-131). */
+int trot_one_body_apply_lp(const float* expm32, const double* w_in, int W, int N, int n, double* w_out, int prec,
+                           void* stream);
 
 /* As trot_taylor_apply with V complex64 and complex64 terms; the running sum w stays complex128.
  * Replaces: taylor_expm_action (trot/prop/utils.py:20-29). */

@@ -108,6 +108,26 @@ def test_split_bf16_nt_gemm(gpu, shape, cx):
         assert rel(out.cpu().numpy(), ref + 0.5j * (a @ b.T)) < TOL["bf16x3"]
 
 
+@pytest.mark.parametrize("shape", [(64, 16, 30, 7), (37, 5, 21, 3), (130, 64, 40, 4), (512, 64, 16, 2)])
+def test_taylor_b3_symblk(gpu, shape):
+    """Split-BF16 Taylor (dense bf16 V planes expanded from sym-block complex64) against the oracle."""
+    from trot_amd import _lib
+
+    N, n, G, W = shape
+    p = problem(N, n, G, W, seed=35)
+    rng = np.random.default_rng(6)
+    xbar = rng.standard_normal((W, G)) + 1j * rng.standard_normal((W, G))
+    v_ref = O.make_vhs(xbar, p["chol"].reshape(G, -1), N)
+    blk32 = _lib.pack_symblk(_t(p["chol"]), out_f32=True)
+    v32 = _lib.vhs_build_lp(_t(xbar), blk32, "f32")
+    sdt = 0.1
+    ref = O.taylor_expm_action(1j * sdt, v_ref, p["wa"], 6)
+    out = _lib.taylor_apply_b3_symblk(v32, _t(p["wa"]), sdt, 6)
+    assert rel(out.cpu().numpy() - p["wa"], ref - p["wa"]) < 4 * TOL["bf16x3"]
+    out0 = _lib.taylor_apply_b3_symblk(v32, _t(p["wa"]), sdt, 0)
+    assert np.array_equal(out0.cpu().numpy(), p["wa"])
+
+
 @pytest.mark.parametrize("prec,tprec", [("f32", "f32"), ("bf16x3", "f32"), ("bf16x3", "bf16x3"), ("f32", "f64")])
 def test_mixed_steps_track_fp64(gpu, prec, tprec):
     """A few mixed-precision steps stay close to the FP64 oracle with identical fields."""

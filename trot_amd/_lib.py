@@ -66,6 +66,9 @@ _SIGS = {
     "trot_pack_symblk": (_i, [_vp, _i, _i, _vp, _i, _vp]),
     "trot_taylor_apply_symblk": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
     "trot_taylor_apply_lp_symblk": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _i, _vp, _sz, _vp]),
+    "trot_one_body_apply_lp": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _vp]),
+    "trot_taylor_b3_workspace": (_sz, [_i, _i, _i]),
+    "trot_taylor_apply_b3_symblk": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
     "trot_b3_rows_pad": (_l, [_i]),
     "trot_b3_k_pad": (_l, [_i]),
     "trot_b3_planes_bytes": (_sz, [_i, _i, _i]),
@@ -401,6 +404,32 @@ def force_bias_lp(green_t: torch.Tensor, rot_chol_t32: torch.Tensor, alpha: floa
     _req(out, "fb", C128, (W, G))
     _check(lib().trot_force_bias_lp(_p(green_t), _p(rot_chol_t32), W, G, N * n, float(alpha), int(accumulate),
                                     _p(out), _prec(prec), _stream()), "trot_force_bias_lp")
+    return out
+
+
+def one_body_apply_lp(expm32: torch.Tensor, w: torch.Tensor, prec: str) -> torch.Tensor:
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    _req(expm32, "exp_h1_half (f32)", F32, (N, N))
+    out = torch.empty_like(w)
+    _check(lib().trot_one_body_apply_lp(_p(expm32), _p(w), W, N, n, _p(out), _prec(prec), _stream()),
+           "trot_one_body_apply_lp")
+    return out
+
+
+def taylor_apply_b3_symblk(vpk32: torch.Tensor, w: torch.Tensor, sqrt_dt: float, n_terms: int,
+                           inplace: bool = False) -> torch.Tensor:
+    """Taylor series with split-BF16 products and a complex128 running sum; V complex64 in sym-block storage."""
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    _req(vpk32, "vhs (sym-block, complex64)", C64, (W, symblk_cols(N)))
+    if W > 65535:
+        raise ValueError("taylor_apply_b3_symblk: at most 65535 walkers per call")
+    out = w if inplace else w.clone()
+    L = lib()
+    ws = _ws(L.trot_taylor_b3_workspace(W, N, n), w.device)
+    _check(L.trot_taylor_apply_b3_symblk(_p(vpk32), _p(out), W, N, n, float(sqrt_dt), int(n_terms), _p(ws),
+                                         ws.numel(), _stream()), "trot_taylor_apply_b3_symblk")
     return out
 
 

@@ -66,6 +66,22 @@ int trot_force_bias_lp(const double* green_t, const float* rot_chol_t, int W, in
   return TROT_OK;
 }
 
+// w_out[b] (complex128) = expm32 (N x N, f32) . w_in[b] (complex128), FP32 / split-BF16 MFMA
+int trot_one_body_apply_lp(const float* expm32, const double* w_in, int W, int N, int n, double* w_out, int prec,
+                           void* stream) {
+  if (!expm32 || !w_in || !w_out) return fail(TROT_ERR_INVALID, "one_body_apply_lp: null pointer");
+  if (w_in == w_out) return fail(TROT_ERR_INVALID, "one_body_apply_lp: in-place not supported");
+  if (W < 1 || N < 1 || n < 1) return fail(TROT_ERR_INVALID, "one_body_apply_lp: bad sizes");
+  GemmLPP<float, double, double> p{};
+  p.A = expm32; p.lda = N; p.sA = 0;
+  p.B = w_in; p.ldb = n; p.sB = (long)N * n;
+  p.C = w_out; p.ldc = n; p.sC = (long)N * n;
+  p.M = N; p.N = n; p.K = N; p.ar = 1.f;
+  DISPATCH_MODE(prec, HIPCHK((launch_gemm_lp<64, 64, MODE, false, true, float, double, double, true, true, false, 0>(
+                              p, W, (hipStream_t)stream))));
+  return TROT_OK;
+}
+
 size_t trot_taylor_lp_workspace(int W, int N, int n) { return 2 * align256((size_t)W * N * n * 8); }
 
 // w (complex128, in/out) <- sum_k (i sqrt(dt) V)^k / k! w with V complex64 and the terms in complex64;

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256, 1) void gemm_b3_nt_kernel(B3P p, int n_mt, int
 }
 
 // C (complex128, M x N, ldc) = alpha * sum_s part[s] (+ beta C), slices summed in order in FP64.
-__global__ void b3_reduce_c128_kernel(const float* __restrict__ part, int S, int M, int N, double* __restrict__ C,
+static __global__ void b3_reduce_c128_kernel(const float* __restrict__ part, int S, int M, int N, double* __restrict__ C,
                                       long ldc, double ar, double ai, double beta) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)M * N) return;

@@ -292,6 +292,15 @@ def _vhs(ctx: CholAfqmcCtx, xbar: torch.Tensor) -> torch.Tensor:
     return fn(*args)
 
 
+def _one_body(ctx: CholAfqmcCtx, w: torch.Tensor) -> torch.Tensor:
+    """exp(-dt/2 h1_eff) w: FP64 MFMA, or FP32 MFMA when the Taylor terms run in reduced precision."""
+    if ctx.taylor_precision == "f64":
+        return _lib.one_body_apply(ctx.exp_h1_half, w)
+    if "exp32" not in ctx.low:
+        ctx.low["exp32"] = ctx.exp_h1_half.to(torch.float32).contiguous()
+    return _lib.one_body_apply_lp(ctx.low["exp32"], w, "f32")
+
+
 def _taylor(ctx: CholAfqmcCtx, v: torch.Tensor, t: torch.Tensor, n_terms: int) -> torch.Tensor:
     sym = ctx.low.get("symblk", False)
     if ctx.taylor_precision == "f64":
@@ -302,6 +311,8 @@ def _taylor(ctx: CholAfqmcCtx, v: torch.Tensor, t: torch.Tensor, n_terms: int) -
         return _lib.taylor_apply(v.view(t.shape[0], ctx.norb, ctx.norb), t, ctx.sqrt_dt, n_terms, inplace=True)
     if v.dtype != torch.complex64:
         raise ValueError("a reduced-precision Taylor series needs a complex64 VHS")
+    if sym and ctx.taylor_precision == "bf16x3":
+        return _lib.taylor_apply_b3_symblk(v, t, ctx.sqrt_dt, n_terms, inplace=True)
     if sym:
         return _lib.taylor_apply_lp_symblk(v, t, ctx.sqrt_dt, n_terms, ctx.taylor_precision, inplace=True)
     return _lib.taylor_apply_lp(v.view(t.shape[0], ctx.norb, ctx.norb), t, ctx.sqrt_dt, n_terms,
@@ -324,9 +335,9 @@ def apply_trotter(walkers, xbar: torch.Tensor, ctx: CholAfqmcCtx, n_terms: int):
         w1 = min(W, w0 + chunk)
         v = _vhs(ctx, xbar[w0:w1])
         for i, s in enumerate(spins):
-            t = _lib.one_body_apply(ctx.exp_h1_half, s[w0:w1])
+            t = _one_body(ctx, s[w0:w1])
             t = _taylor(ctx, v, t, n_terms)
-            r = _lib.one_body_apply(ctx.exp_h1_half, t)
+            r = _one_body(ctx, t)
             if chunk == W:
                 outs[i] = r
             else:

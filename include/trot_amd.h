@@ -172,14 +172,28 @@ int trot_gemm_b3(const void* a_planes, int a_complex, const void* b_planes, int 
                  long ldc, double alpha_re, double alpha_im, double beta, int ksplit, void* ws, size_t ws_bytes,
                  void* stream);
 
-/* Taylor series with every product on split-BF16 MFMA and the running sum in complex128: V (complex64,
- * sym-block storage, (W, trot_symblk_cols(N))) is expanded once into dense bf16 hi/lo planes, the terms
- * are kept as split planes of T^T.  Workspace trot_taylor_b3_workspace(W, N, n).
+/* Taylor series with every product on split-BF16 MFMA: V (complex64, sym-block storage,
+ * (W, trot_symblk_cols(N))) is read as stored and split into bf16 hi/lo planes in LDS; the terms travel
+ * as split planes of T^T, are summed in FP32 and added to w (complex128, in place) in FP64.
+ * Workspace trot_taylor_b3_workspace(W, N, n).
  * Replaces: taylor_expm_action (trot/prop/utils.py:20-29) in _apply_two_body_array
  * (trot/prop/chol_afqmc_ops.py:152-162). */
 size_t trot_taylor_b3_workspace(int W, int N, int n);
 int trot_taylor_apply_b3_symblk(const float* vhs_symblk, double* w, int W, int N, int n, double sqrt_dt, int n_terms,
                                 void* ws, size_t ws_bytes, void* stream);
+
+/* (E - I) for E = exp(-dt/2 h1_eff) (N x N, FP64) as split bf16 planes, trot_onebody_b3_planes_bytes(N) bytes;
+ * built once per propagation context (exp_h1_half, trot/prop/chol_afqmc_ops.py:116). */
+size_t trot_onebody_b3_planes_bytes(int N);
+int trot_onebody_b3_planes(const double* expm, int N, void* planes, void* stream);
+
+/* One spin set: w_out <- E . taylor(i sqrt(dt), V, E . w_in, K) with every product on split-BF16 MFMA; each
+ * one-body factor is applied as w + (E - I) w with the identity term exact in FP64.  w_out may equal w_in.
+ * Workspace trot_taylor_b3_workspace(W, N, n).
+ * Replaces: the per-spin body of _apply_trotter_r / _apply_trotter_u (trot/prop/chol_afqmc_ops.py:198-222):
+ * _apply_one_body_half_array (:130-131), _apply_two_body_array (:152-162), _apply_one_body_half_array. */
+int trot_trotter_b3_symblk(const float* vhs_symblk, const void* onebody_planes, const double* w_in, double* w_out,
+                           int W, int N, int n, double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, void* stream);
 
 /* ---- reduced-precision forms (prec = TROT_PREC_F32 or TROT_PREC_BF16X3) ----
  * Cholesky-side operands are FP32 copies made at context build; FP64 walker-side operands are rounded

@@ -108,24 +108,65 @@ def test_split_bf16_nt_gemm(gpu, shape, cx):
         assert rel(out.cpu().numpy(), ref + 0.5j * (a @ b.T)) < TOL["bf16x3"]
 
 
-@pytest.mark.parametrize("shape", [(64, 16, 30, 7), (37, 5, 21, 3), (130, 64, 40, 4), (512, 64, 16, 2)])
-def test_taylor_b3_symblk(gpu, shape):
-    """Split-BF16 Taylor (dense bf16 V planes expanded from sym-block complex64) against the oracle."""
+B3_SHAPES = [(64, 16, 30, 7), (37, 5, 21, 3), (130, 64, 40, 4), (512, 64, 16, 2), (96, 70, 20, 9)]
+
+
+def _b3_case(shape, seed):
     from trot_amd import _lib
 
     N, n, G, W = shape
-    p = problem(N, n, G, W, seed=35)
-    rng = np.random.default_rng(6)
+    p = problem(N, n, G, W, seed=seed)
+    rng = np.random.default_rng(seed + 1)
     xbar = rng.standard_normal((W, G)) + 1j * rng.standard_normal((W, G))
     v_ref = O.make_vhs(xbar, p["chol"].reshape(G, -1), N)
     blk32 = _lib.pack_symblk(_t(p["chol"]), out_f32=True)
     v32 = _lib.vhs_build_lp(_t(xbar), blk32, "f32")
+    w = rng.standard_normal((W, N, n)) + 1j * rng.standard_normal((W, N, n))
+    return p, v_ref, v32, w
+
+
+@pytest.mark.parametrize("shape", B3_SHAPES)
+@pytest.mark.parametrize("n_terms", [1, 2, 6])
+def test_taylor_b3_symblk(gpu, shape, n_terms):
+    """Split-BF16 Taylor (V read from sym-block complex64, FP32 term sum) against the oracle.
+
+    Ragged N (37, 130), n above one 64-column tile (70), walker counts that are not a
+    multiple of 8, and 1 / 2 / 6 terms (first-and-last, first-then-last, middle terms).
+    """
+    from trot_amd import _lib
+
+    _, v_ref, v32, w = _b3_case(shape, 35)
     sdt = 0.1
-    ref = O.taylor_expm_action(1j * sdt, v_ref, p["wa"], 6)
-    out = _lib.taylor_apply_b3_symblk(v32, _t(p["wa"]), sdt, 6)
-    assert rel(out.cpu().numpy() - p["wa"], ref - p["wa"]) < 4 * TOL["bf16x3"]
-    out0 = _lib.taylor_apply_b3_symblk(v32, _t(p["wa"]), sdt, 0)
-    assert np.array_equal(out0.cpu().numpy(), p["wa"])
+    ref = O.taylor_expm_action(1j * sdt, v_ref, w, n_terms)
+    out = _lib.taylor_apply_b3_symblk(v32, _t(w), sdt, n_terms)
+    # the identity term is exact; compare the propagated change against its own size
+    assert rel(out.cpu().numpy() - w, ref - w) < 4 * TOL["bf16x3"]
+    out0 = _lib.taylor_apply_b3_symblk(v32, _t(w), sdt, 0)
+    assert np.array_equal(out0.cpu().numpy(), w)
+
+
+@pytest.mark.parametrize("shape", B3_SHAPES)
+@pytest.mark.parametrize("n_terms", [0, 1, 6])
+def test_trotter_b3_symblk(gpu, shape, n_terms):
+    """E . taylor(E w) on split-BF16 MFMA against the oracle's FP64 products, in and out of place."""
+    import scipy.linalg
+    from trot_amd import _lib
+
+    N = shape[0]
+    p, v_ref, v32, w = _b3_case(shape, 36)
+    E = scipy.linalg.expm(-0.05 * p["h1"])
+    sdt = 0.1
+    ref = np.einsum("ij,wjp->wip", E, O.taylor_expm_action(1j * sdt, v_ref, np.einsum("ij,wjp->wip", E, w), n_terms))
+    planes = _lib.onebody_b3_planes(_t(E))
+    wt = _t(w)
+    out = _lib.trotter_b3_symblk(v32, wt, planes, sdt, n_terms)
+    assert np.array_equal(wt.cpu().numpy(), w)  # input untouched out of place
+    assert rel(out.cpu().numpy() - w, ref - w) < 4 * TOL["bf16x3"]
+    out2 = _lib.trotter_b3_symblk(v32, wt, planes, sdt, n_terms, out=wt)
+    assert out2.data_ptr() == wt.data_ptr()
+    assert np.array_equal(out2.cpu().numpy(), out.cpu().numpy())
+    with pytest.raises(ValueError):
+        _lib.trotter_b3_symblk(v32, wt, _lib.onebody_b3_planes(_t(np.eye(N + 128))), sdt, n_terms)
 
 
 @pytest.mark.parametrize("prec,tprec", [("f32", "f32"), ("bf16x3", "f32"), ("bf16x3", "bf16x3"), ("f32", "f64")])

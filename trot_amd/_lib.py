@@ -69,6 +69,9 @@ _SIGS = {
     "trot_one_body_apply_lp": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _vp]),
     "trot_taylor_b3_workspace": (_sz, [_i, _i, _i]),
     "trot_taylor_apply_b3_symblk": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
+    "trot_onebody_b3_planes_bytes": (_sz, [_i]),
+    "trot_onebody_b3_planes": (_i, [_vp, _i, _vp, _vp]),
+    "trot_trotter_b3_symblk": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
     "trot_b3_rows_pad": (_l, [_i]),
     "trot_b3_k_pad": (_l, [_i]),
     "trot_b3_planes_bytes": (_sz, [_i, _i, _i]),
@@ -423,13 +426,40 @@ def taylor_apply_b3_symblk(vpk32: torch.Tensor, w: torch.Tensor, sqrt_dt: float,
     W, N, n = w.shape
     _req(w, "walkers", C128, (W, N, n))
     _req(vpk32, "vhs (sym-block, complex64)", C64, (W, symblk_cols(N)))
-    if W > 65535:
-        raise ValueError("taylor_apply_b3_symblk: at most 65535 walkers per call")
     out = w if inplace else w.clone()
     L = lib()
     ws = _ws(L.trot_taylor_b3_workspace(W, N, n), w.device)
     _check(L.trot_taylor_apply_b3_symblk(_p(vpk32), _p(out), W, N, n, float(sqrt_dt), int(n_terms), _p(ws),
                                          ws.numel(), _stream()), "trot_taylor_apply_b3_symblk")
+    return out
+
+
+def onebody_b3_planes(expm: torch.Tensor) -> torch.Tensor:
+    """Split bf16 planes of (expm - I) for trotter_b3_symblk (raw bytes, int16 storage)."""
+    N = expm.shape[0]
+    _req(expm, "expm", F64, (N, N))
+    L = lib()
+    out = torch.empty(L.trot_onebody_b3_planes_bytes(N) // 2, dtype=torch.int16, device=expm.device)
+    _check(L.trot_onebody_b3_planes(_p(expm), N, _p(out), _stream()), "trot_onebody_b3_planes")
+    return out
+
+
+def trotter_b3_symblk(vpk32: torch.Tensor, w: torch.Tensor, onebody_planes: torch.Tensor, sqrt_dt: float,
+                      n_terms: int, out: torch.Tensor | None = None) -> torch.Tensor:
+    """E . taylor(i sqrt(dt) V, E . w, K) on split-BF16 MFMA; ``out`` may be ``w`` (in place)."""
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    _req(vpk32, "vhs (sym-block, complex64)", C64, (W, symblk_cols(N)))
+    L = lib()
+    if onebody_planes.dtype != torch.int16 or onebody_planes.numel() * 2 != L.trot_onebody_b3_planes_bytes(N):
+        raise ValueError("trotter_b3_symblk: onebody_planes must come from onebody_b3_planes for this N")
+    if out is None:
+        out = torch.empty_like(w)
+    else:
+        _req(out, "out", C128, (W, N, n))
+    ws = _ws(L.trot_taylor_b3_workspace(W, N, n), w.device)
+    _check(L.trot_trotter_b3_symblk(_p(vpk32), _p(onebody_planes), _p(w), _p(out), W, N, n, float(sqrt_dt),
+                                    int(n_terms), _p(ws), ws.numel(), _stream()), "trot_trotter_b3_symblk")
     return out
 
 

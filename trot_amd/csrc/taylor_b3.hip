@@ -1,20 +1,50 @@
-// Taylor series of exp(i sqrt(dt) V) w on 3-pass split-BF16 MFMA (trot/prop/utils.py:20-29).
+// Walker propagation  w <- E . T_K(i sqrt(dt) V) . E w  on 3-pass split-BF16 MFMA
+// (trot/prop/chol_afqmc_ops.py:130-131, 152-162; trot/prop/utils.py:20-29).
 //
-// Per walker: V (N x N, complex, symmetric), w (N x n, complex128).
-//   T_0 = w;  T_k = (i sqrt(dt) / k) V T_{k-1};  w += T_k        k = 1 .. K
+// Per walker: V (N x N, complex, symmetric, sym-block complex64 as the VHS GEMM writes it),
+// w (N x n, complex128), E = exp(-dt/2 h1_eff) (N x N, real, shared by all walkers).
 //
-// Operand storage (bf16 planes x = hi + lo, [re_hi, re_lo, im_hi, im_lo]):
-//   V planes  (W, 4, Np, Np), Np = N rounded up to 128, zero padded: row i, K = column, contiguous;
-//   T planes  (W, 4, np, Np), np = n rounded up to 64: T^T, so that K (= orbital) is contiguous.
-// V planes are expanded from the sym-block complex64 VHS once per step; T_0 planes are the split
-// transpose of w; every Taylor kernel writes the next term's planes and adds the term to w (FP64).
+//   one body:  w <- w + (E - I) w              identity term exact in FP64
+//   Taylor:    T_0 = w;  T_k = (i sqrt(dt) / k) V T_{k-1};  w <- w + sum_k T_k
 //
-// taylor_b3_kernel: grid (Np/128 * np/64, W), 256 threads, tile 128 (orbitals) x 64 (occupied) x 32,
-// waves 2 x 2 of 64 x 32 (4 x 2 MFMA tiles).  Complex x complex = 4 real products x 3 passes:
-// re += Ar.Br + Ai.(-Bi), im += Ar.Bi + Ai.Br (the -Bi fragment is a sign flip of its bf16 words).
+// The right operand of every product is a set of split planes of the walker-side matrix
+// (x = hi + lo, bf16, [re_hi, re_lo, im_hi, im_lo]) stored transposed, (W, 4, np, Np) with
+// Np = N rounded up to 128 and np = n rounded up to 64, zero padded, so that K (the orbital)
+// is contiguous.  Each kernel writes the planes its successor reads: T_k for the next term,
+// the updated walker after the last term and after the first one-body product.
+// The Taylor terms k >= 1 are summed in FP32 (S, (W, Np, np) complex64); the last term adds
+// S + T_K to w in FP64.  The terms are small next to w, so FP32 rounding of their sum stays
+// far below the split-BF16 product error.
+//
+// prop_b3_kernel: one 128 (orbitals) x 64 (occupied) output tile per workgroup, 256 threads,
+// K step 32; waves 2 x 2 of 64 x 32 (4 x 2 MFMA tiles).  The V tile is read straight from
+// sym-block storage: the 8 x 8 blocks above the diagonal land in LDS as stored, the mirrored
+// ones transposed, and both are split into bf16 planes on the way.  Workgroups are ordered so
+// that the row tiles of one walker run on one XCD (workgroup b runs on XCD b % 8) and share
+// its T planes and the mirrored V blocks in that XCD's L2.
+// Complex x complex = 4 real products x 3 passes: re += Ar.Br + Ai.(-Bi), im += Ar.Bi + Ai.Br
+// (the -Bi fragment is a sign flip of its bf16 words); real E x complex = 2 x 3 passes.
 #include "capi_util.hpp"
 #include "common.hpp"
 #include "gemm_b3.hpp"
+
+namespace {
+
+constexpr int PB_BM = 128, PB_BN = 64, PB_BK = 32, PB_LDR = 40;
+
+struct PropB3P {
+  const float* vsym;    // Taylor: V, sym-block complex64 (W, vcols)
+  long vcols;           // complex numbers per walker in vsym
+  const bf16raw* ep;    // one body: (E - I) planes [hi, lo], each (Np, Np)
+  const bf16raw* tp;    // input planes (W, 4, np, Np)
+  bf16raw* tn;          // output planes (W, 4, np, Np), or null
+  float* S;             // Taylor partial sum (W, Np, np) complex64
+  const double* win;    // walkers read for the FP64 update (W, N, n) complex128
+  double* w;            // walkers written
+  int W, N, n, Np, np_, nb;
+  float c;              // Taylor coefficient sqrt(dt) / k
+  int first, last;      // Taylor term position
+};
 
 __device__ __forceinline__ bf16x8_t neg_bf16x8(bf16x8_t x) {
   u32x4_t u = __builtin_bit_cast(u32x4_t, x);
@@ -22,44 +52,104 @@ __device__ __forceinline__ bf16x8_t neg_bf16x8(bf16x8_t x) {
   return __builtin_bit_cast(bf16x8_t, u);
 }
 
-__global__ __launch_bounds__(256, 1) void taylor_b3_kernel(const bf16raw* __restrict__ vp,
-                                                           const bf16raw* __restrict__ tp, bf16raw* __restrict__ tn,
-                                                           double* __restrict__ w, int N, int n, int Np, int np_,
-                                                           float c) {
-  constexpr int BM = 128, BN = 64, BK = 32, LDR = 40;
-  __shared__ __attribute__((aligned(16))) bf16raw As[4][BM][LDR];
+__device__ __forceinline__ unsigned pack2(bf16raw a, bf16raw b) { return (unsigned)a | ((unsigned)b << 16); }
+
+// REAL_A: product (E - I) . planes (one body); else V . planes (one Taylor term).
+template <bool REAL_A>
+__global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
+  constexpr int BM = PB_BM, BN = PB_BN, BK = PB_BK, LDR = PB_LDR;
+  constexpr int PA = REAL_A ? 2 : 4;
+  __shared__ __attribute__((aligned(16))) bf16raw As[PA][BM][LDR];
   __shared__ __attribute__((aligned(16))) bf16raw Bs[4][BN][LDR];
-  const int n_rt = Np / BM;
-  const int rt = blockIdx.x % n_rt, ct = blockIdx.x / n_rt;
-  const long b = blockIdx.y;
+
+  const int n_rt = p.Np / BM;
+  const int tpw = n_rt * (p.np_ / BN);
+  const int b = blockIdx.x;
+  const int grp = b >> 3;
+  const int tile = grp % tpw;
+  const long walker = (long)(grp / tpw) * 8 + (b & 7);
+  if (walker >= p.W) return;
+  const int rt = tile % n_rt, ct = tile / n_rt;
   const int m0 = rt * BM, n0 = ct * BN;
-  const long vplane = (long)Np * Np, tplane = (long)np_ * Np;
-  const bf16raw* Ab = vp + b * 4 * vplane + (long)m0 * Np;
-  const bf16raw* Bb = tp + b * 4 * tplane + (long)n0 * Np;
+  const long tplane = (long)p.np_ * p.Np;
+  const bf16raw* Bb = p.tp + walker * 4 * tplane + (long)n0 * p.Np;
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int row0 = t >> 2, kq = (t & 3) * 8;  // A: segments t, t + 256 (rows row0, row0 + 64); B: segment t
+  const int row0 = t >> 2, kq = (t & 3) * 8;  // 16-byte segment of a plane row
 
-  u32x4_t stg[12];
-#define TB3_LOAD(k0)                                                                              \
-  do {                                                                                            \
-    const long off0 = (long)row0 * Np + (k0) + kq;                                                \
-    const long off1 = off0 + 64L * Np;                                                            \
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                               \
-      stg[2 * q] = *reinterpret_cast<const u32x4_t*>(Ab + q * vplane + off0);                     \
-      stg[2 * q + 1] = *reinterpret_cast<const u32x4_t*>(Ab + q * vplane + off1);                 \
-      stg[8 + q] = *reinterpret_cast<const u32x4_t*>(Bb + q * tplane + off0);                     \
-    }                                                                                             \
-  } while (0)
-#define TB3_STORE()                                                                               \
-  do {                                                                                            \
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                               \
-      *reinterpret_cast<u32x4_t*>(&As[q][row0][kq]) = stg[2 * q];                                 \
-      *reinterpret_cast<u32x4_t*>(&As[q][row0 + 64][kq]) = stg[2 * q + 1];                        \
-      *reinterpret_cast<u32x4_t*>(&Bs[q][row0][kq]) = stg[8 + q];                                 \
-    }                                                                                             \
-  } while (0)
+  // V staging: thread t reads stored rows 2 aq, 2 aq + 1 (128 contiguous bytes) of block ablk
+  const int ablk = t >> 2, aq = t & 3;
+  const int rbl = ablk >> 2, kbl = ablk & 3;
+  const int rb = (m0 >> 3) + rbl;
+  const float* Vb = REAL_A ? nullptr : p.vsym + walker * p.vcols * 2;
+  const bf16raw* Eb = REAL_A ? p.ep + (long)m0 * p.Np : nullptr;
+  bool up = true;
+
+  f4_t ast[8];
+  u32x4_t bst[4];
+  auto load = [&](int k0) {
+    if constexpr (REAL_A) {
+      const long off0 = (long)row0 * p.Np + k0 + kq;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const u32x4_t x0 = *reinterpret_cast<const u32x4_t*>(Eb + q * (long)p.Np * p.Np + off0);
+        const u32x4_t x1 = *reinterpret_cast<const u32x4_t*>(Eb + q * (long)p.Np * p.Np + off0 + 64L * p.Np);
+        ast[4 * q] = __builtin_bit_cast(f4_t, x0);
+        ast[4 * q + 1] = __builtin_bit_cast(f4_t, x1);
+      }
+    } else {
+      const int kb = (k0 >> 3) + kbl;
+      up = rb <= kb;
+      if (rb < p.nb && kb < p.nb) {
+        const long s = up ? symblk_index(rb, kb, p.nb) : symblk_index(kb, rb, p.nb);
+        const f4_t* src = reinterpret_cast<const f4_t*>(Vb + s * 128 + 32 * aq);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ast[j] = src[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ast[j] = f4_t{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+    const long offb = (long)row0 * p.Np + k0 + kq;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bst[q] = *reinterpret_cast<const u32x4_t*>(Bb + q * tplane + offb);
+  };
+  auto store = [&]() {
+    if constexpr (REAL_A) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        *reinterpret_cast<u32x4_t*>(&As[q][row0][kq]) = __builtin_bit_cast(u32x4_t, ast[4 * q]);
+        *reinterpret_cast<u32x4_t*>(&As[q][row0 + 64][kq]) = __builtin_bit_cast(u32x4_t, ast[4 * q + 1]);
+      }
+    } else {
+      // complex c = 8 srl + sc of the two stored rows: (re, im) = ast[c / 2][2 (c % 2)], [2 (c % 2) + 1]
+      bf16raw h[4][16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        split_bf16_raw(ast[c >> 1][2 * (c & 1)], h[0][c], h[1][c]);
+        split_bf16_raw(ast[c >> 1][2 * (c & 1) + 1], h[2][c], h[3][c]);
+      }
+      if (up) {  // stored row 2 aq + srl is tile row 8 rbl + 2 aq + srl, k = 8 kbl + sc
+#pragma unroll
+        for (int srl = 0; srl < 2; ++srl)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const bf16raw* v = &h[q][8 * srl];
+            *reinterpret_cast<u32x4_t*>(&As[q][8 * rbl + 2 * aq + srl][8 * kbl]) =
+                (u32x4_t){pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7])};
+          }
+      } else {  // mirrored block: stored (2 aq + srl, sc) is tile row 8 rbl + sc, k = 8 kbl + 2 aq + srl
+#pragma unroll
+        for (int sc = 0; sc < 8; ++sc)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            *reinterpret_cast<unsigned*>(&As[q][8 * rbl + sc][8 * kbl + 2 * aq]) = pack2(h[q][sc], h[q][8 + sc]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) *reinterpret_cast<u32x4_t*>(&Bs[q][row0][kq]) = bst[q];
+  };
 
   f4_t acc[2][4][2];
 #pragma unroll
@@ -69,58 +159,64 @@ __global__ __launch_bounds__(256, 1) void taylor_b3_kernel(const bf16raw* __rest
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[q][i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
-  const int nk = Np / BK;
+  const int nk = p.Np / BK;
   const int ko = 8 * (lane >> 4);
-  TB3_LOAD(0);
+  load(0);
   for (int kc = 0; kc < nk; ++kc) {
     __syncthreads();
-    TB3_STORE();
+    store();
     __syncthreads();
-    if (kc + 1 < nk) TB3_LOAD((kc + 1) * BK);
-    // B fragments: [j][plane] with plane 0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo; im negated copies
-    bf16x8_t bf[2][4], bn[2][2];
+    if (kc + 1 < nk) load((kc + 1) * BK);
+    // B fragments [j][plane], plane 0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo
+    bf16x8_t bf[2][4];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int col = wn * 32 + j * 16 + (lane & 15);
 #pragma unroll
       for (int q = 0; q < 4; ++q) bf[j][q] = *reinterpret_cast<const bf16x8_t*>(&Bs[q][col][ko]);
-      bn[j][0] = neg_bf16x8(bf[j][2]);
-      bn[j][1] = neg_bf16x8(bf[j][3]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int row = wm * 64 + i * 16 + (lane & 15);
-      bf16x8_t a[4];
+      bf16x8_t a[PA];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) a[q] = *reinterpret_cast<const bf16x8_t*>(&As[q][row][ko]);
+      for (int q = 0; q < PA; ++q) a[q] = *reinterpret_cast<const bf16x8_t*>(&As[q][row][ko]);
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         f4_t cr = acc[0][i][j], ci = acc[1][i][j];
-        // re += Ar.Br - Ai.Bi
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][0], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][1], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][0], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], bn[j][0], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bn[j][1], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bn[j][0], cr, 0, 0, 0);
-        // im += Ar.Bi + Ai.Br
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][2], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][3], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][2], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], bf[j][0], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][1], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][0], ci, 0, 0, 0);
+        if constexpr (REAL_A) {  // re += E.Br, im += E.Bi
+          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][0], cr, 0, 0, 0);
+          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][1], cr, 0, 0, 0);
+          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][0], cr, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][2], ci, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][3], ci, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][2], ci, 0, 0, 0);
+        } else {
+          const bf16x8_t nh = neg_bf16x8(bf[j][2]), nl = neg_bf16x8(bf[j][3]);
+          // re += Ar.Br - Ai.Bi
+          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][0], cr, 0, 0, 0);
+          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][1], cr, 0, 0, 0);
+          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][0], cr, 0, 0, 0);
+          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], nh, cr, 0, 0, 0);
+          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], nl, cr, 0, 0, 0);
+          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], nh, cr, 0, 0, 0);
+          // im += Ar.Bi + Ai.Br
+          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][2], ci, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][3], ci, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][2], ci, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], bf[j][0], ci, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][1], ci, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][0], ci, 0, 0, 0);
+        }
         acc[0][i][j] = cr;
         acc[1][i][j] = ci;
       }
     }
   }
-#undef TB3_LOAD
-#undef TB3_STORE
 
-  // epilogue: T = i c acc -> (re, im) = (-c acc_im, c acc_re).  Lane holds rows 4 (l/16) + r, column l % 16.
-  bf16raw* Tn = tn + b * 4 * tplane;
-  double* wb = w + 2 * b * (long)N * n;
+  // epilogue.  Lane holds rows 4 (l / 16) + r, column l % 16 of each 16 x 16 tile.
+  bf16raw* Tn = p.tn ? p.tn + walker * 4 * tplane : nullptr;
+  const bool to_w = REAL_A || p.last;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -130,65 +226,52 @@ __global__ __launch_bounds__(256, 1) void taylor_b3_kernel(const bf16raw* __rest
       bf16raw hr[4], lr[4], hi[4], li[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float tre = -c * acc[1][i][j][r], tim = c * acc[0][i][j][r];
-        split_bf16_raw(tre, hr[r], lr[r]);
-        split_bf16_raw(tim, hi[r], li[r]);
         const int gi = gi0 + r;
-        if (gi < N && gp < n) {
-          double2* q = reinterpret_cast<double2*>(wb) + (long)gi * n + gp;
-          const double2 o = *q;
-          *q = make_double2(o.x + (double)tre, o.y + (double)tim);
+        float tre, tim;
+        if constexpr (REAL_A) {
+          tre = acc[0][i][j][r];
+          tim = acc[1][i][j][r];
+        } else {  // T = i c acc
+          tre = -p.c * acc[1][i][j][r];
+          tim = p.c * acc[0][i][j][r];
         }
+        float pr = tre, pi = tim;  // value whose planes are written
+        if (!REAL_A) {
+          float2* sp = reinterpret_cast<float2*>(p.S) + (walker * p.Np + gi) * p.np_ + gp;
+          const float2 so = p.first ? make_float2(0.f, 0.f) : *sp;
+          if (!p.last) {
+            *sp = make_float2(so.x + tre, so.y + tim);
+          } else {
+            tre += so.x;
+            tim += so.y;
+          }
+        }
+        if (to_w) {
+          pr = pi = 0.f;
+          if (gi < p.N && gp < p.n) {
+            const long e = (walker * p.N + gi) * p.n + gp;
+            const double2 o = reinterpret_cast<const double2*>(p.win)[e];
+            const double2 v = make_double2(o.x + (double)tre, o.y + (double)tim);
+            reinterpret_cast<double2*>(p.w)[e] = v;
+            pr = (float)v.x;
+            pi = (float)v.y;
+          }
+        }
+        split_bf16_raw(pr, hr[r], lr[r]);
+        split_bf16_raw(pi, hi[r], li[r]);
       }
-      // 4 consecutive rows of T^T row gp: one 8-byte store per plane
-      const long o = (long)gp * Np + gi0;
-      typedef unsigned short us4 __attribute__((ext_vector_type(4)));
-      *reinterpret_cast<us4*>(Tn + 0 * tplane + o) = (us4){hr[0], hr[1], hr[2], hr[3]};
-      *reinterpret_cast<us4*>(Tn + 1 * tplane + o) = (us4){lr[0], lr[1], lr[2], lr[3]};
-      *reinterpret_cast<us4*>(Tn + 2 * tplane + o) = (us4){hi[0], hi[1], hi[2], hi[3]};
-      *reinterpret_cast<us4*>(Tn + 3 * tplane + o) = (us4){li[0], li[1], li[2], li[3]};
+      if (Tn) {  // 4 consecutive rows of row gp of the transposed planes: one 8-byte store per plane
+        const long o = (long)gp * p.Np + gi0;
+        typedef unsigned short us4 __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<us4*>(Tn + 0 * tplane + o) = (us4){hr[0], hr[1], hr[2], hr[3]};
+        *reinterpret_cast<us4*>(Tn + 1 * tplane + o) = (us4){lr[0], lr[1], lr[2], lr[3]};
+        *reinterpret_cast<us4*>(Tn + 2 * tplane + o) = (us4){hi[0], hi[1], hi[2], hi[3]};
+        *reinterpret_cast<us4*>(Tn + 3 * tplane + o) = (us4){li[0], li[1], li[2], li[3]};
+      }
     }
 }
 
-// V planes (W, 4, Np, Np) from the sym-block complex64 VHS (W, P*64): one 8 x 8 block (bi <= bj)
-// per 64-thread workgroup, written at (bi, bj) and, transposed, at (bj, bi).  Grid (nb, nb, W).
-__global__ __launch_bounds__(64) void expand_vhs_b3_kernel(const float* __restrict__ vsym, bf16raw* __restrict__ vp,
-                                                           int nb, int Np, long cols) {
-  const int bi = blockIdx.x, bj = blockIdx.y;
-  if (bi > bj) return;
-  __shared__ float2 blk[64];
-  const long b = blockIdx.z;
-  const int t = threadIdx.x;
-  blk[t] = reinterpret_cast<const float2*>(vsym)[b * cols + symblk_index(bi, bj, nb) * 64 + t];
-  __syncthreads();
-  const long plane = (long)Np * Np;
-  bf16raw* base = vp + b * 4 * plane;
-  const int a = t >> 3, e = t & 7;
-  {  // block (bi, bj): element (a, e) at row 8 bi + a, column 8 bj + e
-    const float2 v = blk[t];
-    const long o = (long)(8 * bi + a) * Np + 8 * bj + e;
-    bf16raw h, l;
-    split_bf16_raw(v.x, h, l);
-    base[o] = h;
-    base[plane + o] = l;
-    split_bf16_raw(v.y, h, l);
-    base[2 * plane + o] = h;
-    base[3 * plane + o] = l;
-  }
-  if (bi != bj) {  // mirrored block (bj, bi): element (a, e) is V[8 bi + e][8 bj + a]
-    const float2 v = blk[e * 8 + a];
-    const long o = (long)(8 * bj + a) * Np + 8 * bi + e;
-    bf16raw h, l;
-    split_bf16_raw(v.x, h, l);
-    base[o] = h;
-    base[plane + o] = l;
-    split_bf16_raw(v.y, h, l);
-    base[2 * plane + o] = h;
-    base[3 * plane + o] = l;
-  }
-}
-
-// T^T planes (W, 4, np, Np) from walkers w (W, N, n) complex128, zero padded.  Grid (Np/64, np/64, W),
+// Transposed planes (W, 4, np, Np) of walkers w (W, N, n) complex128, zero padded.  Grid (Np/64, np/64, W),
 // 256 threads; a 64 x 64 tile goes through LDS so that reads run along p and writes along i.
 __global__ __launch_bounds__(256) void split_transpose_w_kernel(const double* __restrict__ w, bf16raw* __restrict__ tp,
                                                                 int N, int n, int Np, int np_) {
@@ -219,47 +302,147 @@ __global__ __launch_bounds__(256) void split_transpose_w_kernel(const double* __
   }
 }
 
-static inline int pad_up(int x, int m) { return (x + m - 1) / m * m; }
+// (E - I) split into planes [hi, lo] of (Np, Np), zero padded.
+__global__ void onebody_b3_planes_kernel(const double* __restrict__ e, int N, int Np, bf16raw* __restrict__ out) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)Np * Np;
+  if (idx >= total) return;
+  const int i = (int)(idx / Np), k = (int)(idx - (idx / Np) * Np);
+  const double v = (i < N && k < N) ? e[(long)i * N + k] - (i == k ? 1.0 : 0.0) : 0.0;
+  bf16raw h, l;
+  split_bf16_raw((float)v, h, l);
+  out[idx] = h;
+  out[total + idx] = l;
+}
+
+inline int pad_up(int x, int m) { return (x + m - 1) / m * m; }
+
+struct B3Buffers {
+  bf16raw* t0;
+  bf16raw* t1;
+  float* S;
+};
+
+size_t b3_workspace(int W, int N, int n) {
+  const size_t Np = pad_up(N, PB_BM), np_ = pad_up(n, PB_BN);
+  return 2 * align256((size_t)W * 4 * np_ * Np * 2) + align256((size_t)W * Np * np_ * 8);
+}
+
+template <bool REAL_A>
+int launch_prop(const PropB3P& P, hipStream_t s) {
+  const int tpw = (P.Np / PB_BM) * (P.np_ / PB_BN);
+  const long nwg = (long)((P.W + 7) / 8) * 8 * tpw;
+  if (nwg > 0x7fffffffL) return fail(TROT_ERR_INVALID, "propagation b3: too many walkers");
+  hipLaunchKernelGGL(prop_b3_kernel<REAL_A>, dim3((unsigned)nwg), dim3(256), 0, s, P);
+  HIPCHK(hipGetLastError());
+  return TROT_OK;
+}
+
+// w_out <- [E .] T_K(i sqrt(dt) V) [. E] w_in; one-body factors when ep is non-null.
+int run_b3(const float* vsym, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
+           double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, hipStream_t s) {
+  const int Np = pad_up(N, PB_BM), np_ = pad_up(n, PB_BN), nb = (N + 7) / 8;
+  Carve cv{(char*)ws, ws_bytes, 0};
+  bf16raw* bufs[2] = {cv.take<bf16raw>((size_t)W * 4 * np_ * Np), cv.take<bf16raw>((size_t)W * 4 * np_ * Np)};
+  float* S = cv.take<float>((size_t)W * Np * np_ * 2);
+  if (!bufs[0] || !bufs[1] || !S) return fail(TROT_ERR_WORKSPACE, "propagation b3: workspace");
+  if (!ep && n_terms == 0) {
+    if (w_out != w_in) HIPCHK(hipMemcpyAsync(w_out, w_in, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
+    return TROT_OK;
+  }
+  hipLaunchKernelGGL(split_transpose_w_kernel, dim3(Np / 64, np_ / 64, W), dim3(256), 0, s, w_in, bufs[0], N, n, Np,
+                     np_);
+  HIPCHK(hipGetLastError());
+  int cur = 0;
+  PropB3P P{};
+  P.vsym = vsym;
+  P.vcols = symblk_count(nb) * 64;
+  P.ep = ep;
+  P.S = S;
+  P.W = W;
+  P.N = N;
+  P.n = n;
+  P.Np = Np;
+  P.np_ = np_;
+  P.nb = nb;
+  if (ep) {  // first one-body factor: w_out = w_in + (E - I) w_in, with its planes
+    P.tp = bufs[cur];
+    P.tn = bufs[cur ^ 1];
+    P.win = w_in;
+    P.w = w_out;
+    if (int rc = launch_prop<true>(P, s)) return rc;
+    cur ^= 1;
+  } else if (w_out != w_in) {
+    HIPCHK(hipMemcpyAsync(w_out, w_in, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
+  }
+  for (int k = 1; k <= n_terms; ++k) {
+    P.tp = bufs[cur];
+    P.first = k == 1;
+    P.last = k == n_terms;
+    P.tn = (P.last && !ep) ? nullptr : bufs[cur ^ 1];
+    P.win = w_out;
+    P.w = w_out;
+    P.c = (float)(sqrt_dt / k);
+    if (int rc = launch_prop<false>(P, s)) return rc;
+    cur ^= 1;
+  }
+  if (ep) {  // second one-body factor on the planes of the propagated walker
+    P.tp = bufs[cur];
+    P.tn = nullptr;
+    P.win = w_out;
+    P.w = w_out;
+    if (int rc = launch_prop<true>(P, s)) return rc;
+  }
+  return TROT_OK;
+}
+
+int check_sizes(const char* who, int W, int N, int n, int n_terms) {
+  if (W < 1 || W > 65535 || N < 1 || n < 1 || n_terms < 0) {
+    char msg[128];
+    snprintf(msg, sizeof(msg), "%s: bad sizes", who);
+    return fail(TROT_ERR_INVALID, msg);
+  }
+  return TROT_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
-size_t trot_taylor_b3_workspace(int W, int N, int n) {
-  const size_t Np = pad_up(N, 128), np_ = pad_up(n, 64);
-  return align256((size_t)W * 4 * Np * Np * 2) + 2 * align256((size_t)W * 4 * np_ * Np * 2);
-}
+size_t trot_taylor_b3_workspace(int W, int N, int n) { return b3_workspace(W, N, n); }
 
 // w (complex128, in/out) <- sum_{k=0..K} (i sqrt(dt) V)^k / k! w with V given in sym-block storage as
-// complex64 (W, trot_symblk_cols(N)).  All products on split-BF16 MFMA, the sum in FP64.
+// complex64 (W, trot_symblk_cols(N)).  Products on split-BF16 MFMA; terms summed in FP32, added to w in FP64.
 int trot_taylor_apply_b3_symblk(const float* vhs_symblk, double* w, int W, int N, int n, double sqrt_dt, int n_terms,
                                 void* ws, size_t ws_bytes, void* stream) {
   if (!vhs_symblk || !w) return fail(TROT_ERR_INVALID, "taylor_apply_b3: null pointer");
-  if (W < 1 || N < 1 || n < 1 || n_terms < 0 || W > 65535) return fail(TROT_ERR_INVALID, "taylor_apply_b3: bad sizes");
-  if (ws_bytes < trot_taylor_b3_workspace(W, N, n)) return fail(TROT_ERR_WORKSPACE, "taylor_apply_b3: workspace");
-  const int Np = pad_up(N, 128), np_ = pad_up(n, 64), nb = (N + 7) / 8;
-  Carve cv{(char*)ws, ws_bytes, 0};
-  bf16raw* vp = cv.take<bf16raw>((size_t)W * 4 * Np * Np);
-  bf16raw* t0 = cv.take<bf16raw>((size_t)W * 4 * np_ * Np);
-  bf16raw* t1 = cv.take<bf16raw>((size_t)W * 4 * np_ * Np);
-  hipStream_t s = (hipStream_t)stream;
-  if (n_terms == 0) return TROT_OK;
-  // padding rows / columns of V beyond 8 nb are never written by the expansion: clear them
-  if (Np > 8 * nb) HIPCHK(hipMemsetAsync(vp, 0, (size_t)W * 4 * Np * Np * 2, s));
-  hipLaunchKernelGGL(expand_vhs_b3_kernel, dim3(nb, nb, W), dim3(64), 0, s, vhs_symblk, vp, nb, Np,
-                     (long)symblk_count(nb) * 64);
+  if (int rc = check_sizes("taylor_apply_b3", W, N, n, n_terms)) return rc;
+  if (ws_bytes < b3_workspace(W, N, n)) return fail(TROT_ERR_WORKSPACE, "taylor_apply_b3: workspace");
+  return run_b3(vhs_symblk, nullptr, w, w, W, N, n, sqrt_dt, n_terms, ws, ws_bytes, (hipStream_t)stream);
+}
+
+size_t trot_onebody_b3_planes_bytes(int N) {
+  const size_t Np = pad_up(N, PB_BM);
+  return 2 * Np * Np * 2;
+}
+
+int trot_onebody_b3_planes(const double* expm, int N, void* planes, void* stream) {
+  if (!expm || !planes || N < 1) return fail(TROT_ERR_INVALID, "onebody_b3_planes: bad arguments");
+  const int Np = pad_up(N, PB_BM);
+  const long total = (long)Np * Np;
+  hipLaunchKernelGGL(onebody_b3_planes_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, expm, N, Np, (bf16raw*)planes);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(split_transpose_w_kernel, dim3(Np / 64, np_ / 64, W), dim3(256), 0, s, w, t0, N, n, Np, np_);
-  HIPCHK(hipGetLastError());
-  bf16raw* prev = t0;
-  bf16raw* next = t1;
-  for (int k = 1; k <= n_terms; ++k) {
-    hipLaunchKernelGGL(taylor_b3_kernel, dim3((Np / 128) * (np_ / 64), W), dim3(256), 0, s, vp, prev, next, w, N, n,
-                       Np, np_, (float)(sqrt_dt / k));
-    HIPCHK(hipGetLastError());
-    bf16raw* tmp = prev;
-    prev = next;
-    next = tmp;
-  }
   return TROT_OK;
+}
+
+int trot_trotter_b3_symblk(const float* vhs_symblk, const void* onebody_planes, const double* w_in, double* w_out,
+                           int W, int N, int n, double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, void* stream) {
+  if (!vhs_symblk || !onebody_planes || !w_in || !w_out) return fail(TROT_ERR_INVALID, "trotter_b3: null pointer");
+  if (int rc = check_sizes("trotter_b3", W, N, n, n_terms)) return rc;
+  if (ws_bytes < b3_workspace(W, N, n)) return fail(TROT_ERR_WORKSPACE, "trotter_b3: workspace");
+  return run_b3(vhs_symblk, (const bf16raw*)onebody_planes, w_in, w_out, W, N, n, sqrt_dt, n_terms, ws, ws_bytes,
+                (hipStream_t)stream);
 }
 
 }  // extern "C"

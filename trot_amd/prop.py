@@ -193,6 +193,9 @@ def build_prop_ctx(ham_data, rdm1, dt: float, precision: str = "f64", walker_chu
     elif precision != "f64":
         # the reference's mixed mode stores chol_flat as float32 (chol_afqmc_ops.py:117)
         ctx.low["chol_flat32"] = ctx.chol_flat.to(torch.float32).contiguous()
+    if ctx.low["symblk"] and ctx.taylor_precision == "bf16x3":
+        # whole propagation on split-BF16 MFMA: (E - I) planes for the one-body factors
+        ctx.low["onebody_b3"] = _lib.onebody_b3_planes(ctx.exp_h1_half)
     return ctx
 
 
@@ -319,6 +322,16 @@ def _taylor(ctx: CholAfqmcCtx, v: torch.Tensor, t: torch.Tensor, n_terms: int) -
                                 ctx.taylor_precision, inplace=True)
 
 
+def _trotter_b3(ctx: CholAfqmcCtx, v: torch.Tensor, w: torch.Tensor, n_terms: int) -> torch.Tensor:
+    """One spin set through E T_K E on split-BF16 MFMA (one-body factors as w + (E - I) w)."""
+    fn = _lib.trotter_b3_symblk
+    args = (v, w.contiguous(), ctx.low["onebody_b3"], ctx.sqrt_dt, n_terms)
+    timer = ctx.low.get("timer")
+    if timer is not None:
+        return timer.wrap("trotter", fn, *args)
+    return fn(*args)
+
+
 def apply_trotter(walkers, xbar: torch.Tensor, ctx: CholAfqmcCtx, n_terms: int):
     """trot/prop/chol_afqmc_ops.py:198-222: E T_K(i sqrt(dt) V(xbar)) E w, V shared by both spins.
 
@@ -335,6 +348,13 @@ def apply_trotter(walkers, xbar: torch.Tensor, ctx: CholAfqmcCtx, n_terms: int):
         w1 = min(W, w0 + chunk)
         v = _vhs(ctx, xbar[w0:w1])
         for i, s in enumerate(spins):
+            if "onebody_b3" in ctx.low:
+                r = _trotter_b3(ctx, v, s[w0:w1], n_terms)
+                if chunk == W:
+                    outs[i] = r
+                else:
+                    outs[i][w0:w1] = r
+                continue
             t = _one_body(ctx, s[w0:w1])
             t = _taylor(ctx, v, t, n_terms)
             r = _one_body(ctx, t)

@@ -195,6 +195,22 @@ int trot_onebody_b3_planes(const double* expm, int N, void* planes, void* stream
 int trot_trotter_b3_symblk(const float* vhs_symblk, const void* onebody_planes, const double* w_in, double* w_out,
                            int W, int N, int n, double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, void* stream);
 
+/* Split-BF16 planes of rot_chol (G, n, N) FP64 for trot_local_energy_b3, trot_rot_chol_b3_bytes(G, n, N) bytes;
+ * made once per measurement context (trot/meas/rhf.py:308-319 builds rot_chol). */
+size_t trot_rot_chol_b3_bytes(int G, int n, int N);
+int trot_rot_chol_b3(const double* rot_chol, int G, int n, int N, void* planes, void* stream);
+
+/* E_L per walker as trot_local_energy, with the exchange contractions f_g = rot_chol[g] . GT on split-BF16
+ * MFMA (FP32 accumulation) for all walkers and Cholesky vectors in one batched GEMM; traces, exchange sums and
+ * the sums over g in FP64.  rc_planes_s from trot_rot_chol_b3.  Second spin NULL: restricted walkers with an
+ * RHF trial.  Workspace trot_local_energy_b3_workspace(W, N, G, second spin present).
+ * Replaces: energy_kernel_rw_rh / _two_body_energy_restricted (trot/meas/rhf.py:41-60, 128-138),
+ * energy_kernel_uw_rh (rhf.py:156-172, 63-98) and trot/meas/uhf.py:205-231. */
+size_t trot_local_energy_b3_workspace(int W, int N, int G, int two_spins);
+int trot_local_energy_b3(const void* rc_planes_a, const void* rc_planes_b, const double* green_t_a,
+                         const double* green_t_b, const double* rot_h1t_a, const double* rot_h1t_b, int W, int N,
+                         int na, int nb, int G, double h0, double* energy, void* ws, size_t ws_bytes, void* stream);
+
 /* ---- reduced-precision forms (prec = TROT_PREC_F32 or TROT_PREC_BF16X3) ----
  * Cholesky-side operands are FP32 copies made at context build; FP64 walker-side operands are rounded
  * (and split) as they are staged into LDS; accumulation is FP32 on MFMA.  The reference's own mixed mode

@@ -199,3 +199,39 @@ def test_mixed_steps_track_fp64(gpu, prec, tprec):
     tol = 50 * TOL[prec if tprec != "bf16x3" else "bf16x3"]
     assert rel(st.walkers.cpu().numpy(), ref.walkers) < tol
     assert rel(st.weights.cpu().numpy(), ref.weights) < tol
+
+
+@pytest.mark.parametrize("shape", [(40, 8, 50, 13), (130, 64, 7, 4), (64, 1, 3, 1), (200, 33, 130, 6)])
+@pytest.mark.parametrize("kind", ["rr", "uu"])
+def test_local_energy_b3(gpu, shape, kind):
+    """Batched split-BF16 exchange GEMM energy against the FP64 oracle and the FP64 kernel.
+
+    Odd walker counts (the last 2-walker tile half empty), G not a multiple of 4, nocc = 64 (full
+    64-row blocks) and nocc = 1.
+    """
+    from trot_amd import _lib, meas, trial
+    from trot_amd.core import HamChol, System
+
+    N, n, G, W = shape
+    nb = max(1, n - 1) if kind == "uu" else n
+    p = problem(N, n, G, W, seed=37, nocc_b=nb)
+    ham = HamChol(p["h0"], p["h1"], p["chol"]).to_device()
+    if kind == "uu":
+        sys_ = System(N, (n, nb), "unrestricted")
+        tr = trial.UhfTrial(p["ca"], p["cb"])
+        ops = meas.make_uhf_meas_ops(sys_, precision="bf16x3")
+        ops64 = meas.make_uhf_meas_ops(sys_)
+        ref = O.make_uhf_ops(p["h0"], p["h1"], p["chol"], p["ca"], p["cb"])
+        w, wn = (_t(p["wa"]), _t(p["wb"])), (p["wa"], p["wb"])
+    else:
+        sys_ = System(N, (n, n), "restricted")
+        tr = trial.RhfTrial(p["ca"])
+        ops = meas.make_rhf_meas_ops(sys_, precision="bf16x3")
+        ops64 = meas.make_rhf_meas_ops(sys_)
+        ref = O.make_rhf_ops(p["h0"], p["h1"], p["chol"], p["ca"])
+        w, wn = _t(p["wa"]), p["wa"]
+    e = ops.require_kernel("energy")(w, ham, ops.build_meas_ctx(ham, tr), tr).cpu().numpy()
+    e64 = ops64.require_kernel("energy")(w, ham, ops64.build_meas_ctx(ham, tr), tr).cpu().numpy()
+    e_ref = ref.energy(wn)
+    assert rel(e64 - p["h0"], e_ref - p["h0"]) < 1e-10
+    assert rel(e - p["h0"], e_ref - p["h0"]) < 4 * TOL["bf16x3"]

@@ -69,6 +69,10 @@ _SIGS = {
     "trot_one_body_apply_lp": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _vp]),
     "trot_taylor_b3_workspace": (_sz, [_i, _i, _i]),
     "trot_taylor_apply_b3_symblk": (_i, [_vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
+    "trot_rot_chol_b3_bytes": (_sz, [_i, _i, _i]),
+    "trot_rot_chol_b3": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "trot_local_energy_b3_workspace": (_sz, [_i, _i, _i, _i]),
+    "trot_local_energy_b3": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _d, _vp, _vp, _sz, _vp]),
     "trot_onebody_b3_planes_bytes": (_sz, [_i]),
     "trot_onebody_b3_planes": (_i, [_vp, _i, _vp, _vp]),
     "trot_trotter_b3_symblk": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
@@ -431,6 +435,42 @@ def taylor_apply_b3_symblk(vpk32: torch.Tensor, w: torch.Tensor, sqrt_dt: float,
     ws = _ws(L.trot_taylor_b3_workspace(W, N, n), w.device)
     _check(L.trot_taylor_apply_b3_symblk(_p(vpk32), _p(out), W, N, n, float(sqrt_dt), int(n_terms), _p(ws),
                                          ws.numel(), _stream()), "trot_taylor_apply_b3_symblk")
+    return out
+
+
+def rot_chol_b3(rot_chol: torch.Tensor) -> torch.Tensor:
+    """Split-BF16 planes of rot_chol (G, n, N) for local_energy_b3 (raw bytes, int16 storage)."""
+    G, n, N = rot_chol.shape
+    _req(rot_chol, "rot_chol", F64, (G, n, N))
+    L = lib()
+    out = torch.empty(L.trot_rot_chol_b3_bytes(G, n, N) // 2, dtype=torch.int16, device=rot_chol.device)
+    _check(L.trot_rot_chol_b3(_p(rot_chol), G, n, N, _p(out), _stream()), "trot_rot_chol_b3")
+    return out
+
+
+def local_energy_b3(rcp_a: torch.Tensor, G: int, green_t_a: torch.Tensor, rot_h1t_a: torch.Tensor, h0: float,
+                    rcp_b: torch.Tensor | None = None, green_t_b: torch.Tensor | None = None,
+                    rot_h1t_b: torch.Tensor | None = None) -> torch.Tensor:
+    """E_L (W,) complex128 with split-BF16 exchange GEMMs; rcp_s from rot_chol_b3."""
+    W, N, na = green_t_a.shape
+    _req(green_t_a, "green_t_a", C128, (W, N, na))
+    _req(rot_h1t_a, "rot_h1t_a", F64, (N, na))
+    L = lib()
+    nb = 0
+    if green_t_b is not None:
+        nb = green_t_b.shape[2]
+        _req(green_t_b, "green_t_b", C128, (W, N, nb))
+        _req(rot_h1t_b, "rot_h1t_b", F64, (N, nb))
+    for name, rcp, n in (("a", rcp_a, na), ("b", rcp_b, nb)):
+        if rcp is None:
+            continue
+        if rcp.dtype != torch.int16 or rcp.numel() * 2 != L.trot_rot_chol_b3_bytes(G, n, N):
+            raise ValueError(f"local_energy_b3: rcp_{name} must come from rot_chol_b3 for (G, n, N) = ({G}, {n}, {N})")
+    out = torch.empty(W, dtype=C128, device=green_t_a.device)
+    ws = _ws(L.trot_local_energy_b3_workspace(W, N, G, int(green_t_b is not None)), green_t_a.device)
+    _check(L.trot_local_energy_b3(_p(rcp_a), _p(rcp_b), _p(green_t_a), _p(green_t_b), _p(rot_h1t_a),
+                                  _p(rot_h1t_b), W, N, na, nb, G, float(h0), _p(out), _p(ws), ws.numel(),
+                                  _stream()), "trot_local_energy_b3")
     return out
 
 

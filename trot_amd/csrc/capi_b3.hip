@@ -89,11 +89,15 @@ int trot_gemm_b3(const void* a_planes, int a_complex, const void* b_planes, int 
   if (T > 0x7fffffffL / 2) return fail(TROT_ERR_INVALID, "gemm_b3: too many tiles");
   dim3 grid((unsigned)((T + 7) / 8 * 8), (unsigned)ksplit);
   hipStream_t s = (hipStream_t)stream;
-  if (a_complex) {
+  if (a_complex) {  // wide 128 x 256 tiles
+    const int n_nt2 = (N + 255) / 256;
+    const long T2 = (long)n_mt * n_nt2;
+    dim3 grid2((unsigned)((T2 + 7) / 8 * 8), (unsigned)ksplit);
+    const int b_rows = (int)trot_b3_rows_pad(N);
     if (c_f64)
-      hipLaunchKernelGGL((gemm_b3_nt_kernel<true, double>), grid, dim3(256), 0, s, p, n_mt, n_nt);
+      hipLaunchKernelGGL((gemm_b3w_kernel<double>), grid2, dim3(512), 0, s, p, n_mt, n_nt2, b_rows);
     else
-      hipLaunchKernelGGL((gemm_b3_nt_kernel<true, float>), grid, dim3(256), 0, s, p, n_mt, n_nt);
+      hipLaunchKernelGGL((gemm_b3w_kernel<float>), grid2, dim3(512), 0, s, p, n_mt, n_nt2, b_rows);
   } else {
     if (c_f64)
       hipLaunchKernelGGL((gemm_b3_nt_kernel<false, double>), grid, dim3(256), 0, s, p, n_mt, n_nt);

@@ -163,6 +163,15 @@ static hipError_t launch_energy_t(const typename ExOps<Tile>::TA* rc_a, const ty
   return hipGetLastError();
 }
 
+hipError_t launch_energy_finalize(const double* gt_a, const double* gt_b, const double* h1t_a, const double* h1t_b,
+                                  int na, int nb, int N, double h0, const double* partial, int nblk, double* energy,
+                                  int W, hipStream_t s) {
+  if (W <= 0) return hipSuccess;
+  hipLaunchKernelGGL(energy_finalize_kernel, dim3(W), dim3(256), 0, s, gt_a, gt_b, h1t_a, h1t_b, na, nb, N, h0,
+                     partial, nblk, energy);
+  return hipGetLastError();
+}
+
 hipError_t launch_energy(const double* rc_a, const double* rc_b, const double* gt_a, const double* gt_b,
                          const double* h1t_a, const double* h1t_b, int na, int nb, int N, int G, int W, double h0,
                          int gb, double* partial, double* energy, hipStream_t s) {

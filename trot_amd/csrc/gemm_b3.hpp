@@ -14,8 +14,9 @@
 // Workgroup 256 threads, tile 128 x 128 x 32; four waves in 2 x 2, each wave
 // 64 x 64 = 4 x 4 MFMA tiles per plane: 96 MFMAs per K step for complex A.
 // The next K chunk is loaded into registers while the current one is
-// multiplied from LDS.  LDS rows are padded to 40 bf16 (80 B): the 16 lanes
-// of a 16-byte fragment read hit 16 disjoint 4-bank groups.
+// multiplied from LDS.  LDS rows hold 32 bf16 (64 B, four 16-byte segments)
+// with segment s of row r at slot s ^ b3_swz(r) (see below), so neither the
+// staging stores nor the fragment reads conflict on LDS banks.
 //
 // Tile order is XCD-aware: workgroup b runs on XCD b % 8, and XCD x is given
 // a contiguous range of tiles ordered row-tile fastest, so consecutive
@@ -41,6 +42,14 @@ struct B3P {
   float beta;
 };
 
+// XOR swizzle of the 16-byte segments of 64-byte LDS rows.  A fragment read takes one segment of
+// 16 consecutive rows; a staging store takes the 4 segments of 4 consecutive rows.  With
+// swz(r) = ((r >> 1) ^ (r >> 3)) & 3 both patterns land on distinct 4-bank groups, for 64 banks
+// served 16 lanes at a time and for 32 banks served 8 lanes at a time.
+__device__ __forceinline__ int b3_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+// element offset of (row r, segment seg) in a tile of 64-byte rows
+__device__ __forceinline__ int b3_lds(int r, int seg) { return r * 32 + 8 * (seg ^ b3_swz(r)); }
+
 __device__ __forceinline__ void split_bf16_raw(float x, bf16raw& hi, bf16raw& lo) {
   const __bf16 h = (__bf16)x;
   const __bf16 l = (__bf16)(x - (float)h);
@@ -48,110 +57,18 @@ __device__ __forceinline__ void split_bf16_raw(float x, bf16raw& hi, bf16raw& lo
   lo = __builtin_bit_cast(bf16raw, l);
 }
 
+// Epilogue of a wave's 64 x 64 tile at (mrow, ncol): D[i = 4 (l / 16) + r][j = l % 16] in register r.
 template <bool AC, typename TC>
-__global__ __launch_bounds__(256, 1) void gemm_b3_nt_kernel(B3P p, int n_mt, int n_nt) {
-  constexpr int BM = 128, BN = 128, BK = 32, LDR = 40;
-  constexpr int PA = AC ? 4 : 2;  // A planes
-  __shared__ __attribute__((aligned(16))) bf16raw As[PA][BM][LDR];
-  __shared__ __attribute__((aligned(16))) bf16raw Bs[2][BN][LDR];
-
-  // XCD-aware tile index
-  const int T = n_mt * n_nt;
-  const int per = (T + 7) / 8;
-  const int b = blockIdx.x;
-  const int tile = (b & 7) * per + (b >> 3);
-  if (tile >= T) return;
-  const int mt = tile % n_mt, nt = tile / n_mt;
-  const int m0 = mt * BM, n0 = nt * BN;
-
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-
-  // staging: A PA planes x 2 segments, B 2 planes x 2 segments (16 B each), flat so that every
-  // index is a compile-time constant after unrolling (the array then lives in VGPRs, not scratch)
-  constexpr int NA = PA * 2, NB = 4;
-  u32x4_t stg[NA + NB];
-  const bf16raw* Ab = p.A + (long)m0 * p.Kp;
-  const bf16raw* Bb = p.B + (long)n0 * p.Kp;
-  const int row0 = t >> 2, kq = (t & 3) * 8;  // segment t; segment t + 256 is row0 + 64
-#define B3_LOAD(k0)                                                                              \
-  do {                                                                                           \
-    const long off0 = (long)row0 * p.Kp + (k0) + kq;                                             \
-    const long off1 = off0 + 64L * p.Kp;                                                         \
-    _Pragma("unroll") for (int q = 0; q < PA; ++q) {                                             \
-      stg[2 * q] = *reinterpret_cast<const u32x4_t*>(Ab + q * p.a_plane + off0);                   \
-      stg[2 * q + 1] = *reinterpret_cast<const u32x4_t*>(Ab + q * p.a_plane + off1);               \
-    }                                                                                            \
-    _Pragma("unroll") for (int q = 0; q < 2; ++q) {                                              \
-      stg[NA + 2 * q] = *reinterpret_cast<const u32x4_t*>(Bb + q * p.b_plane + off0);              \
-      stg[NA + 2 * q + 1] = *reinterpret_cast<const u32x4_t*>(Bb + q * p.b_plane + off1);          \
-    }                                                                                            \
-  } while (0)
-#define B3_STORE()                                                                               \
-  do {                                                                                           \
-    _Pragma("unroll") for (int q = 0; q < PA; ++q) {                                             \
-      *reinterpret_cast<u32x4_t*>(&As[q][row0][kq]) = stg[2 * q];                                  \
-      *reinterpret_cast<u32x4_t*>(&As[q][row0 + 64][kq]) = stg[2 * q + 1];                         \
-    }                                                                                            \
-    _Pragma("unroll") for (int q = 0; q < 2; ++q) {                                              \
-      *reinterpret_cast<u32x4_t*>(&Bs[q][row0][kq]) = stg[NA + 2 * q];                             \
-      *reinterpret_cast<u32x4_t*>(&Bs[q][row0 + 64][kq]) = stg[NA + 2 * q + 1];                    \
-    }                                                                                            \
-  } while (0)
-
-  constexpr int PC = AC ? 2 : 1;
-  f4_t acc[PC][4][4];
-#pragma unroll
-  for (int q = 0; q < PC; ++q)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[q][i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
-
-  const int kbeg = blockIdx.y * p.k_slice;
-  const int nk = min(p.k_slice, p.Kp - kbeg) / BK;
-  const int ko = 8 * (lane >> 4);
-  if (nk > 0) B3_LOAD(kbeg);
-  for (int kc = 0; kc < nk; ++kc) {
-    __syncthreads();
-    B3_STORE();
-    __syncthreads();
-    if (kc + 1 < nk) B3_LOAD(kbeg + (kc + 1) * BK);
-    bf16x8_t bh[4], bl[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int col = wn * 64 + j * 16 + (lane & 15);
-      bh[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[0][col][ko]);
-      bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[1][col][ko]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = wm * 64 + i * 16 + (lane & 15);
-#pragma unroll
-      for (int q = 0; q < PC; ++q) {
-        const bf16x8_t ah = *reinterpret_cast<const bf16x8_t*>(&As[2 * q][row][ko]);
-        const bf16x8_t al = *reinterpret_cast<const bf16x8_t*>(&As[2 * q + 1][row][ko]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          f4_t c = acc[q][i][j];
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], c, 0, 0, 0);
-          acc[q][i][j] = c;
-        }
-      }
-    }
-  }
-
-  // epilogue: D[i = 4 (l / 16) + r][j = l % 16] in register r
+__device__ __forceinline__ void b3_epilogue(const B3P& p, const f4_t (&acc)[AC ? 2 : 1][4][4], int mrow, int ncol,
+                                            int lane) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gi = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
-        const int gj = n0 + wn * 64 + j * 16 + (lane & 15);
+        const int gi = mrow + i * 16 + 4 * (lane >> 4) + r;
+        const int gj = ncol + j * 16 + (lane & 15);
         if (gi >= p.M || gj >= p.N) continue;
         if (p.part) {  // split K: raw partial sums, reduced by b3_reduce_kernel
           const long po = ((long)blockIdx.y * p.M + gi) * p.N + gj;
@@ -186,6 +103,232 @@ __global__ __launch_bounds__(256, 1) void gemm_b3_nt_kernel(B3P p, int n_mt, int
           }
         }
       }
+}
+
+template <bool AC, typename TC>
+__global__ __launch_bounds__(256, 1) void gemm_b3_nt_kernel(B3P p, int n_mt, int n_nt) {
+  constexpr int BM = 128, BN = 128, BK = 32;
+  constexpr int PA = AC ? 4 : 2;  // A planes
+  __shared__ __attribute__((aligned(16))) bf16raw As[PA][BM * BK];
+  __shared__ __attribute__((aligned(16))) bf16raw Bs[2][BN * BK];
+
+  // XCD-aware tile index
+  const int T = n_mt * n_nt;
+  const int per = (T + 7) / 8;
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per + (b >> 3);
+  if (tile >= T) return;
+  const int mt = tile % n_mt, nt = tile / n_mt;
+  const int m0 = mt * BM, n0 = nt * BN;
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  // staging: A PA planes x 2 segments, B 2 planes x 2 segments (16 B each), flat so that every
+  // index is a compile-time constant after unrolling (the array then lives in VGPRs, not scratch)
+  constexpr int NA = PA * 2, NB = 4;
+  u32x4_t stg[NA + NB];
+  const bf16raw* Ab = p.A + (long)m0 * p.Kp;
+  const bf16raw* Bb = p.B + (long)n0 * p.Kp;
+  const int row0 = t >> 2, kq = (t & 3) * 8;  // segment t; segment t + 256 is row0 + 64
+  const int st0 = b3_lds(row0, t & 3), st1 = b3_lds(row0 + 64, t & 3);
+#define B3_LOAD(k0)                                                                              \
+  do {                                                                                           \
+    const long off0 = (long)row0 * p.Kp + (k0) + kq;                                             \
+    const long off1 = off0 + 64L * p.Kp;                                                         \
+    _Pragma("unroll") for (int q = 0; q < PA; ++q) {                                             \
+      stg[2 * q] = *reinterpret_cast<const u32x4_t*>(Ab + q * p.a_plane + off0);                   \
+      stg[2 * q + 1] = *reinterpret_cast<const u32x4_t*>(Ab + q * p.a_plane + off1);               \
+    }                                                                                            \
+    _Pragma("unroll") for (int q = 0; q < 2; ++q) {                                              \
+      stg[NA + 2 * q] = *reinterpret_cast<const u32x4_t*>(Bb + q * p.b_plane + off0);              \
+      stg[NA + 2 * q + 1] = *reinterpret_cast<const u32x4_t*>(Bb + q * p.b_plane + off1);          \
+    }                                                                                            \
+  } while (0)
+#define B3_STORE()                                                                               \
+  do {                                                                                           \
+    _Pragma("unroll") for (int q = 0; q < PA; ++q) {                                             \
+      *reinterpret_cast<u32x4_t*>(&As[q][st0]) = stg[2 * q];                                       \
+      *reinterpret_cast<u32x4_t*>(&As[q][st1]) = stg[2 * q + 1];                                   \
+    }                                                                                            \
+    _Pragma("unroll") for (int q = 0; q < 2; ++q) {                                              \
+      *reinterpret_cast<u32x4_t*>(&Bs[q][st0]) = stg[NA + 2 * q];                                  \
+      *reinterpret_cast<u32x4_t*>(&Bs[q][st1]) = stg[NA + 2 * q + 1];                              \
+    }                                                                                            \
+  } while (0)
+
+  constexpr int PC = AC ? 2 : 1;
+  f4_t acc[PC][4][4];
+#pragma unroll
+  for (int q = 0; q < PC; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[q][i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+  const int kbeg = blockIdx.y * p.k_slice;
+  const int nk = min(p.k_slice, p.Kp - kbeg) / BK;
+  const int seg = lane >> 4;
+  if (nk > 0) B3_LOAD(kbeg);
+  for (int kc = 0; kc < nk; ++kc) {
+    __syncthreads();
+    B3_STORE();
+    __syncthreads();
+    if (kc + 1 < nk) B3_LOAD(kbeg + (kc + 1) * BK);
+    bf16x8_t bh[4], bl[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = wn * 64 + j * 16 + (lane & 15);
+      bh[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[0][b3_lds(col, seg)]);
+      bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[1][b3_lds(col, seg)]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = wm * 64 + i * 16 + (lane & 15);
+#pragma unroll
+      for (int q = 0; q < PC; ++q) {
+        const bf16x8_t ah = *reinterpret_cast<const bf16x8_t*>(&As[2 * q][b3_lds(row, seg)]);
+        const bf16x8_t al = *reinterpret_cast<const bf16x8_t*>(&As[2 * q + 1][b3_lds(row, seg)]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          f4_t c = acc[q][i][j];
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], c, 0, 0, 0);
+          acc[q][i][j] = c;
+        }
+      }
+    }
+  }
+
+  b3_epilogue<AC, TC>(p, acc, m0 + wm * 64, n0 + wn * 64, lane);
+}
+
+// Wide form for complex A: workgroup 512 threads, tile 128 (complex rows) x 256 x 32, eight waves in
+// 2 x 4 of 64 x 64.  Per K step the tile fetches 64 KB for 12.6 MFLOP (197 flop/B, against 131 for
+// the 128 x 128 tile), which is what bounds the VHS and force-bias products when their operands
+// stream from beyond L2.  LDS holds two stages (128 KB); one barrier per K step: the stage for
+// k + 1 is stored while k is multiplied from the other, and the global loads for k + 2 are issued
+// right after that store.  b_rows: padded row count of the B planes (rows beyond it read as zero).
+constexpr int B3W_BM = 128, B3W_BN = 256, B3W_BK = 32;
+constexpr int B3W_SMEM = 2 * (4 * B3W_BM + 2 * B3W_BN) * B3W_BK * 2;  // bytes
+
+// Tile order: workgroup b runs on XCD b % 8 and XCD x takes a contiguous range of tiles; within it,
+// strips of GM row tiles are walked column by column, row tile fastest, so that the workgroups
+// resident on one XCD at a time cover about GM row tiles x (resident / GM) column tiles.
+__device__ __forceinline__ bool b3_tile_of(int b, int n_mt, int n_nt, int GM, int& mt, int& nt) {
+  const int T = n_mt * n_nt;
+  const int per = (T + 7) / 8;
+  const int tile = (b & 7) * per + (b >> 3);
+  if (tile >= T) return false;
+  const int strip = tile / (GM * n_nt);
+  const int u = tile - strip * GM * n_nt;
+  const int h = min(GM, n_mt - strip * GM);
+  mt = strip * GM + u % h;
+  nt = u / h;
+  return true;
+}
+
+// Main loop of the wide form: acc[q][i][j] (q = re, im of A) for the wave's 64 x 64 sub-tile
+// (rows wm * 64 .., columns wn * 64 ..; wm = wave / 4, wn = wave % 4) of the tile at (m0, n0).
+__device__ __forceinline__ void b3w_mainloop(const B3P& p, int m0, int n0, int b_rows, char* smem,
+                                             f4_t (&acc)[2][4][4]) {
+  constexpr int BM = B3W_BM, BN = B3W_BN, BK = B3W_BK;
+  typedef bf16raw AsT[4][BM * BK];
+  typedef bf16raw BsT[2][BN * BK];
+  AsT* As = reinterpret_cast<AsT*>(smem);                                  // [2 stages]
+  BsT* Bs = reinterpret_cast<BsT*>(smem + 2 * sizeof(AsT));                // [2 stages]
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 2, wn = wave & 3;
+  const int row0 = t >> 2, kq = (t & 3) * 8;  // A row row0; B rows row0, row0 + 128
+  const int st0 = b3_lds(row0, t & 3), st1 = b3_lds(row0 + 128, t & 3);
+  const bool bok0 = n0 + row0 < b_rows, bok1 = n0 + row0 + 128 < b_rows;
+  const bf16raw* Ab = p.A + (long)(m0 + row0) * p.Kp + kq;
+  const bf16raw* Bb = p.B + (long)(n0 + row0) * p.Kp + kq;
+  const long b128 = 128L * p.Kp;
+
+  u32x4_t stg[8];
+  const u32x4_t zero4 = {0u, 0u, 0u, 0u};
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) stg[q] = *reinterpret_cast<const u32x4_t*>(Ab + q * p.a_plane + k0);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      stg[4 + 2 * q] = bok0 ? *reinterpret_cast<const u32x4_t*>(Bb + q * p.b_plane + k0) : zero4;
+      stg[5 + 2 * q] = bok1 ? *reinterpret_cast<const u32x4_t*>(Bb + q * p.b_plane + b128 + k0) : zero4;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) *reinterpret_cast<u32x4_t*>(&As[buf][q][st0]) = stg[q];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      *reinterpret_cast<u32x4_t*>(&Bs[buf][q][st0]) = stg[4 + 2 * q];
+      *reinterpret_cast<u32x4_t*>(&Bs[buf][q][st1]) = stg[5 + 2 * q];
+    }
+  };
+
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[q][i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+  const int kbeg = blockIdx.y * p.k_slice;
+  const int nk = min(p.k_slice, p.Kp - kbeg) / BK;
+  const int seg = lane >> 4;
+  if (nk > 0) {
+    load(kbeg);
+    store(0);
+    if (nk > 1) load(kbeg + BK);
+  }
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    const int cur = kc & 1;
+    bf16x8_t bh[4], bl[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = wn * 64 + j * 16 + (lane & 15);
+      bh[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[cur][0][b3_lds(col, seg)]);
+      bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[cur][1][b3_lds(col, seg)]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = wm * 64 + i * 16 + (lane & 15);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const bf16x8_t ah = *reinterpret_cast<const bf16x8_t*>(&As[cur][2 * q][b3_lds(row, seg)]);
+        const bf16x8_t al = *reinterpret_cast<const bf16x8_t*>(&As[cur][2 * q + 1][b3_lds(row, seg)]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          f4_t c = acc[q][i][j];
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], c, 0, 0, 0);
+          acc[q][i][j] = c;
+        }
+      }
+    }
+    if (kc + 1 < nk) {
+      store(cur ^ 1);
+      if (kc + 2 < nk) load(kbeg + (kc + 2) * BK);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename TC>
+__global__ __launch_bounds__(512, 1) void gemm_b3w_kernel(B3P p, int n_mt, int n_nt, int b_rows) {
+  __shared__ __attribute__((aligned(16))) char smem[B3W_SMEM];
+  int mt, nt;
+  if (!b3_tile_of(blockIdx.x, n_mt, n_nt, 16, mt, nt)) return;
+  const int m0 = mt * B3W_BM, n0 = nt * B3W_BN;
+  f4_t acc[2][4][4];
+  b3w_mainloop(p, m0, n0, b_rows, smem, acc);
+  const int wave = threadIdx.x >> 6;
+  b3_epilogue<true, TC>(p, acc, m0 + (wave >> 2) * 64, n0 + (wave & 3) * 64, threadIdx.x & 63);
 }
 
 // C (complex128, M x N, ldc) = alpha * sum_s part[s] (+ beta C), slices summed in order in FP64.

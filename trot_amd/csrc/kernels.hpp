@@ -23,6 +23,9 @@ hipError_t launch_energy(const double* rc_a, const double* rc_b, const double* g
 hipError_t launch_energy_lp(const float* rc_a, const float* rc_b, const double* gt_a, const double* gt_b,
                             const double* h1t_a, const double* h1t_b, int na, int nb, int N, int G, int W, double h0,
                             int gb, double* partial, double* energy, int mode, hipStream_t s);
+hipError_t launch_energy_finalize(const double* gt_a, const double* gt_b, const double* h1t_a, const double* h1t_b,
+                                  int na, int nb, int N, double h0, const double* partial, int nblk, double* energy,
+                                  int W, hipStream_t s);
 hipError_t launch_mf_trace(const double* chol, const double* dm, int G, int N, double* mf, hipStream_t s);
 hipError_t launch_sum_batches(const double* src, double* dst, int nb, long len, double scale, int accumulate,
                               hipStream_t s);

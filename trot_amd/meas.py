@@ -149,6 +149,13 @@ class Engine:
             spin.low[name] = getattr(spin, name).to(torch.float32).contiguous()
         return spin.low[name]
 
+    @staticmethod
+    def _b3(spin: SpinCtx) -> torch.Tensor:
+        """Split-BF16 planes of rot_chol for the batched exchange GEMM, made once per context."""
+        if "rot_chol_b3" not in spin.low:
+            spin.low["rot_chol_b3"] = _lib.rot_chol_b3(spin.rot_chol.contiguous())
+        return spin.low["rot_chol_b3"]
+
     def overlap_green(self, walkers, trial_data):
         return overlap_green(walkers, trial_data, want_green=True)
 
@@ -187,6 +194,12 @@ class Engine:
                 return _lib.local_energy(sa.rot_chol, greens[0], sa.rot_h1t, h0)
             return _lib.local_energy(sa.rot_chol, greens[0], sa.rot_h1t, h0, sb.rot_chol, greens[1], sb.rot_h1t)
         p = self.precision
+        if p == "bf16x3":
+            G = sa.rot_chol.shape[0]
+            if self.kind == "rr":
+                return _lib.local_energy_b3(self._b3(sa), G, greens[0], sa.rot_h1t, h0)
+            return _lib.local_energy_b3(self._b3(sa), G, greens[0], sa.rot_h1t, h0, self._b3(sb), greens[1],
+                                        sb.rot_h1t)
         if self.kind == "rr":
             return _lib.local_energy_lp(self._f32(sa, "rot_chol"), greens[0], sa.rot_h1t, h0, p)
         return _lib.local_energy_lp(self._f32(sa, "rot_chol"), greens[0], sa.rot_h1t, h0, p,

@@ -35,6 +35,25 @@ def test_overlap_green(gpu, shape):
     assert rel(gt.cpu().numpy(), np.swapaxes(ref_g, 1, 2)) < TOL
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 1, 9), (80, 17, 1, 6), (40, 33, 1, 3), (5, 1, 1, 2)])
+def test_overlap_green_pivoting(gpu, shape):
+    """Walkers far from the trial: C^T w has no dominant diagonal, so most elimination steps swap rows."""
+    from trot_amd import _lib
+
+    N, n, G, W = shape
+    p = problem(N, n, G, W, seed=12)
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal((W, N, n)) + 1j * rng.standard_normal((W, N, n))
+    m = np.einsum("ip,wiq->wpq", p["ca"], w)
+    # row-permuted identity-like blocks make the first pivots fall off the diagonal
+    if n > 1:
+        assert np.any(np.argmax(np.abs(m[:, :, 0].real) + np.abs(m[:, :, 0].imag), axis=1) != 0)
+    det, gt = _lib.overlap_green(_t(p["ca"]), _t(w))
+    torch.cuda.synchronize()
+    assert rel(det.cpu().numpy(), np.linalg.det(m)) < TOL
+    assert rel(gt.cpu().numpy(), np.swapaxes(O.half_green(w, p["ca"]), 1, 2)) < TOL
+
+
 @pytest.mark.parametrize("shape", SHAPES[:4])
 def test_force_bias_restricted(gpu, shape):
     from trot_amd import meas, trial

@@ -338,14 +338,10 @@ int launch_prop(const PropB3P& P, hipStream_t s) {
   return TROT_OK;
 }
 
-// w_out <- [E .] T_K(i sqrt(dt) V) [. E] w_in; one-body factors when ep is non-null.
-int run_b3(const float* vsym, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
-           double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, hipStream_t s) {
+// w_out <- [E .] T_K(i sqrt(dt) V) [. E] w_in for W walkers (one chunk); buffers sized for them.
+int run_b3_chunk(const float* vsym, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
+                 double sqrt_dt, int n_terms, bf16raw* const bufs[2], float* S, hipStream_t s) {
   const int Np = pad_up(N, PB_BM), np_ = pad_up(n, PB_BN), nb = (N + 7) / 8;
-  Carve cv{(char*)ws, ws_bytes, 0};
-  bf16raw* bufs[2] = {cv.take<bf16raw>((size_t)W * 4 * np_ * Np), cv.take<bf16raw>((size_t)W * 4 * np_ * Np)};
-  float* S = cv.take<float>((size_t)W * Np * np_ * 2);
-  if (!bufs[0] || !bufs[1] || !S) return fail(TROT_ERR_WORKSPACE, "propagation b3: workspace");
   if (!ep && n_terms == 0) {
     if (w_out != w_in) HIPCHK(hipMemcpyAsync(w_out, w_in, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
     return TROT_OK;
@@ -392,6 +388,36 @@ int run_b3(const float* vsym, const bf16raw* ep, const double* w_in, double* w_o
     P.win = w_out;
     P.w = w_out;
     if (int rc = launch_prop<true>(P, s)) return rc;
+  }
+  return TROT_OK;
+}
+
+// w_out <- [E .] T_K(i sqrt(dt) V) [. E] w_in; one-body factors when ep is non-null.
+int run_b3(const float* vsym, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
+           double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, hipStream_t s) {
+  const int Np = pad_up(N, PB_BM), np_ = pad_up(n, PB_BN), nb = (N + 7) / 8;
+  Carve cv{(char*)ws, ws_bytes, 0};
+  bf16raw* bufs[2] = {cv.take<bf16raw>((size_t)W * 4 * np_ * Np), cv.take<bf16raw>((size_t)W * 4 * np_ * Np)};
+  float* S = cv.take<float>((size_t)W * Np * np_ * 2);
+  if (!bufs[0] || !bufs[1] || !S) return fail(TROT_ERR_WORKSPACE, "propagation b3: workspace");
+  // Walker chunks whose V, term planes and partial sums (about 1.8 MB per walker at N = 512, n = 64)
+  // stay within the 256 MB Infinity Cache across the whole series, while each launch still has at least
+  // 512 workgroups (two per CU).  Measured at N = 512, n = 64, W = 2048: 128 walkers per chunk 10.5 ms,
+  // all at once 11.9 ms, 64 per chunk 14.3 ms (fewer workgroups than CUs can hold).
+  const long vcols = symblk_count(nb) * 64;
+  const double per_walker = 8.0 * vcols + 2.0 * 4 * np_ * Np * 2 + 8.0 * Np * np_;
+  const int tpw = (Np / PB_BM) * (np_ / PB_BN);
+  int chunk = (int)(200e6 / per_walker);
+  chunk = chunk < (512 + tpw - 1) / tpw ? (512 + tpw - 1) / tpw : chunk;
+  chunk = (chunk + 7) / 8 * 8;
+  if (chunk > W) chunk = W;
+  for (int c0 = 0; c0 < W; c0 += chunk) {
+    const int wc = W - c0 < chunk ? W - c0 : chunk;
+    bf16raw* cb[2] = {bufs[0] + (size_t)c0 * 4 * np_ * Np, bufs[1] + (size_t)c0 * 4 * np_ * Np};
+    const size_t wo = (size_t)c0 * N * n * 2;
+    if (int rc = run_b3_chunk(vsym + (size_t)c0 * vcols * 2, ep, w_in + wo, w_out + wo, wc, N, n, sqrt_dt, n_terms,
+                              cb, S + (size_t)c0 * Np * np_ * 2, s))
+      return rc;
   }
   return TROT_OK;
 }

@@ -15,88 +15,114 @@
 #include "kernels.hpp"
 
 // ---------------------------------------------------------------------------
-// Gauss-Jordan inversion with partial pivoting (|re|+|im| pivot, as LAPACK's
-// izamax), in place in LDS.  One workgroup per walker, n <= TROT_MAX_NOCC.
-// det = prod(pivots) * (-1)^swaps.
+// Gauss-Jordan inversion with partial pivoting (|re|+|im| pivot, first index on
+// ties, as LAPACK's izamax), det = prod(pivots) * (-1)^swaps.  One workgroup of
+// four waves per walker, n <= 64; the matrix is padded to 64 x 64 with an
+// identity block and held in registers: element (i, j) in lane i of wave j / 16,
+// register j % 16.  Per elimination step the pivot search is a shuffle
+// reduction inside the wave that owns column k, the row swap is a lane shuffle
+// inside every wave, and row k reaches the other lanes of its wave by shuffle:
+// two barriers per step (pivot index; column k).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__ m, double* __restrict__ minv,
                                                       double* __restrict__ det, int n) {
-  extern __shared__ double sh[];
-  cplx* a = reinterpret_cast<cplx*>(sh);
-  cplx* col = a + n * n;
-  int* perm = reinterpret_cast<int*>(col + n);
-  __shared__ double redv[256];
-  __shared__ int redi[256];
-  const int t = threadIdx.x;
+  __shared__ cplx colk[64];
+  __shared__ int perm[64];
+  __shared__ int outcol[64];
+  __shared__ int s_piv;
+  const int t = threadIdx.x, lane = t & 63, wq = t >> 6;
   const long w = blockIdx.x;
   const double* mw = m + 2 * w * n * n;
-  for (int e = t; e < n * n; e += 256) a[e] = cload(mw, e);
-  __syncthreads();
+  cplx a[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int j = wq * 16 + r;
+    a[r] = (lane < n && j < n) ? cload(mw, (long)lane * n + j) : cplx{lane == j ? 1.0 : 0.0, 0.0};
+  }
   cplx dacc = {1.0, 0.0};
   for (int k = 0; k < n; ++k) {
-    double best = -1.0;
-    int bi = n;
-    for (int i = k + t; i < n; i += 256) {
-      const double v = cabs1(a[i * n + k]);
-      if (v > best) { best = v; bi = i; }
-    }
-    redv[t] = best;
-    redi[t] = bi;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (t < s) {
-        const double v2 = redv[t + s];
-        const int i2 = redi[t + s];
-        if (v2 > redv[t] || (v2 == redv[t] && i2 < redi[t])) { redv[t] = v2; redi[t] = i2; }
+    const int kw = k >> 4, kr = k & 15;
+    if (wq == kw) {
+      cplx x = a[0];
+#pragma unroll
+      for (int r = 1; r < 16; ++r)
+        if (r == kr) x = a[r];
+      const double c = cabs1(x);
+      double v = (lane >= k && lane < n && c == c) ? c : -1.0;  // NaN never wins
+      int bi = lane;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(v, off);
+        const int b2 = __shfl_xor(bi, off);
+        if (v2 > v || (v2 == v && b2 < bi)) {
+          v = v2;
+          bi = b2;
+        }
       }
-      __syncthreads();
-    }
-    int pr = redi[0];
-    if (pr >= n) pr = k;  // column of NaNs: keep the diagonal
-    __syncthreads();
-    if (t == 0) perm[k] = pr;
-    if (pr != k) {
-      for (int j = t; j < n; j += 256) {
-        const cplx x = a[k * n + j];
-        a[k * n + j] = a[pr * n + j];
-        a[pr * n + j] = x;
+      if (lane == 0) {
+        const int pr = v < 0.0 ? k : bi;  // column of NaNs: keep the diagonal
+        s_piv = pr;
+        perm[k] = pr;
       }
     }
     __syncthreads();
-    const cplx piv = a[k * n + k];
+    const int pr = s_piv;
+    if (pr != k) {  // swap rows k and pr: lanes k and pr of every wave
+      const int src = lane == k ? pr : (lane == pr ? k : lane);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        a[r].re = __shfl(a[r].re, src);
+        a[r].im = __shfl(a[r].im, src);
+      }
+    }
+    if (wq == kw) {
+      cplx x = a[0];
+#pragma unroll
+      for (int r = 1; r < 16; ++r)
+        if (r == kr) x = a[r];
+      colk[lane] = x;
+    }
+    __syncthreads();
+    const cplx piv = colk[k];
     if (t == 0) {
       dacc = cmul(dacc, piv);
       if (pr != k) dacc = {-dacc.re, -dacc.im};
     }
     const cplx ipiv = cinv(piv);
-    for (int i = t; i < n; i += 256) col[i] = a[i * n + k];
-    __syncthreads();
-    for (int j = t; j < n; j += 256) {
-      const cplx x = (j == k) ? cplx{1.0, 0.0} : a[k * n + j];
-      a[k * n + j] = cmul(x, ipiv);
-    }
-    __syncthreads();
-    for (int e = t; e < n * n; e += 256) {
-      const int i = e / n, j = e - (e / n) * n;
-      if (i == k) continue;
-      const cplx x = (j == k) ? cplx{0.0, 0.0} : a[e];
-      a[e] = csub(x, cmul(col[i], a[k * n + j]));
-    }
-    __syncthreads();
-  }
-  for (int k = n - 1; k >= 0; --k) {
-    const int pr = perm[k];
-    if (pr != k) {
-      for (int i = t; i < n; i += 256) {
-        const cplx x = a[i * n + k];
-        a[i * n + k] = a[i * n + pr];
-        a[i * n + pr] = x;
+    const cplx f = colk[lane];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int j = wq * 16 + r;
+      cplx rk = {__shfl(a[r].re, k), __shfl(a[r].im, k)};  // row k, this column
+      if (j == k) rk = {1.0, 0.0};
+      rk = cmul(rk, ipiv);
+      if (lane == k) {
+        a[r] = rk;
+      } else {
+        const cplx x = (j == k) ? cplx{0.0, 0.0} : a[r];
+        a[r] = csub(x, cmul(f, rk));
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
+  if (t == 0) {  // undo the row swaps on the columns: column c of the inverse is column col[c] here
+    for (int c = 0; c < 64; ++c) outcol[c] = c;
+    for (int k = n - 1; k >= 0; --k) {
+      const int pr = perm[k];
+      const int x = outcol[k];
+      outcol[k] = outcol[pr];
+      outcol[pr] = x;
+    }
+  }
+  __syncthreads();
+  if (t < 64) colk[outcol[t]].re = (double)t;  // inverse map: here column j -> output column
+  __syncthreads();
   double* ow = minv + 2 * w * n * n;
-  for (int e = t; e < n * n; e += 256) cstore(ow, e, a[e]);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int j = wq * 16 + r;
+    if (lane < n && j < n) cstore(ow, (long)lane * n + (int)colk[j].re, a[r]);
+  }
   if (t == 0) cstore(det, w, dacc);
 }
 
@@ -327,10 +353,7 @@ static hipError_t set_lds(const void* f, size_t bytes) {
 hipError_t launch_inv_det(const double* m, double* minv, double* det, int n, int W, hipStream_t s) {
   if (W <= 0) return hipSuccess;
   if (n < 1 || n > TROT_MAX_NOCC) return hipErrorInvalidValue;
-  const size_t lds = (size_t)n * n * 16 + (size_t)n * 16 + (size_t)n * 4;
-  hipError_t e = set_lds((const void*)inv_det_kernel, lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(inv_det_kernel, dim3(W), dim3(256), lds, s, m, minv, det, n);
+  hipLaunchKernelGGL(inv_det_kernel, dim3(W), dim3(256), 0, s, m, minv, det, n);
   return hipGetLastError();
 }
 

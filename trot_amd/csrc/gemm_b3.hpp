@@ -323,7 +323,7 @@ template <typename TC>
 __global__ __launch_bounds__(512, 1) void gemm_b3w_kernel(B3P p, int n_mt, int n_nt, int b_rows) {
   __shared__ __attribute__((aligned(16))) char smem[B3W_SMEM];
   int mt, nt;
-  if (!b3_tile_of(blockIdx.x, n_mt, n_nt, 16, mt, nt)) return;
+  if (!b3_tile_of(blockIdx.x, n_mt, n_nt, 8, mt, nt)) return;
   const int m0 = mt * B3W_BM, n0 = nt * B3W_BN;
   f4_t acc[2][4][4];
   b3w_mainloop(p, m0, n0, b_rows, smem, acc);

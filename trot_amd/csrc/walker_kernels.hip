@@ -24,6 +24,14 @@
 // inside every wave, and row k reaches the other lanes of its wave by shuffle:
 // two barriers per step (pivot index; column k).
 // ---------------------------------------------------------------------------
+// Value of x in lane l (l wave-uniform), through the scalar unit.
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 __global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__ m, double* __restrict__ minv,
                                                       double* __restrict__ det, int n) {
   __shared__ cplx colk[64];
@@ -68,11 +76,12 @@ __global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__
     __syncthreads();
     const int pr = s_piv;
     if (pr != k) {  // swap rows k and pr: lanes k and pr of every wave
-      const int src = lane == k ? pr : (lane == pr ? k : lane);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        a[r].re = __shfl(a[r].re, src);
-        a[r].im = __shfl(a[r].im, src);
+        const cplx xk = {readlane_d(a[r].re, k), readlane_d(a[r].im, k)};
+        const cplx xp = {readlane_d(a[r].re, pr), readlane_d(a[r].im, pr)};
+        if (lane == k) a[r] = xp;
+        if (lane == pr) a[r] = xk;
       }
     }
     if (wq == kw) {
@@ -93,7 +102,7 @@ __global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int j = wq * 16 + r;
-      cplx rk = {__shfl(a[r].re, k), __shfl(a[r].im, k)};  // row k, this column
+      cplx rk = {readlane_d(a[r].re, k), readlane_d(a[r].im, k)};  // row k, this column
       if (j == k) rk = {1.0, 0.0};
       rk = cmul(rk, ipiv);
       if (lane == k) {

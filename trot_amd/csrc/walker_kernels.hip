@@ -75,13 +75,14 @@ __global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__
     }
     __syncthreads();
     const int pr = s_piv;
-    if (pr != k) {  // swap rows k and pr: lanes k and pr of every wave
+    if (pr != k) {  // swap rows k and pr: lanes k and pr of every wave (selects, no branches)
+      const bool is_k = lane == k, is_p = lane == pr;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const cplx xk = {readlane_d(a[r].re, k), readlane_d(a[r].im, k)};
-        const cplx xp = {readlane_d(a[r].re, pr), readlane_d(a[r].im, pr)};
-        if (lane == k) a[r] = xp;
-        if (lane == pr) a[r] = xk;
+        const double kre = readlane_d(a[r].re, k), kim = readlane_d(a[r].im, k);
+        const double pre = readlane_d(a[r].re, pr), pim = readlane_d(a[r].im, pr);
+        a[r].re = is_k ? pre : (is_p ? kre : a[r].re);
+        a[r].im = is_k ? pim : (is_p ? kim : a[r].im);
       }
     }
     if (wq == kw) {
@@ -98,19 +99,20 @@ __global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__
       if (pr != k) dacc = {-dacc.re, -dacc.im};
     }
     const cplx ipiv = cinv(piv);
-    const cplx f = colk[lane];
+    // with u_j = a[k][j] (1 at j = k): row k becomes u ipiv, every other row i becomes
+    // a[i][j] - a[i][k] ipiv u_j with column k taken as 0.  One update for all lanes:
+    // a <- x - g u, (x, g) = (0, -ipiv) on lane k and (a with column k zeroed, a[i][k] ipiv) elsewhere.
+    const bool is_k = lane == k;
+    const cplx fi = cmul(colk[lane], ipiv);
+    const cplx g = {is_k ? -ipiv.re : fi.re, is_k ? -ipiv.im : fi.im};
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int j = wq * 16 + r;
-      cplx rk = {readlane_d(a[r].re, k), readlane_d(a[r].im, k)};  // row k, this column
-      if (j == k) rk = {1.0, 0.0};
-      rk = cmul(rk, ipiv);
-      if (lane == k) {
-        a[r] = rk;
-      } else {
-        const cplx x = (j == k) ? cplx{0.0, 0.0} : a[r];
-        a[r] = csub(x, cmul(f, rk));
-      }
+      cplx u = {readlane_d(a[r].re, k), readlane_d(a[r].im, k)};  // row k, this column
+      if (j == k) u = {1.0, 0.0};
+      const bool zero = is_k || j == k;
+      const cplx x = {zero ? 0.0 : a[r].re, zero ? 0.0 : a[r].im};
+      a[r] = csub(x, cmul(g, u));
     }
   }
   __syncthreads();

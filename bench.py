@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--no-block", action="store_true", help="skip the block-tail timing")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (profiling runs)")
     ap.add_argument("--cpu-walkers", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device copies of 1 GiB (scale reference for rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()

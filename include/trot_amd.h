@@ -62,6 +62,13 @@ int trot_overlap_combine(const double* det_a, const double* det_b, double* overl
 int trot_force_bias(const double* green_t, const double* rot_chol_t, int W, int G, long Nn, double alpha,
                     int accumulate, double* fb, void* stream);
 
+/* As trot_force_bias, with K = Nn split into slices when W x G alone does not fill the GPU (small walker sets,
+ * fragments): FP64 partial products in ws (trot_force_bias_workspace(W, G, Nn) bytes, 0 when no split),
+ * summed in slice order.  The slice count depends on (W, G, Nn) only.  Replaces: as trot_force_bias. */
+size_t trot_force_bias_workspace(int W, int G, long Nn);
+int trot_force_bias_splitk(const double* green_t, const double* rot_chol_t, int W, int G, long Nn, double alpha,
+                           int accumulate, double* fb, void* ws, size_t ws_bytes, void* stream);
+
 /* fs = -sqrt(dt)(i fb - mf); xbar = x - fs; shift_term = sum_g xbar mf; fb_term = sum_g (x fs - fs^2/2).
  * Replaces: trot/prop/afqmc.py:99-103.  fields (W, G) real; fb, xbar (W, G) complex; mf (G) complex. */
 int trot_field_shift(const double* fields, const double* fb, const double* mf, double sqrt_dt, int W, int G,
@@ -81,6 +88,19 @@ int trot_one_body_apply(const double* expm, const double* w_in, int W, int N, in
 size_t trot_taylor_workspace(int W, int N, int n);
 int trot_taylor_apply(const double* vhs, double* w, int W, int N, int n, double sqrt_dt, int n_terms, void* ws,
                       size_t ws_bytes, void* stream);
+
+/* Tile shape of the FP64 Taylor-term GEMMs of trot_taylor_apply (symblk 0) and trot_taylor_apply_symblk
+ * (symblk 1) for one (W, N, n): 0 = 64x64 (default), 1 = 64x32, 2 = 32x64, 3 = 32x32 (orbitals x occupied).
+ * Every shape gives bit-identical results; only the speed differs.  trot_autotune_taylor times one term in
+ * each shape on scratch operands carved from ws (trot_autotune_taylor_workspace bytes), synchronizing the
+ * stream (an error while the stream is being captured), and records the fastest in a process-wide table
+ * guarded by a mutex; ms_out (4 floats) may be NULL.  No counterpart in the reference (XLA picks its own
+ * tiles); used by the batched fragment form, whose walker sets have uneven small sizes. */
+size_t trot_autotune_taylor_workspace(int W, int N, int n, int symblk);
+int trot_autotune_taylor(int W, int N, int n, int symblk, void* ws, size_t ws_bytes, void* stream, int* shape_out,
+                         float* ms_out);
+int trot_set_taylor_tile(int W, int N, int n, int symblk, int shape);
+int trot_get_taylor_tile(int W, int N, int n, int symblk);
 
 /* In place: w_s <- E . taylor(i sqrt(dt), V(xbar), E . w_s, K) for one or two spin sets sharing V,
  * in walker chunks of `chunk` (<= 0: all).  walkers_b NULL for restricted walkers.

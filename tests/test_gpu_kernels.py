@@ -54,6 +54,25 @@ def test_overlap_green_pivoting(gpu, shape):
     assert rel(gt.cpu().numpy(), np.swapaxes(O.half_green(w, p["ca"]), 1, 2)) < TOL
 
 
+@pytest.mark.parametrize("shape", [(70, 11, 40, 7), (200, 40, 97, 128)])
+def test_force_bias_split_k(gpu, shape):
+    """K = N n split into slices (uneven last slice), FP64 partials summed in order; accumulate path."""
+    from trot_amd import _lib
+
+    N, n, G, W = shape
+    lib = _lib.lib()
+    assert lib.trot_force_bias_workspace(W, G, N * n) > 0  # the split path runs
+    p = problem(N, n, G, W, seed=13)
+    rc = np.einsum("ip,gij->gpj", p["ca"], p["chol"])  # rot_chol (G, n, N)
+    ref = 2.0 * np.einsum("gpj,wpj->wg", rc, O.half_green(p["wa"], p["ca"]))  # trot/meas/rhf.py:119-125
+    _, gt = _lib.overlap_green(_t(p["ca"]), _t(p["wa"]))
+    rct = _t(np.ascontiguousarray(np.swapaxes(rc, 1, 2)))
+    fb = _lib.force_bias(gt, rct, 2.0)
+    assert rel(fb.cpu().numpy(), ref) < TOL
+    fb2 = _lib.force_bias(gt, rct, 1.0, out=fb.clone(), accumulate=True)
+    assert rel(fb2.cpu().numpy(), 1.5 * ref) < TOL
+
+
 @pytest.mark.parametrize("shape", SHAPES[:4])
 def test_force_bias_restricted(gpu, shape):
     from trot_amd import meas, trial
@@ -310,3 +329,34 @@ def test_symblk_vhs_and_taylor(gpu, N):
         v32 = _lib.vhs_build_lp(_t(xbar), blk32, prec)
         out = _lib.taylor_apply_lp_symblk(v32, _t(p["wa"]), sdt, 6, prec)
         assert rel(out.cpu().numpy() - p["wa"], ref - p["wa"]) < 4 * tol
+
+
+@pytest.mark.parametrize("shape", [(40, 8, 6), (130, 33, 3), (64, 64, 2)])
+@pytest.mark.parametrize("symblk", [False, True])
+def test_taylor_tile_shapes_bit_identical(gpu, shape, symblk):
+    """Every FP64 Taylor tile shape gives the same bits; the autotuner records one of them."""
+    from trot_amd import _lib
+
+    N, n, W = shape
+    p = problem(N, n, 17, W, seed=19)
+    rng = np.random.default_rng(3)
+    xbar = rng.standard_normal((W, 17)) + 1j * rng.standard_normal((W, 17))
+    v_ref = O.make_vhs(xbar, p["chol"].reshape(17, -1), N)
+    ref = O.taylor_expm_action(1j * 0.1, v_ref, p["wa"], 6)
+    if symblk:
+        vin = _lib.vhs_build(_t(xbar), _lib.pack_symblk(_t(p["chol"])))
+        run = lambda: _lib.taylor_apply_symblk(vin, _t(p["wa"]), 0.1, 6).cpu().numpy()  # noqa: E731
+    else:
+        vin = _t(v_ref)
+        run = lambda: _lib.taylor_apply(vin, _t(p["wa"]), 0.1, 6).cpu().numpy()  # noqa: E731
+    outs = []
+    for tile in range(4):
+        _lib.set_taylor_tile(W, N, n, symblk, tile)
+        assert _lib.get_taylor_tile(W, N, n, symblk) == tile
+        outs.append(run())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    assert rel(outs[0], ref) < TOL
+    best, ms = _lib.autotune_taylor(W, N, n, symblk)
+    assert 0 <= best < 4 and all(m > 0 for m in ms) and _lib.get_taylor_tile(W, N, n, symblk) == best
+    assert np.array_equal(run(), outs[0])

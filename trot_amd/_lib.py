@@ -38,6 +38,12 @@ _SIGS = {
     "trot_overlap_green": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _sz, _vp]),
     "trot_overlap_combine": (_i, [_vp, _vp, _vp, _i, _vp]),
     "trot_force_bias": (_i, [_vp, _vp, _i, _i, _l, _d, _i, _vp, _vp]),
+    "trot_force_bias_workspace": (_sz, [_i, _i, _l]),
+    "trot_autotune_taylor_workspace": (_sz, [_i, _i, _i, _i]),
+    "trot_autotune_taylor": (_i, [_i, _i, _i, _i, _vp, _sz, _vp, _vp, _vp]),
+    "trot_set_taylor_tile": (_i, [_i, _i, _i, _i, _i]),
+    "trot_get_taylor_tile": (_i, [_i, _i, _i, _i]),
+    "trot_force_bias_splitk": (_i, [_vp, _vp, _i, _i, _l, _d, _i, _vp, _vp, _sz, _vp]),
     "trot_field_shift": (_i, [_vp, _vp, _vp, _d, _i, _i, _vp, _vp, _vp, _vp]),
     "trot_vhs_build": (_i, [_vp, _vp, _i, _i, _l, _vp, _vp]),
     "trot_one_body_apply": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
@@ -188,8 +194,12 @@ def force_bias(green_t: torch.Tensor, rot_chol_t: torch.Tensor, alpha: float, ou
             raise ValueError("accumulate needs an output tensor")
         out = torch.empty((W, G), dtype=C128, device=green_t.device)
     _req(out, "fb", C128, (W, G))
-    _check(lib().trot_force_bias(_p(green_t), _p(rot_chol_t), W, G, N * n, float(alpha), int(accumulate), _p(out),
-                                 _stream()), "trot_force_bias")
+    L = lib()
+    nbytes = L.trot_force_bias_workspace(W, G, N * n)
+    ws = _ws(nbytes, green_t.device) if nbytes else None
+    _check(L.trot_force_bias_splitk(_p(green_t), _p(rot_chol_t), W, G, N * n, float(alpha), int(accumulate),
+                                    _p(out), _p(ws), 0 if ws is None else ws.numel(), _stream()),
+           "trot_force_bias_splitk")
     return out
 
 
@@ -436,6 +446,28 @@ def taylor_apply_b3_symblk(vpk32: torch.Tensor, w: torch.Tensor, sqrt_dt: float,
     _check(L.trot_taylor_apply_b3_symblk(_p(vpk32), _p(out), W, N, n, float(sqrt_dt), int(n_terms), _p(ws),
                                          ws.numel(), _stream()), "trot_taylor_apply_b3_symblk")
     return out
+
+
+TAYLOR_TILES = ("64x64", "64x32", "32x64", "32x32")
+
+
+def autotune_taylor(W: int, N: int, n: int, symblk: bool, device="cuda"):
+    """Time the FP64 Taylor-term tile shapes for (W, N, n) and keep the fastest; returns (shape, ms per shape)."""
+    L = lib()
+    ws = _ws(L.trot_autotune_taylor_workspace(W, N, n, int(symblk)), torch.device(device))
+    shape = ctypes.c_int(-1)
+    ms = (ctypes.c_float * 4)()
+    _check(L.trot_autotune_taylor(W, N, n, int(symblk), _p(ws), ws.numel(), _stream(), ctypes.byref(shape), ms),
+           "trot_autotune_taylor")
+    return shape.value, [float(x) for x in ms]
+
+
+def set_taylor_tile(W: int, N: int, n: int, symblk: bool, shape: int) -> None:
+    _check(lib().trot_set_taylor_tile(W, N, n, int(symblk), int(shape)), "trot_set_taylor_tile")
+
+
+def get_taylor_tile(W: int, N: int, n: int, symblk: bool) -> int:
+    return int(lib().trot_get_taylor_tile(W, N, n, int(symblk)))
 
 
 def rot_chol_b3(rot_chol: torch.Tensor) -> torch.Tensor:

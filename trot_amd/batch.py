@@ -1,0 +1,193 @@
+"""Many independent walker sets of uneven size advanced together: one HIP graph launch per step.
+
+The batching form for LNO-style fragments: every fragment keeps its own Hamiltonian, trial,
+propagation and measurement contexts and walker state (the reference runs them one after another,
+each through trot/prop/afqmc.py:82-144).  ``FragmentBatch.capture`` records one phaseless step of every
+fragment into a single HIP graph (torch.cuda.CUDAGraph on the ROCm stream) over static state buffers,
+with the step's auxiliary fields read from one flat device buffer; ``step`` copies the next fields into
+that buffer and replays the graph.  A step of all fragments is then one field copy and one graph launch
+instead of about thirty kernel launches per fragment, and the result is bit-identical to stepping the
+fragments one at a time: the replay runs the same kernels on the same data in the same order.
+
+Captured code must not synchronise with the host, allocate outside torch's graph pool or record memset
+nodes; the FP64 step path (trot_amd.prop.afqmc_step with given fields) satisfies all three.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Any, List
+
+import torch
+
+from .prop import PropState
+
+
+@dataclass
+class Fragment:
+    """One walker set with everything its step needs (the arguments of PropOps.step)."""
+
+    sys: Any
+    ham: Any
+    trial: Any
+    trial_ops: Any
+    meas_ops: Any
+    prop_ops: Any
+    params: Any
+    prop_ctx: Any
+    meas_ctx: Any
+    state: PropState
+    n_fields: int = field(init=False)
+
+    def __post_init__(self):
+        self.n_fields = int(self.prop_ctx.chol_flat.shape[0])
+
+    @property
+    def n_walkers(self) -> int:
+        return int(self.state.weights.shape[0])
+
+    def step(self, fields: torch.Tensor | None = None) -> PropState:
+        return self.prop_ops.step(self.state, params=self.params, ham_data=self.ham, trial_data=self.trial,
+                                  trial_ops=self.trial_ops, meas_ops=self.meas_ops, meas_ctx=self.meas_ctx,
+                                  prop_ctx=self.prop_ctx, fields=fields)
+
+
+def _tensors(state: PropState):
+    """The device tensors of a state, in a fixed order (walkers and greens may be spin tuples)."""
+    out = []
+    for v in (state.walkers, state.weights, state.overlaps, state.pop_control_ene_shift, state.e_estimate,
+              state.node_encounters, state.green):
+        if v is None:
+            continue
+        out.extend(v if isinstance(v, tuple) else (v,))
+    return out
+
+
+def _static_copy(state: PropState) -> PropState:
+    """A state whose tensors are fresh contiguous copies (the graph's input buffers)."""
+    def cp(v):
+        if v is None:
+            return None
+        if isinstance(v, tuple):
+            return tuple(x.detach().clone().contiguous() for x in v)
+        return v.detach().clone().contiguous()
+
+    green = state.green
+    return PropState(walkers=cp(state.walkers), weights=cp(state.weights), overlaps=cp(state.overlaps),
+                     rng_key=state.rng_key, pop_control_ene_shift=cp(state.pop_control_ene_shift),
+                     e_estimate=cp(state.e_estimate), node_encounters=cp(state.node_encounters), green=cp(green))
+
+
+def _cost(f: Fragment) -> float:
+    """Relative cost of one step of a fragment: its VHS and Taylor flops (N^2 (G + 8 K n) per walker)."""
+    N = f.prop_ctx.norb
+    n = sum(f.sys.nelec) if hasattr(f.sys, "nelec") else 1
+    return float(f.n_walkers) * N * N * (f.n_fields + 8 * f.params.n_exp_terms * n)
+
+
+class FragmentBatch:
+    """A list of fragments stepped together through one captured HIP graph.
+
+    ``n_streams`` independent branches: fragments are dealt to streams by decreasing cost onto the
+    least-loaded stream (longest-processing-time first), each branch runs its fragments in order, and
+    the graph joins the branches at the end, so fragments on different branches may run concurrently.
+    Per fragment the kernels and their order are the same as in a one-by-one run.
+    """
+
+    def __init__(self, fragments: List[Fragment], n_streams: int = 4):
+        if not fragments:
+            raise ValueError("FragmentBatch needs at least one fragment")
+        if n_streams < 1:
+            raise ValueError("n_streams must be >= 1")
+        self.fragments = fragments
+        load = [0.0] * n_streams
+        self.branch = [[] for _ in range(n_streams)]
+        for i in sorted(range(len(fragments)), key=lambda i: -_cost(fragments[i])):
+            b = min(range(n_streams), key=lambda k: load[k])
+            self.branch[b].append(i)
+            load[b] += _cost(fragments[i])
+        self.branch = [sorted(b) for b in self.branch if b]
+        self.streams = [torch.cuda.Stream() for _ in self.branch]  # default priority
+        sizes = [f.n_walkers * f.n_fields for f in fragments]
+        self.offsets = [0]
+        for s in sizes:
+            self.offsets.append(self.offsets[-1] + s)
+        self.fields = torch.zeros(self.offsets[-1], dtype=torch.float64, device="cuda")
+        self.graph = None
+        self.steps_done = 0
+
+    def autotune(self):
+        """Pick the FP64 Taylor-term tile shape of every fragment's (walkers, N, n) per spin by timing
+        (trot_autotune_taylor).  Runs eagerly, before capture; the choice changes speed, not results."""
+        from . import _lib
+
+        picked = {}
+        for f in self.fragments:
+            sym = bool(f.prop_ctx.low.get("symblk"))
+            for n in sorted(set(f.sys.nelec)):
+                key = (f.n_walkers, f.prop_ctx.norb, n, sym)
+                if key not in picked:
+                    picked[key] = _lib.autotune_taylor(*key)[0]
+        return picked
+
+    def field_views(self):
+        return [self.fields[a:b].view(f.n_walkers, f.n_fields)
+                for f, a, b in zip(self.fragments, self.offsets[:-1], self.offsets[1:])]
+
+    def pack_fields(self, per_fragment) -> torch.Tensor:
+        """One flat device tensor from per-fragment (W_f, G_f) fields, in the batch's layout."""
+        return torch.cat([torch.as_tensor(x, dtype=torch.float64).reshape(-1) for x in per_fragment]).to("cuda")
+
+    def _step_one(self, i, views):
+        f = self.fragments[i]
+        new = f.step(views[i])
+        for dst, src in zip(_tensors(f.state), _tensors(new)):
+            dst.copy_(src)
+        f.state = f.state._replace(rng_key=new.rng_key)
+
+    def _step_all(self, views):
+        main = torch.cuda.current_stream()
+        for st, idx in zip(self.streams, self.branch):
+            st.wait_stream(main)
+            with torch.cuda.stream(st):
+                for i in idx:
+                    self._step_one(i, views)
+        for st in self.streams:
+            main.wait_stream(st)
+
+    def capture(self, warmup_fields: torch.Tensor):
+        """Record one step of every fragment.  The warm-up step (eager, on a side stream, as graph
+        capture requires) advances every fragment by one step with ``warmup_fields``."""
+        for f in self.fragments:
+            if f.state.green is None:
+                _, g = f.meas_ops.engine.overlap_green(f.state.walkers, f.trial)
+                f.state = f.state._replace(green=g)
+            f.state = _static_copy(f.state)
+        views = self.field_views()
+        self.fields.copy_(warmup_fields)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self._step_all(views)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        keys = [f.state.rng_key for f in self.fragments]
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._step_all(views)
+        # capture records the step without running it: the states are those after the warm-up step
+        for f, k in zip(self.fragments, keys):
+            f.state = f.state._replace(rng_key=k)
+        self.steps_done = 1
+        return self
+
+    def step(self, fields_flat: torch.Tensor):
+        """One phaseless step of every fragment: copy the fields in, replay the graph."""
+        if self.graph is None:
+            raise RuntimeError("FragmentBatch.step before capture")
+        self.fields.copy_(fields_flat)
+        self.graph.replay()
+        for f in self.fragments:
+            s, c = f.state.rng_key
+            f.state = f.state._replace(rng_key=(s, c + 1))
+        self.steps_done += 1

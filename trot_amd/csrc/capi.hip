@@ -12,9 +12,14 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "common.hpp"
 #include "gemm.hpp"
 #include "kernels.hpp"
+
 #include "capi_util.hpp"
 
 thread_local char g_trot_err[512];
@@ -84,6 +89,44 @@ int trot_force_bias(const double* green_t, const double* rot_chol_t, int W, int 
   return TROT_OK;
 }
 
+// Split K for small walker counts: S slices of K, FP64 partial products in ws, summed in slice order.
+// S is a function of the shape only (same shape, same S, same rounding): enough slices for about
+// 1024 workgroups, each slice at least 256 long, at most 64 slices.
+static int force_bias_slices(int W, int G, long Nn) {
+  const long tiles = (long)((G + 63) / 64) * ((W + 63) / 64);
+  long S = (1024 + tiles - 1) / tiles;
+  const long smax = Nn / 256;
+  if (S > smax) S = smax;
+  if (S > 64) S = 64;
+  return S < 1 ? 1 : (int)S;
+}
+
+size_t trot_force_bias_workspace(int W, int G, long Nn) {
+  if (W < 1 || G < 1 || Nn < 1) return 0;
+  const int S = force_bias_slices(W, G, Nn);
+  return S > 1 ? align256((size_t)S * W * G * 16) : 0;
+}
+
+int trot_force_bias_splitk(const double* green_t, const double* rot_chol_t, int W, int G, long Nn, double alpha,
+                           int accumulate, double* fb, void* ws, size_t ws_bytes, void* stream) {
+  if (!green_t || !rot_chol_t || !fb) return fail(TROT_ERR_INVALID, "force_bias: null pointer");
+  if (W < 1 || G < 1 || Nn < 1) return fail(TROT_ERR_INVALID, "force_bias: bad sizes");
+  const int S = force_bias_slices(W, G, Nn);
+  if (S == 1) return trot_force_bias(green_t, rot_chol_t, W, G, Nn, alpha, accumulate, fb, stream);
+  if (!ws || ws_bytes < trot_force_bias_workspace(W, G, Nn)) return fail(TROT_ERR_WORKSPACE, "force_bias: workspace");
+  const long ks = (Nn + S - 1) / S;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  GemmP p{};
+  p.A = green_t; p.lda = Nn; p.sA = ks;
+  p.B = rot_chol_t; p.ldb = Nn; p.sB = ks;
+  p.C = part; p.ldc = G; p.sC = (long)W * G;
+  p.M = W; p.N = G; p.K = (int)ks; p.Klast = (int)(Nn - (S - 1) * ks); p.ar = 1.0;
+  HIPCHK((launch_gemm_f64<64, 64, true, false, true, false, false, EPI_STORE>(p, S, s)));
+  HIPCHK(launch_sum_batches(part, fb, S, 2L * W * G, alpha, accumulate, s));
+  return TROT_OK;
+}
+
 // ---------------------------------------------------------------------------
 // shifted fields
 // ---------------------------------------------------------------------------
@@ -132,6 +175,32 @@ int trot_one_body_apply(const double* expm, const double* w_in, int W, int N, in
 // ---------------------------------------------------------------------------
 size_t trot_taylor_workspace(int W, int N, int n) { return 2 * align256((size_t)W * N * n * 16); }
 
+// Tile shape of the FP64 Taylor-term GEMM (M = N orbitals, N = n occupied, one product per walker).
+// Every shape accumulates K in the same ascending order of 4-wide MFMA steps, so the choice changes
+// speed only, never a bit of the result.  Choices come from trot_autotune_taylor (timed on the caller's
+// stream) or trot_set_taylor_tile and live in one process-wide table behind a mutex; the default is 64x64.
+enum { TILE_64x64 = 0, TILE_64x32 = 1, TILE_32x64 = 2, TILE_32x32 = 3, TILE_COUNT = 4 };
+static std::mutex g_tile_mu;
+static std::map<std::tuple<int, int, int, int>, int> g_tile;  // (storage, W, N, n) -> shape
+
+static int taylor_tile(int sym, int W, int N, int n) {
+  std::lock_guard<std::mutex> lk(g_tile_mu);
+  auto it = g_tile.find(std::make_tuple(sym, W, N, n));
+  return it == g_tile.end() ? TILE_64x64 : it->second;
+}
+
+extern "C++" {
+template <bool ASYM>
+static hipError_t launch_taylor_term(int shape, const GemmP& p, int W, hipStream_t s) {
+  switch (shape) {
+    case TILE_64x32: return launch_gemm_f64<64, 32, true, true, true, true, false, EPI_TAYLOR, ASYM>(p, W, s);
+    case TILE_32x64: return launch_gemm_f64<32, 64, true, true, true, true, false, EPI_TAYLOR, ASYM>(p, W, s);
+    case TILE_32x32: return launch_gemm_f64<32, 32, true, true, true, true, false, EPI_TAYLOR, ASYM>(p, W, s);
+    default: return launch_gemm_f64<64, 64, true, true, true, true, false, EPI_TAYLOR, ASYM>(p, W, s);
+  }
+}
+}  // extern "C++"
+
 int trot_taylor_apply(const double* vhs, double* w, int W, int N, int n, double sqrt_dt, int n_terms, void* ws,
                       size_t ws_bytes, void* stream) {
   if (!vhs || !w) return fail(TROT_ERR_INVALID, "taylor_apply: null pointer");
@@ -142,6 +211,7 @@ int trot_taylor_apply(const double* vhs, double* w, int W, int N, int n, double 
   if (!t0 || !t1) return fail(TROT_ERR_WORKSPACE, "taylor_apply: workspace too small");
   hipStream_t s = (hipStream_t)stream;
   if (n_terms == 0) return TROT_OK;
+  const int shape = taylor_tile(0, W, N, n);
   HIPCHK(hipMemcpyAsync(t0, w, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
   double* prev = t0;
   double* next = t1;
@@ -152,7 +222,7 @@ int trot_taylor_apply(const double* vhs, double* w, int W, int N, int n, double 
     p.C = next; p.ldc = n; p.sC = (long)N * n;
     p.C2 = w; p.ldc2 = n; p.sC2 = (long)N * n;
     p.M = N; p.N = n; p.K = N; p.ar = 0.0; p.ai = sqrt_dt / k;
-    HIPCHK((launch_gemm_f64<64, 64, true, true, true, true, false, EPI_TAYLOR>(p, W, s)));
+    HIPCHK(launch_taylor_term<false>(shape, p, W, s));
     double* tmp = prev;
     prev = next;
     next = tmp;
@@ -185,6 +255,7 @@ int trot_taylor_apply_symblk(const double* vhs_symblk, double* w, int W, int N, 
   if (!t0 || !t1) return fail(TROT_ERR_WORKSPACE, "taylor_apply_symblk: workspace too small");
   hipStream_t s = (hipStream_t)stream;
   if (n_terms == 0) return TROT_OK;
+  const int shape = taylor_tile(1, W, N, n);
   HIPCHK(hipMemcpyAsync(t0, w, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
   const int nb = (N + 7) / 8;
   double* prev = t0;
@@ -196,11 +267,89 @@ int trot_taylor_apply_symblk(const double* vhs_symblk, double* w, int W, int N, 
     p.C = next; p.ldc = n; p.sC = (long)N * n;
     p.C2 = w; p.ldc2 = n; p.sC2 = (long)N * n;
     p.M = N; p.N = n; p.K = N; p.ar = 0.0; p.ai = sqrt_dt / k;
-    HIPCHK((launch_gemm_f64<64, 64, true, true, true, true, false, EPI_TAYLOR, true>(p, W, s)));
+    HIPCHK(launch_taylor_term<true>(shape, p, W, s));
     double* tmp = prev;
     prev = next;
     next = tmp;
   }
+  return TROT_OK;
+}
+
+size_t trot_autotune_taylor_workspace(int W, int N, int n, int symblk) {
+  if (W < 1 || N < 1 || n < 1) return 0;
+  const size_t v = (size_t)W * (symblk ? trot_symblk_cols(N) : (long)N * N) * 16;
+  return align256(v) + 3 * align256((size_t)W * N * n * 16);
+}
+
+int trot_set_taylor_tile(int W, int N, int n, int symblk, int shape) {
+  if (shape < 0 || shape >= TILE_COUNT) return fail(TROT_ERR_INVALID, "set_taylor_tile: unknown shape");
+  std::lock_guard<std::mutex> lk(g_tile_mu);
+  g_tile[std::make_tuple(symblk ? 1 : 0, W, N, n)] = shape;
+  return TROT_OK;
+}
+
+int trot_get_taylor_tile(int W, int N, int n, int symblk) { return taylor_tile(symblk ? 1 : 0, W, N, n); }
+
+// Times one Taylor term (K = 1: one product and the accumulation) in each tile shape on zeroed scratch
+// operands carved from ws, three launches per shape after one warm-up, and records the fastest for
+// (W, N, n, storage).  Synchronizes the stream: not for use while the stream is being captured.
+int trot_autotune_taylor(int W, int N, int n, int symblk, void* ws, size_t ws_bytes, void* stream, int* shape_out,
+                         float* ms_out) {
+  if (W < 1 || N < 1 || n < 1) return fail(TROT_ERR_INVALID, "autotune_taylor: bad sizes");
+  if (W > 65535) return fail(TROT_ERR_INVALID, "autotune_taylor: at most 65535 walkers");
+  if (ws_bytes < trot_autotune_taylor_workspace(W, N, n, symblk))
+    return fail(TROT_ERR_WORKSPACE, "autotune_taylor: workspace");
+  hipStream_t s = (hipStream_t)stream;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  HIPCHK(hipStreamIsCapturing(s, &cap));
+  if (cap != hipStreamCaptureStatusNone) return fail(TROT_ERR_INVALID, "autotune_taylor: stream is being captured");
+  Carve c{(char*)ws, ws_bytes, 0};
+  const long vcols = symblk ? trot_symblk_cols(N) : (long)N * N;
+  double* v = c.take<double>((size_t)W * vcols * 2);
+  double* w = c.take<double>((size_t)W * N * n * 2);
+  double* t0 = c.take<double>((size_t)W * N * n * 2);
+  double* t1 = c.take<double>((size_t)W * N * n * 2);
+  HIPCHK(hipMemsetAsync(ws, 0, trot_autotune_taylor_workspace(W, N, n, symblk), s));
+  GemmP p{};
+  p.A = v; p.lda = symblk ? (N + 7) / 8 : N; p.sA = vcols;
+  p.B = t0; p.ldb = n; p.sB = (long)N * n;
+  p.C = t1; p.ldc = n; p.sC = (long)N * n;
+  p.C2 = w; p.ldc2 = n; p.sC2 = (long)N * n;
+  p.M = N; p.N = n; p.K = N; p.ar = 0.0; p.ai = 0.1;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int best = TILE_64x64;
+  float best_ms = 0.f;
+  int rc = TROT_OK;
+  for (int shape = 0; shape < TILE_COUNT && rc == TROT_OK; ++shape) {
+    hipError_t e = symblk ? launch_taylor_term<true>(shape, p, W, s) : launch_taylor_term<false>(shape, p, W, s);
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    for (int r = 0; r < 3 && e == hipSuccess; ++r)
+      e = symblk ? launch_taylor_term<true>(shape, p, W, s) : launch_taylor_term<false>(shape, p, W, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) {
+      rc = hip_status(e, "autotune_taylor");
+      break;
+    }
+    ms /= 3.f;
+    if (ms_out) ms_out[shape] = ms;
+    if (shape == 0 || ms < best_ms) {
+      best = shape;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != TROT_OK) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    g_tile[std::make_tuple(symblk ? 1 : 0, W, N, n)] = best;
+  }
+  if (shape_out) *shape_out = best;
   return TROT_OK;
 }
 

@@ -32,6 +32,7 @@ struct GemmP {
   double* C2;  // EPI_TAYLOR accumulation target
   long ldc2, sC2;
   int M, N, K;
+  int Klast;      // K of the last batch when nonzero (split K with a shorter final slice)
   double ar, ai;  // alpha
   double beta;    // EPI_STORE: C = alpha * acc + beta * C (beta real)
 };
@@ -204,7 +205,8 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
   const double* A = p.A + (AC ? 2 : 1) * (long)b * p.sA;
   const double* B = p.B + (BC ? 2 : 1) * (long)b * p.sB;
   typename T::Acc acc;
-  T::run(A, p.lda, B, p.ldb, p.M, p.N, p.K, m0, n0, sm, acc);
+  const int K = (p.Klast > 0 && b == (int)gridDim.z - 1) ? p.Klast : p.K;
+  T::run(A, p.lda, B, p.ldb, p.M, p.N, K, m0, n0, sm, acc);
 
   constexpr int PC = T::PC;
   double* C = p.C + PC * (long)b * p.sC;

@@ -48,7 +48,7 @@ def test_hydrogen_atom_radial():
     assert ints.enuc == 0.0
 
 
-@pytest.mark.parametrize("dist", [1.8, 3.6])
+@pytest.mark.parametrize("dist", [1.8])
 def test_one_electron_integrals_quadrature(dist):
     coords = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, dist]])
     ints = H.integrals(coords)
@@ -57,7 +57,7 @@ def test_one_electron_integrals_quadrature(dist):
 
     def cyl(f):  # int f dV over the axis-symmetric integrand, rho in [0, 12], z in [-12, dist + 12]
         v, _ = integrate.dblquad(lambda rho, z: 2 * np.pi * rho * f(rho, 0.0, z), -12, dist + 12, 0, 12,
-                                 epsabs=1e-11, epsrel=1e-10)
+                                 epsabs=1e-10, epsrel=1e-9)
         return v
 
     S = cyl(lambda x, y, z: _phi(a, c, A, x, y, z) * _phi(a, c, B, x, y, z))

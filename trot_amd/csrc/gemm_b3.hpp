@@ -294,27 +294,34 @@ __device__ __forceinline__ void b3w_mainloop(const B3P& p, int m0, int n0, int b
       bh[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[cur][0][b3_lds(col, seg)]);
       bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[cur][1][b3_lds(col, seg)]);
     }
+    // MFMAs of row tiles 0-1, then the next stage's LDS stores (its loads were issued one K step ago)
+    // and the loads two stages ahead, then row tiles 2-3: the stores overlap MFMA work instead of
+    // forming a tail before the barrier.
+    auto mfma_rows = [&](int i0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = wm * 64 + i * 16 + (lane & 15);
+      for (int i = i0; i < i0 + 2; ++i) {
+        const int row = wm * 64 + i * 16 + (lane & 15);
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const bf16x8_t ah = *reinterpret_cast<const bf16x8_t*>(&As[cur][2 * q][b3_lds(row, seg)]);
-        const bf16x8_t al = *reinterpret_cast<const bf16x8_t*>(&As[cur][2 * q + 1][b3_lds(row, seg)]);
+        for (int q = 0; q < 2; ++q) {
+          const bf16x8_t ah = *reinterpret_cast<const bf16x8_t*>(&As[cur][2 * q][b3_lds(row, seg)]);
+          const bf16x8_t al = *reinterpret_cast<const bf16x8_t*>(&As[cur][2 * q + 1][b3_lds(row, seg)]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          f4_t c = acc[q][i][j];
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], c, 0, 0, 0);
-          acc[q][i][j] = c;
+          for (int j = 0; j < 4; ++j) {
+            f4_t c = acc[q][i][j];
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], c, 0, 0, 0);
+            acc[q][i][j] = c;
+          }
         }
       }
-    }
+    };
+    mfma_rows(0);
     if (kc + 1 < nk) {
       store(cur ^ 1);
       if (kc + 2 < nk) load(kbeg + (kc + 2) * BK);
     }
+    mfma_rows(2);
     __syncthreads();
   }
 }

@@ -30,7 +30,7 @@
 
 namespace {
 
-constexpr int PB_BM = 128, PB_BN = 64, PB_BK = 32, PB_LDR = 40;
+constexpr int PB_BM = 128, PB_BN = 64, PB_BK = 32;
 
 struct PropB3P {
   const float* vsym;    // Taylor: V, sym-block complex64 (W, vcols)
@@ -57,10 +57,12 @@ __device__ __forceinline__ unsigned pack2(bf16raw a, bf16raw b) { return (unsign
 // REAL_A: product (E - I) . planes (one body); else V . planes (one Taylor term).
 template <bool REAL_A>
 __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
-  constexpr int BM = PB_BM, BN = PB_BN, BK = PB_BK, LDR = PB_LDR;
+  constexpr int BM = PB_BM, BN = PB_BN, BK = PB_BK;
   constexpr int PA = REAL_A ? 2 : 4;
-  __shared__ __attribute__((aligned(16))) bf16raw As[PA][BM][LDR];
-  __shared__ __attribute__((aligned(16))) bf16raw Bs[4][BN][LDR];
+  // 64-byte LDS rows (32 bf16 of K), 16-byte segments XOR-swizzled by b3_lds: 48 KB.  Two workgroups
+  // per CU (a third, possible for the one-body form, measured no faster)
+  __shared__ __attribute__((aligned(16))) bf16raw As[PA][BM * BK];
+  __shared__ __attribute__((aligned(16))) bf16raw Bs[4][BN * BK];
 
   const int n_rt = p.Np / BM;
   const int tpw = n_rt * (p.np_ / BN);
@@ -119,8 +121,8 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
     if constexpr (REAL_A) {
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        *reinterpret_cast<u32x4_t*>(&As[q][row0][kq]) = __builtin_bit_cast(u32x4_t, ast[4 * q]);
-        *reinterpret_cast<u32x4_t*>(&As[q][row0 + 64][kq]) = __builtin_bit_cast(u32x4_t, ast[4 * q + 1]);
+        *reinterpret_cast<u32x4_t*>(&As[q][b3_lds(row0, t & 3)]) = __builtin_bit_cast(u32x4_t, ast[4 * q]);
+        *reinterpret_cast<u32x4_t*>(&As[q][b3_lds(row0 + 64, t & 3)]) = __builtin_bit_cast(u32x4_t, ast[4 * q + 1]);
       }
     } else {
       // complex c = 8 srl + sc of the two stored rows: (re, im) = ast[c / 2][2 (c % 2)], [2 (c % 2) + 1]
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const bf16raw* v = &h[q][8 * srl];
-            *reinterpret_cast<u32x4_t*>(&As[q][8 * rbl + 2 * aq + srl][8 * kbl]) =
+            *reinterpret_cast<u32x4_t*>(&As[q][b3_lds(8 * rbl + 2 * aq + srl, kbl)]) =
                 (u32x4_t){pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7])};
           }
       } else {  // mirrored block: stored (2 aq + srl, sc) is tile row 8 rbl + sc, k = 8 kbl + 2 aq + srl
@@ -144,11 +146,11 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
         for (int sc = 0; sc < 8; ++sc)
 #pragma unroll
           for (int q = 0; q < 4; ++q)
-            *reinterpret_cast<unsigned*>(&As[q][8 * rbl + sc][8 * kbl + 2 * aq]) = pack2(h[q][sc], h[q][8 + sc]);
+            *reinterpret_cast<unsigned*>(&As[q][b3_lds(8 * rbl + sc, kbl) + 2 * aq]) = pack2(h[q][sc], h[q][8 + sc]);
       }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) *reinterpret_cast<u32x4_t*>(&Bs[q][row0][kq]) = bst[q];
+    for (int q = 0; q < 4; ++q) *reinterpret_cast<u32x4_t*>(&Bs[q][b3_lds(row0, t & 3)]) = bst[q];
   };
 
   f4_t acc[2][4][2];
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
       for (int j = 0; j < 2; ++j) acc[q][i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
   const int nk = p.Np / BK;
-  const int ko = 8 * (lane >> 4);
+  const int seg = lane >> 4;
   load(0);
   for (int kc = 0; kc < nk; ++kc) {
     __syncthreads();
@@ -173,14 +175,14 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
     for (int j = 0; j < 2; ++j) {
       const int col = wn * 32 + j * 16 + (lane & 15);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) bf[j][q] = *reinterpret_cast<const bf16x8_t*>(&Bs[q][col][ko]);
+      for (int q = 0; q < 4; ++q) bf[j][q] = *reinterpret_cast<const bf16x8_t*>(&Bs[q][b3_lds(col, seg)]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int row = wm * 64 + i * 16 + (lane & 15);
       bf16x8_t a[PA];
 #pragma unroll
-      for (int q = 0; q < PA; ++q) a[q] = *reinterpret_cast<const bf16x8_t*>(&As[q][row][ko]);
+      for (int q = 0; q < PA; ++q) a[q] = *reinterpret_cast<const bf16x8_t*>(&As[q][b3_lds(row, seg)]);
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         f4_t cr = acc[0][i][j], ci = acc[1][i][j];

@@ -61,6 +61,44 @@ def test_error_codes_without_gpu():
     assert h.trot_orthonormalize(dummy, 2, 100, 70, dummy, 1 << 30, None) == -2  # n > 64
 
 
+def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
+    """Argument checks of the split-BF16 propagation / energy, split-K force bias and tile tuner entry
+    points return their codes before any GPU work is enqueued."""
+    from trot_amd import _lib
+
+    h = _lib.lib()
+    d = ctypes.c_void_p(16)
+    big = 1 << 40
+    # trotter / Taylor on split planes
+    assert h.trot_trotter_b3_symblk(None, d, d, d, 4, 8, 2, 0.1, 6, d, big, None) == -1
+    assert h.trot_trotter_b3_symblk(d, d, d, d, 0, 8, 2, 0.1, 6, d, big, None) == -1
+    assert h.trot_trotter_b3_symblk(d, d, d, d, 70000, 8, 2, 0.1, 6, d, big, None) == -1
+    need = h.trot_taylor_b3_workspace(4, 8, 2)
+    assert h.trot_trotter_b3_symblk(d, d, d, d, 4, 8, 2, 0.1, 6, d, need - 1, None) == -4
+    assert h.trot_taylor_apply_b3_symblk(d, d, 4, 8, 2, 0.1, -1, d, big, None) == -1
+    assert h.trot_onebody_b3_planes(None, 8, d, None) == -1
+    assert h.trot_onebody_b3_planes_bytes(129) == 2 * 256 * 256 * 2  # rows and columns padded to 128
+    # split-BF16 local energy
+    assert h.trot_local_energy_b3(None, None, d, None, d, None, 4, 8, 2, 0, 3, 0.0, d, d, big, None) == -1
+    assert h.trot_local_energy_b3(d, None, d, None, d, None, 4, 100, 65, 0, 3, 0.0, d, d, big, None) == -2
+    need = h.trot_local_energy_b3_workspace(4, 8, 3, 0)
+    assert need > 0
+    assert h.trot_local_energy_b3(d, None, d, None, d, None, 4, 8, 2, 0, 3, 0.0, d, d, need - 1, None) == -4
+    assert h.trot_rot_chol_b3(d, 3, 65, 8, d, None) == -2
+    assert h.trot_rot_chol_b3_bytes(5, 2, 8) == 2 * (8 * 64) * 128 * 2  # G padded to 4 blocks of 64 rows
+    # split-K force bias: no split (no workspace) at large W x G, slices for a small walker set
+    assert h.trot_force_bias_workspace(2048, 2560, 512 * 64) == 0
+    assert h.trot_force_bias_workspace(128, 800, 200 * 40) > 0
+    assert h.trot_force_bias_splitk(None, d, 4, 4, 4, 2.0, 0, d, None, 0, None) == -1
+    assert h.trot_force_bias_splitk(d, d, 128, 800, 8000, 2.0, 0, d, None, 0, None) == -4
+    # Taylor tile tuner and table
+    assert h.trot_set_taylor_tile(4, 8, 2, 1, 7) == -1
+    assert h.trot_autotune_taylor(0, 8, 2, 1, d, big, None, None, None) == -1
+    assert h.trot_autotune_taylor(4, 8, 2, 1, d, 0, None, None, None) == -4
+    assert h.trot_set_taylor_tile(3, 5, 2, 0, 3) == 0 and h.trot_get_taylor_tile(3, 5, 2, 0) == 3
+    assert h.trot_get_taylor_tile(3, 5, 2, 1) == 0  # other storage: default 64 x 64
+
+
 def test_workspace_queries_are_sizes():
     from trot_amd import _lib
 

@@ -24,6 +24,7 @@
 #define TROT_AMD_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -143,6 +144,26 @@ int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t w
 size_t trot_sr_workspace(int W);
 int trot_sr_comb(const double* weights, double zeta, int W, int* idx, double* weights_new, void* ws,
                  size_t ws_bytes, void* stream);
+
+/* trot_sr_comb with the comb offset zeta read from device memory (one double), so that the block
+ * tail can run inside a captured HIP graph with zeta drawn on the device (trot_philox_uniform).
+ * Replaces: _sr_indices with the uniform of trot/prop/blocks.py:270-271 (trot/walkers.py:249-254). */
+int trot_sr_comb_dev(const double* weights, const double* zeta, int W, int* idx, double* weights_new, void* ws,
+                     size_t ws_bytes, void* stream);
+
+/* Counter-based random numbers: Philox4x64-10 (Salmon et al., SC'11; numpy.random.Philox), block j of
+ * draw `counter` for purpose p is Philox(counter words (j, counter, p, 0), key words (seed, 0)).
+ * `state`, when non-null, is a device array {seed, counter} read in place of the by-value pair, so
+ * a captured graph draws new numbers on every replay; trot_rng_advance adds `by` to state[1].
+ * trot_philox_normal: n N(0,1) values, Box-Muller on the 53-bit uniforms of (x0, x1), (x2, x3) of block
+ *   j -> values 4j..4j+3 (purpose 0).  Replaces: the threefry field draw of trot/prop/afqmc.py:91-93.
+ * trot_philox_uniform: one uniform in [0, 1), (x0 >> 11) 2^-53 of block 0, purpose 1.
+ *   Replaces: the comb offset draw of trot/prop/blocks.py:270-271.
+ * trot_philox_raw: the raw 64-bit words of blocks 0..n_blocks-1 (4 per block), for checking. */
+int trot_philox_raw(uint64_t seed, uint64_t counter, uint64_t purpose, long n_blocks, uint64_t* out, void* stream);
+int trot_philox_normal(const uint64_t* state, uint64_t seed, uint64_t counter, long n, double* out, void* stream);
+int trot_philox_uniform(const uint64_t* state, uint64_t seed, uint64_t counter, double* out, void* stream);
+int trot_rng_advance(uint64_t* state, uint64_t by, void* stream);
 
 /* dst[i, :] = src[idx[i], :] for rows of row_doubles doubles (walkers, Green's functions, overlaps).
  * Replaces: the gather w[idx] of trot/walkers.py:272-278. */

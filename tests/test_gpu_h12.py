@@ -58,7 +58,7 @@ def test_h12_blocks_match_oracle(gpu, h12):
     for b in range(NBLK):
         fseq = [rng.standard_normal((W, G)) for _ in range(NPS)]
         seed, ctr = state.rng_key
-        zeta, _ = pctx.fields.uniform((seed, ctr + NPS))  # the draw the block's SR uses
+        zeta = float(pctx.fields.uniform((seed, ctr + NPS))[0])  # the draw the block's SR uses
         state, obs = blocks.block(state, sys=sys_, params=params, ham_data=ham, trial_data=tr, trial_ops=tops,
                                   meas_ops=mops, meas_ctx=mctx, prop_ops=pops, prop_ctx=pctx,
                                   fields_seq=[torch.from_numpy(f).cuda() for f in fseq])

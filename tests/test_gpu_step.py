@@ -159,7 +159,7 @@ def test_block_matches_oracle(gpu, kind):
     ref = O.init_prop_state(w0, ref_ops, W)
     fseq = p["fields"][:params.n_prop_steps]
     seed, ctr = state.rng_key
-    zeta, _ = pctx.fields.uniform((seed, ctr + params.n_prop_steps))  # the key the block's SR draw uses
+    zeta = float(pctx.fields.uniform((seed, ctr + params.n_prop_steps))[0])  # the block's SR draw
     state, obs = blocks.block(state, sys=sys, params=params, ham_data=ham, trial_data=tr, trial_ops=tops,
                               meas_ops=mops, meas_ctx=mctx, prop_ops=pops, prop_ctx=pctx,
                               fields_seq=[torch.from_numpy(f).cuda() for f in fseq])

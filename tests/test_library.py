@@ -97,6 +97,15 @@ def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
     assert h.trot_autotune_taylor(4, 8, 2, 1, d, 0, None, None, None) == -4
     assert h.trot_set_taylor_tile(3, 5, 2, 0, 3) == 0 and h.trot_get_taylor_tile(3, 5, 2, 0) == 3
     assert h.trot_get_taylor_tile(3, 5, 2, 1) == 0  # other storage: default 64 x 64
+    # counter-based random numbers and SR with a device comb offset
+    assert h.trot_philox_raw(0, 0, 0, 4, None, None) == -1
+    assert h.trot_philox_raw(0, 0, 0, -1, d, None) == -1
+    assert h.trot_philox_normal(None, 0, 0, 8, None, None) == -1
+    assert h.trot_philox_normal(None, 0, 0, -2, d, None) == -1
+    assert h.trot_philox_uniform(None, 0, 0, None, None) == -1
+    assert h.trot_rng_advance(None, 1, None) == -1
+    assert h.trot_sr_comb_dev(d, None, 4, d, d, d, big, None) == -1
+    assert h.trot_sr_comb_dev(d, d, 4, d, d, d, 0, None) == -4
 
 
 def test_workspace_queries_are_sizes():

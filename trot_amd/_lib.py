@@ -30,6 +30,7 @@ _i = ctypes.c_int
 _l = ctypes.c_long
 _d = ctypes.c_double
 _sz = ctypes.c_size_t
+_u64 = ctypes.c_uint64
 
 _SIGS = {
     "trot_version": (ctypes.c_char_p, []),
@@ -60,6 +61,11 @@ _SIGS = {
     "trot_sr_workspace": (_sz, [_i]),
     "trot_sr_comb": (_i, [_vp, _d, _i, _vp, _vp, _vp, _sz, _vp]),
     "trot_gather_rows": (_i, [_vp, _vp, _vp, _i, _l, _vp]),
+    "trot_sr_comb_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _sz, _vp]),
+    "trot_philox_raw": (_i, [_u64, _u64, _u64, _l, _vp, _vp]),
+    "trot_philox_normal": (_i, [_vp, _u64, _u64, _l, _vp, _vp]),
+    "trot_philox_uniform": (_i, [_vp, _u64, _u64, _vp, _vp]),
+    "trot_rng_advance": (_i, [_vp, _u64, _vp]),
     "trot_build_rot_chol": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
     "trot_build_prop_workspace": (_sz, [_i, _i]),
     "trot_build_prop_tensors": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
@@ -333,16 +339,73 @@ def orthonormalize(w: torch.Tensor) -> torch.Tensor:
     return out
 
 
-def sr_comb(weights: torch.Tensor, zeta: float):
+def sr_comb(weights: torch.Tensor, zeta):
+    """Comb indices and weights; ``zeta`` is a float or a one-element FP64 GPU tensor (read on the device)."""
     W = weights.shape[0]
     _req(weights, "weights", F64, (W,))
     L = lib()
     idx = torch.empty((W,), dtype=torch.int32, device=weights.device)
     w_new = torch.empty_like(weights)
     ws = _ws(L.trot_sr_workspace(W), weights.device)
-    _check(L.trot_sr_comb(_p(weights), float(zeta), W, _p(idx), _p(w_new), _p(ws), ws.numel(), _stream()),
-           "trot_sr_comb")
+    if isinstance(zeta, torch.Tensor):
+        _req(zeta, "zeta", F64, (1,))
+        _check(L.trot_sr_comb_dev(_p(weights), _p(zeta), W, _p(idx), _p(w_new), _p(ws), ws.numel(), _stream()),
+               "trot_sr_comb_dev")
+    else:
+        _check(L.trot_sr_comb(_p(weights), float(zeta), W, _p(idx), _p(w_new), _p(ws), ws.numel(), _stream()),
+               "trot_sr_comb")
     return idx, w_new
+
+
+# ---------------------------------------------------------------------------
+# counter-based random numbers (Philox4x64-10)
+# ---------------------------------------------------------------------------
+_U64 = (1 << 64) - 1
+
+
+def rng_state(seed: int, counter: int, device="cuda") -> torch.Tensor:
+    """A device {seed, counter} pair (int64 words holding the uint64 values)."""
+    words = [int(seed) & _U64, int(counter) & _U64]
+    words = [w - (1 << 64) if w >= (1 << 63) else w for w in words]
+    return torch.tensor(words, dtype=torch.int64, device=device)
+
+
+def _state_args(state, seed, counter):
+    if state is not None:
+        _req(state, "rng state", torch.int64, (2,))
+        return _p(state), 0, 0
+    return None, int(seed) & _U64, int(counter) & _U64
+
+
+def philox_raw(seed: int, counter: int, purpose: int, n_blocks: int, device="cuda") -> torch.Tensor:
+    """(n_blocks, 4) raw 64-bit words (as int64) of Philox4x64-10 blocks 0..n_blocks-1."""
+    out = torch.empty((n_blocks, 4), dtype=torch.int64, device=device)
+    _check(lib().trot_philox_raw(int(seed) & _U64, int(counter) & _U64, int(purpose) & _U64, int(n_blocks), _p(out),
+                                 _stream()), "trot_philox_raw")
+    return out
+
+
+def philox_normal(shape, *, state: torch.Tensor | None = None, seed: int = 0, counter: int = 0,
+                  device="cuda") -> torch.Tensor:
+    """N(0,1) FP64 values of draw ``counter`` (or of the device state's counter when ``state`` is given)."""
+    out = torch.empty(shape, dtype=F64, device=device)
+    sp, sd, ct = _state_args(state, seed, counter)
+    _check(lib().trot_philox_normal(sp, sd, ct, out.numel(), _p(out), _stream()), "trot_philox_normal")
+    return out
+
+
+def philox_uniform(*, state: torch.Tensor | None = None, seed: int = 0, counter: int = 0,
+                   device="cuda") -> torch.Tensor:
+    """One uniform in [0, 1) as a (1,) FP64 GPU tensor."""
+    out = torch.empty((1,), dtype=F64, device=device)
+    sp, sd, ct = _state_args(state, seed, counter)
+    _check(lib().trot_philox_uniform(sp, sd, ct, _p(out), _stream()), "trot_philox_uniform")
+    return out
+
+
+def rng_advance(state: torch.Tensor, by: int = 1) -> None:
+    _req(state, "rng state", torch.int64, (2,))
+    _check(lib().trot_rng_advance(_p(state), int(by), _stream()), "trot_rng_advance")
 
 
 def gather_rows(src: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:

@@ -59,7 +59,7 @@ def _tensors(state: PropState):
               state.node_encounters, state.green):
         if v is None:
             continue
-        out.extend(v if isinstance(v, tuple) else (v,))
+        out.extend(x for x in (v if isinstance(v, tuple) else (v,)) if x is not None)
     return out
 
 
@@ -69,7 +69,7 @@ def _static_copy(state: PropState) -> PropState:
         if v is None:
             return None
         if isinstance(v, tuple):
-            return tuple(x.detach().clone().contiguous() for x in v)
+            return tuple(None if x is None else x.detach().clone().contiguous() for x in v)
         return v.detach().clone().contiguous()
 
     green = state.green

@@ -502,7 +502,16 @@ int trot_sr_comb(const double* weights, double zeta, int W, int* idx, double* we
   if (!weights || !idx || !weights_new) return fail(TROT_ERR_INVALID, "sr_comb: null pointer");
   if (W < 1) return fail(TROT_ERR_INVALID, "sr_comb: bad sizes");
   if (ws_bytes < trot_sr_workspace(W)) return fail(TROT_ERR_WORKSPACE, "sr_comb: workspace too small");
-  HIPCHK(launch_sr_index(weights, (double*)ws, idx, weights_new, W, zeta, (hipStream_t)stream));
+  HIPCHK(launch_sr_index(weights, (double*)ws, idx, weights_new, W, zeta, nullptr, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+int trot_sr_comb_dev(const double* weights, const double* zeta, int W, int* idx, double* weights_new, void* ws,
+                     size_t ws_bytes, void* stream) {
+  if (!weights || !zeta || !idx || !weights_new) return fail(TROT_ERR_INVALID, "sr_comb_dev: null pointer");
+  if (W < 1) return fail(TROT_ERR_INVALID, "sr_comb_dev: bad sizes");
+  if (ws_bytes < trot_sr_workspace(W)) return fail(TROT_ERR_WORKSPACE, "sr_comb_dev: workspace too small");
+  HIPCHK(launch_sr_index(weights, (double*)ws, idx, weights_new, W, 0.0, zeta, (hipStream_t)stream));
   return TROT_OK;
 }
 

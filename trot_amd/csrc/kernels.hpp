@@ -14,8 +14,9 @@ hipError_t launch_weight_update(double* weights, const double* ov_old, const dou
                                 double h0p_re, double h0p_im, double w_floor, double w_cap, double damping,
                                 hipStream_t s);
 hipError_t launch_chol_rinv(const double* gram, double* rinv, int n, int W, hipStream_t s);
+// zeta_dev: comb offset read on the device when non-null (else zeta)
 hipError_t launch_sr_index(const double* weights, double* cw, int* idx, double* w_new, int W, double zeta,
-                           hipStream_t s);
+                           const double* zeta_dev, hipStream_t s);
 hipError_t launch_gather_rows(const double* src, double* dst, const int* idx, int W, long row, hipStream_t s);
 hipError_t launch_energy(const double* rc_a, const double* rc_b, const double* gt_a, const double* gt_b,
                          const double* h1t_a, const double* h1t_b, int na, int nb, int N, int G, int W, double h0,

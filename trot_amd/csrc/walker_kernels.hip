@@ -292,8 +292,9 @@ __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void sr_index_kernel(const double* __restrict__ weights, double* __restrict__ cw,
                                                         int* __restrict__ idx, double* __restrict__ w_new, int W,
-                                                        double zeta) {
+                                                        double zeta_v, const double* __restrict__ zeta_p) {
   const int t = threadIdx.x;
+  const double zeta = zeta_p ? zeta_p[0] : zeta_v;
   if (t == 0) {
     double acc = 0.0;
     for (int i = 0; i < W; ++i) {
@@ -402,9 +403,9 @@ hipError_t launch_chol_rinv(const double* gram, double* rinv, int n, int W, hipS
 }
 
 hipError_t launch_sr_index(const double* weights, double* cw, int* idx, double* w_new, int W, double zeta,
-                           hipStream_t s) {
+                           const double* zeta_dev, hipStream_t s) {
   if (W <= 0) return hipSuccess;
-  hipLaunchKernelGGL(sr_index_kernel, dim3(1), dim3(1024), 0, s, weights, cw, idx, w_new, W, zeta);
+  hipLaunchKernelGGL(sr_index_kernel, dim3(1), dim3(1024), 0, s, weights, cw, idx, w_new, W, zeta, zeta_dev);
   return hipGetLastError();
 }
 

@@ -78,19 +78,37 @@ class PropState(NamedTuple):
 
 
 class DeviceFields:
-    """N(0,1) fields drawn on the GPU, one torch Philox stream per (seed, counter)."""
+    """N(0,1) fields drawn on the GPU from the counter-based Philox4x64-10 stream: draw ``ctr`` of ``seed``
+    (trot_philox_normal), and the SR comb offset from the same stream (trot_philox_uniform, returned as a
+    one-element GPU tensor, so the block never waits on the host)."""
 
     def __call__(self, key, n_walkers: int, n_fields: int, device):
         seed, ctr = key
-        g = torch.Generator(device=device)
-        g.manual_seed((int(seed) * 1_000_003 + int(ctr)) & 0x7FFFFFFFFFFFFFFF)
-        x = torch.randn((n_walkers, n_fields), dtype=torch.float64, device=device, generator=g)
+        x = _lib.philox_normal((n_walkers, n_fields), seed=int(seed), counter=int(ctr), device=device)
         return x, (seed, ctr + 1)
 
     def uniform(self, key):
         seed, ctr = key
-        rng = np.random.default_rng([int(seed), 104729, int(ctr)])
-        return float(rng.uniform()), (seed, ctr + 1)
+        return _lib.philox_uniform(seed=int(seed), counter=int(ctr)), (seed, ctr + 1)
+
+
+class StateFields:
+    """The DeviceFields stream with (seed, counter) kept in a device array: each draw reads the counter
+    there and advances it on the device, so a captured HIP graph draws the next numbers on every replay.
+    The host key advances alongside (one per draw), as with DeviceFields."""
+
+    def __init__(self, state: torch.Tensor):
+        self.state = state
+
+    def __call__(self, key, n_walkers: int, n_fields: int, device):
+        x = _lib.philox_normal((n_walkers, n_fields), state=self.state, device=device)
+        _lib.rng_advance(self.state, 1)
+        return x, (key[0], key[1] + 1)
+
+    def uniform(self, key):
+        z = _lib.philox_uniform(state=self.state)
+        _lib.rng_advance(self.state, 1)
+        return z, (key[0], key[1] + 1)
 
 
 class NumpyFields:

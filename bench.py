@@ -114,6 +114,30 @@ class Runner:
         return self.pops.step(st, params=e["params"], ham_data=e["ham"], trial_data=e["tr"], trial_ops=e["tops"],
                               meas_ops=self.mops, meas_ctx=self.mctx, prop_ctx=self.pctx, fields=fields)
 
+    def timed_graph(self, st, warmup: int, steps: int, tag: str):
+        """Steps replayed from one recorded HIP graph (trot_amd.graph.GraphStep)."""
+        import torch
+
+        from trot_amd.graph import GraphStep
+
+        e = self.env
+        gs = GraphStep(self.pops)
+        kw = dict(params=e["params"], ham_data=e["ham"], trial_data=e["tr"], trial_ops=e["tops"],
+                  meas_ops=self.mops, meas_ctx=self.mctx, prop_ctx=self.pctx)
+        st = gs(st, **kw)  # eager warm-up step + recording
+        for i in range(warmup):
+            st = gs(st, **kw)
+        torch.cuda.synchronize()
+        log(f"{tag}: recorded, {warmup} warm-up replays done")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = gs(st, **kw)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log(f"{tag}: {steps} steps in {dt:.3f}s ({1e3 * dt / steps:.2f} ms/step)")
+        del gs
+        return st, dt
+
     def timed(self, st, warmup: int, steps: int, tag: str):
         import torch
 
@@ -166,6 +190,9 @@ def main():
     log(f"mixed contexts ready at {time.perf_counter() - t0:.1f}s (sym-block VHS: {bool(mixed.pctx.low.get('symblk'))})")
     st_mix, t_mix, vhs_ms = mixed.timed(st_mix, args.warmup, args.steps, f"mixed {args.precision}")
     value = W * args.steps / t_mix
+    # the same step replayed from one HIP graph per step (fields drawn on the device)
+    st_mix, t_mix_graph = mixed.timed_graph(st_mix, args.warmup, args.steps, f"mixed {args.precision} graph")
+    rate_graph = W * args.steps / t_mix_graph
 
     fp64 = None
     rate64 = None
@@ -200,6 +227,8 @@ def main():
                 "launch_ms": round(vhs_ms, 3), "flop_per_launch": vhs_flop * passes}
 
     block_rate = None
+    block_graph = None
+    small = None
     if not args.no_block:
         # The tail is timed on walkers that stay finite: with the recipe's unscaled L the walkers overflow
         # within a few steps, and the exchange GEMM then runs on degenerate operands about twice as fast
@@ -211,7 +240,11 @@ def main():
         for _ in range(2):
             st_blk = mixed_blk.step(st_blk)
         block_rate = block_inclusive(env_blk, mixed_blk, st_blk, t_mix / args.steps)
+        block_graph = graph_block_rate(env_blk, mixed_blk, st_blk)
         del mixed_blk, st_blk, env_blk
+
+    if not args.no_block:
+        small = launch_bound_case()
 
     cpu = None if args.no_cpu else cpu_baseline(syn, mixed, args, dt)
 
@@ -237,7 +270,10 @@ def main():
         "fp64_ms_per_step": None if rate64 is None else round(1e3 * W / rate64, 3),
         "speedup_vs_fp64": None if rate64 is None else round(value / rate64, 3),
         "accuracy_vs_fp64": accuracy,
+        "graph_step_walker_steps_per_s": round(rate_graph, 2),
         "block_inclusive_walker_steps_per_s": None if block_rate is None else round(block_rate, 2),
+        "block_graph": block_graph,
+        "launch_bound_case": small,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -314,6 +350,97 @@ def block_inclusive(env, r: Runner, st, t_step: float) -> float:
     rate = params.n_walkers * nps / (nps * t_step + t_tail)
     log(f"block tail {1e3 * t_tail:.1f} ms; block-inclusive {rate:.1f} walker-steps/s")
     return rate
+
+
+def graph_block_rate(env, r: Runner, st) -> dict:
+    """One whole block (n_prop_steps steps + tail) replayed from one HIP graph (trot_amd.graph.GraphBlock)."""
+    import torch
+
+    from trot_amd.graph import GraphBlock
+
+    params = env["params"]
+    kw = dict(sys=env["sys"], params=params, ham_data=env["ham"], trial_data=env["tr"], trial_ops=env["tops"],
+              meas_ops=r.mops, meas_ctx=r.mctx, prop_ops=r.pops, prop_ctx=r.pctx)
+    gb = GraphBlock()
+    t0 = time.perf_counter()
+    st, _ = gb(st, **kw)
+    torch.cuda.synchronize()
+    t_rec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st, obs = gb(st, **kw)
+    torch.cuda.synchronize()
+    t_blk = time.perf_counter() - t0
+    rate = params.n_walkers * params.n_prop_steps / t_blk
+    log(f"graph block: eager block + recording {t_rec:.2f}s, replay {1e3 * t_blk:.1f} ms "
+        f"({rate:.1f} walker-steps/s incl. tail), energy {float(obs.scalars['energy']):.6f}")
+    del gb
+    return {"walker_steps_per_s": round(rate, 2), "ms_per_block": round(1e3 * t_blk, 2),
+            "n_prop_steps": params.n_prop_steps}
+
+
+def launch_bound_case() -> dict:
+    """FP64 step and block at N=12, n=6, G=72, W=64 (the H12 STO-6G sizes): eager launches vs HIP graphs.
+
+    The synthetic system of that size stands in for the chain (same shapes, same kernels)."""
+    import torch
+
+    from trot_amd import prop, synthetic, trial
+    from trot_amd.core import HamChol, System
+    from trot_amd.graph import GraphBlock, GraphStep
+
+    N, n, G, W = 12, 6, 72, 64
+    syn = synthetic.make_synthetic(N, n, G, W, 5, 64.0, device=torch.device("cuda"), chol_scale=1.0 / np.sqrt(N))
+    sys_ = System(N, (n, n), "restricted")
+    env = dict(sys=sys_, ham=HamChol(syn.h0, torch.from_numpy(syn.h1).cuda(), syn.chol),
+               tr=trial.RhfTrial(syn.trial), tops=trial.make_rhf_trial_ops(sys_),
+               params=prop.QmcParams(dt=0.005, n_walkers=W, seed=5, n_prop_steps=50), w0=syn.walkers)
+    r = Runner(env, "f64", "f64")
+    kw = dict(params=env["params"], ham_data=env["ham"], trial_data=env["tr"], trial_ops=env["tops"],
+              meas_ops=r.mops, meas_ctx=r.mctx, prop_ctx=r.pctx)
+    steps = 200
+    st = r.init()
+    for _ in range(5):
+        st = r.step(st)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = r.step(st)
+    torch.cuda.synchronize()
+    t_eager = (time.perf_counter() - t0) / steps
+    gs = GraphStep(r.pops)
+    for _ in range(5):
+        st = gs(st, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = gs(st, **kw)
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / steps
+    del gs
+    from trot_amd.blocks import block
+
+    bkw = dict(kw, sys=sys_, prop_ops=r.pops)
+    st, _ = block(st, **bkw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, _ = block(st, **bkw)
+    torch.cuda.synchronize()
+    tb_eager = time.perf_counter() - t0
+    gb = GraphBlock()
+    st, _ = gb(st, **bkw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n_blk = 5
+    for _ in range(n_blk):
+        st, _ = gb(st, **bkw)
+    torch.cuda.synchronize()
+    tb_graph = (time.perf_counter() - t0) / n_blk
+    out = {"workload": "n12_o6_g72_w64 FP64 (H12 STO-6G sizes, synthetic)", "eager_ms_per_step": round(1e3 * t_eager, 4),
+           "graph_ms_per_step": round(1e3 * t_graph, 4), "eager_ms_per_block": round(1e3 * tb_eager, 3),
+           "graph_ms_per_block": round(1e3 * tb_graph, 3), "n_prop_steps": 50,
+           "graph_block_walker_steps_per_s": round(W * 50 / tb_graph, 1)}
+    log(f"launch-bound case: {out}")
+    return out
 
 
 def cpu_baseline(syn, r: Runner, args, dt):

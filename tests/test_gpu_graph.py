@@ -4,6 +4,7 @@
   pinned to numpy.random.Philox), normals against the twin's Box-Muller to 1e-13 (libm rounding
   of log / sincos), the comb offset bit for bit, device-counter mode equal to by-value mode;
 * SR with the comb offset read on the device equals SR with it passed by value;
+* GraphStep (one HIP graph per step) against eager steps, bit for bit, FP64 and mixed;
 * GraphBlock (one HIP graph per block) against the eager block with the same device stream: every
   state tensor and both block scalars bit-identical over three blocks (a warm-up block and two
   replays, with unrelated eager GPU work and host synchronisation between replays), for restricted,
@@ -141,6 +142,30 @@ def test_graph_block_matches_eager(gpu, kind, precision, shape):
     for x, y in zip(_flat(s_x), _flat(eager[2][0])):
         assert torch.equal(x, y)
     assert torch.equal(o_x.scalars["energy"], eager[2][1].scalars["energy"])
+
+
+@pytest.mark.parametrize("precision", ["f64", "bf16x3"])
+def test_graph_step_matches_eager(gpu, precision):
+    from trot_amd.graph import GraphStep
+
+    print(f"[graph step] {precision}", flush=True)
+    kw, init = _setup("restricted", precision, 40, 6, 50, 24)
+    step_kw = {k: v for k, v in kw.items() if k not in ("sys", "prop_ops")}
+    pops = kw["prop_ops"]
+    s_e = init()
+    eager = []
+    for _ in range(4):
+        s_e = pops.step(s_e, **step_kw)
+        eager.append(s_e)
+    gs = GraphStep(pops)
+    s_g = init()
+    for i in range(4):
+        s_g = gs(s_g, **step_kw)
+        torch.cuda.synchronize()
+        assert s_g.rng_key == eager[i].rng_key
+        for x, y in zip(_flat(s_g), _flat(eager[i])):
+            assert torch.equal(x, y), i
+        torch.randn(4096, device="cuda").sum().item()
 
 
 def test_run_qmc_graph_equals_eager(gpu):

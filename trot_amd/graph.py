@@ -26,7 +26,91 @@ from .blocks import BlockObs, block as eager_block
 from .prop import DeviceFields, StateFields
 
 
-class GraphBlock:
+class _Recorded:
+    """Static state buffers, a device counter and one recorded HIP graph of ``run(static) -> (state, scalars)``."""
+
+    def __init__(self):
+        self.graph = None
+        self.static = None
+        self.last = None
+        self.rng = None
+        self.obs = None
+        self.replays = 0
+
+    def _store(self, state_out, scalars):
+        for dst, src in zip(_tensors(self.static), _tensors(state_out)):
+            dst.copy_(src)
+        if scalars is not None:
+            self.obs.copy_(torch.stack([x.reshape(()) for x in scalars]))
+
+    def _load(self, state):
+        if state.green is None:
+            raise ValueError("the state needs its Green's-function cache (PropState.green)")
+        dst, new = _tensors(self.static), _tensors(state)
+        if len(dst) != len(new) or any(a.shape != b.shape for a, b in zip(dst, new)):
+            raise ValueError("the state does not match the recorded shapes")
+        for a, b in zip(dst, new):
+            a.copy_(b)
+        self.rng.copy_(_lib.rng_state(*state.rng_key))
+
+    def _device_ctx(self, kw):
+        pctx = kw["prop_ctx"]
+        if not isinstance(pctx.fields, (DeviceFields, StateFields)):
+            raise ValueError("recording needs the device field stream (DeviceFields) on the propagation context")
+        out = dict(kw)
+        out["prop_ctx"] = dataclasses.replace(pctx, fields=StateFields(self.rng))
+        return out
+
+    def _run(self, state, kw, run, n_draws, n_scalars):
+        seed, ctr = state.rng_key
+        if self.graph is None:
+            if state.green is None:
+                _, g = kw["meas_ops"].engine.overlap_green(state.walkers, kw["trial_data"])
+                state = state._replace(green=g)
+            self.static = _static_copy(state)
+            self.rng = _lib.rng_state(seed, ctr)
+            self.obs = torch.zeros((max(n_scalars, 1),), dtype=torch.float64, device="cuda")
+            args = self._device_ctx(kw)
+            # warm-up: eager, on a side stream (graph capture requires it); it is the first call's result
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                self._store(*run(self.static, args))
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self._store(*run(self.static, args))
+            # recording does not run the work: the buffers hold the warm-up's result and the device
+            # counter has advanced by the warm-up's draws only
+        else:
+            if self.last is None or state.walkers is not self.last.walkers:
+                self._load(state)
+            self.graph.replay()
+        self.replays += 1
+        self.last = self.static._replace(rng_key=(seed, ctr + n_draws))
+        return self.last
+
+
+class GraphStep(_Recorded):
+    """``prop_ops.step`` recorded as one HIP graph: callable with the step's signature (no ``fields``:
+    they are drawn on the device).  The first call runs one step eagerly and records the next."""
+
+    def __init__(self, prop_ops):
+        super().__init__()
+        self.prop_ops = prop_ops
+
+    def __call__(self, state, **kw):
+        if kw.pop("fields", None) is not None:
+            raise ValueError("GraphStep draws its fields on the device; fields is not supported")
+
+        def run(st, args):
+            return self.prop_ops.step(st, **args), None
+
+        return self._run(state, kw, run, 1, 0)
+
+
+class GraphBlock(_Recorded):
     """Callable with the signature of ``trot_amd.blocks.block``; records itself on first use.
 
     The recording is tied to the shapes and contexts of the first call (walker count, Hamiltonian,
@@ -36,88 +120,22 @@ class GraphBlock:
     """
 
     def __init__(self, block_fn=eager_block):
+        super().__init__()
         self.block_fn = block_fn
-        self.graph = None
-        self.static = None
-        self.last = None
-        self.rng = None
-        self.obs = None
-        self.blocks_done = 0
 
-    # -- helpers -----------------------------------------------------------
-    def _kw(self, kw, pctx):
-        out = dict(kw)
-        out["prop_ctx"] = pctx
-        return out
+    @property
+    def blocks_done(self) -> int:
+        return self.replays
 
-    def _store(self, state_out, obs):
-        for dst, src in zip(_tensors(self.static), _tensors(state_out)):
-            dst.copy_(src)
-        self.obs.copy_(torch.stack([obs.scalars["energy"].reshape(()), obs.scalars["weight"].reshape(())]))
-
-    def _load(self, state):
-        src = state
-        if src.green is None:
-            raise ValueError("GraphBlock: the state needs its Green's-function cache (PropState.green)")
-        dst = _tensors(self.static)
-        new = _tensors(src)
-        if len(dst) != len(new) or any(a.shape != b.shape for a, b in zip(dst, new)):
-            raise ValueError("GraphBlock: the state does not match the recorded shapes")
-        for a, b in zip(dst, new):
-            a.copy_(b)
-        self.rng.copy_(_lib.rng_state(*state.rng_key))
-
-    def _result(self, key):
-        self.last = self.static._replace(rng_key=key)
-        obs = BlockObs(scalars={"energy": self.obs[0].clone(), "weight": self.obs[1].clone()}, observables={})
-        return self.last, obs
-
-    # -- the block -----------------------------------------------------------
     def __call__(self, state, **kw):
-        if kw.get("fields_seq") is not None:
+        if kw.pop("fields_seq", None) is not None:
             raise ValueError("GraphBlock draws its fields on the device; fields_seq is not supported")
-        kw.pop("fields_seq", None)
-        pctx = kw["prop_ctx"]
-        params = kw["params"]
-        if not isinstance(pctx.fields, (DeviceFields, StateFields)):
-            raise ValueError("GraphBlock needs the device field stream (DeviceFields) on the propagation context")
-        n_draws = params.n_prop_steps + 1  # one field draw per step, one comb offset
-        if self.graph is None:
-            return self._capture(state, kw, n_draws)
-        if self.last is None or state.walkers is not self.last.walkers:
-            self._load(state)
-        self.graph.replay()
-        self.blocks_done += 1
-        seed, ctr = state.rng_key
-        return self._result((seed, ctr + n_draws))
 
-    def _capture(self, state, kw, n_draws):
-        if state.green is None:
-            engine = kw["meas_ops"].engine
-            _, g = engine.overlap_green(state.walkers, kw["trial_data"])
-            state = state._replace(green=g)
-        seed, ctr = state.rng_key
-        self.static = _static_copy(state)
-        self.rng = _lib.rng_state(seed, ctr)
-        self.obs = torch.zeros((2,), dtype=torch.float64, device="cuda")
-        pctx = dataclasses.replace(kw["prop_ctx"], fields=StateFields(self.rng))
-        args = self._kw(kw, pctx)
+        def run(st, args):
+            out, obs = self.block_fn(st, **args)
+            return out, (obs.scalars["energy"], obs.scalars["weight"])
 
-        # warm-up: the first block, eager, on a side stream (graph capture requires it)
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):
-            out, obs = self.block_fn(self.static, **args)
-            self._store(out, obs)
-        torch.cuda.current_stream().wait_stream(side)
-        torch.cuda.synchronize()
-        del out, obs
-
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            out, obs = self.block_fn(self.static, **args)
-            self._store(out, obs)
-        # recording does not run the block: the buffers hold the warm-up block's result and the
-        # device counter has advanced by that block's draws only
-        self.blocks_done = 1
-        return self._result((seed, ctr + n_draws))
+        n_draws = kw["params"].n_prop_steps + 1  # one field draw per step, one comb offset
+        st = self._run(state, kw, run, n_draws, 2)
+        obs = BlockObs(scalars={"energy": self.obs[0].clone(), "weight": self.obs[1].clone()}, observables={})
+        return st, obs

@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--workload", default="n512_o64_g2560_w2048")
     ap.add_argument("--precision", default="bf16x3", choices=["f32", "bf16x3"])
     ap.add_argument("--taylor-precision", default="bf16x3", choices=["f64", "f32", "bf16x3"])
+    ap.add_argument("--taylor-hi-from", type=int, default=3,
+                    help="first Taylor term computed as one bf16 pass in the split-BF16 propagation (0: none)")
     ap.add_argument("--accuracy-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-block", action="store_true", help="skip the block-tail timing")
@@ -92,13 +94,13 @@ def traffic_from_profiles(tag: str):
 class Runner:
     """One precision configuration of the step on a shared synthetic system."""
 
-    def __init__(self, env, precision: str, taylor_precision: str):
+    def __init__(self, env, precision: str, taylor_precision: str, hi_from: int = 0):
         from trot_amd import meas, prop
 
         self.env = env
         self.mops = meas.make_rhf_meas_ops(env["sys"], precision=precision)
         self.pops = prop.make_prop_ops("restricted", "restricted", precision=precision,
-                                       taylor_precision=taylor_precision)
+                                       taylor_precision=taylor_precision, taylor_hi_from=hi_from)
         self.pctx = self.pops.build_prop_ctx(env["ham"], env["tops"].get_rdm1(env["tr"]), env["params"])
         self.mctx = self.mops.build_meas_ctx(env["ham"], env["tr"])
         self.precision = precision
@@ -184,7 +186,7 @@ def main():
                tr=trial.RhfTrial(syn.trial), tops=trial.make_rhf_trial_ops(sys_),
                params=prop.QmcParams(dt=dt, n_walkers=W, seed=cfg["seed"]), w0=syn.walkers)
 
-    mixed = Runner(env, args.precision, args.taylor_precision)
+    mixed = Runner(env, args.precision, args.taylor_precision, args.taylor_hi_from)
     st_mix = mixed.init()
     torch.cuda.synchronize()
     log(f"mixed contexts ready at {time.perf_counter() - t0:.1f}s (sym-block VHS: {bool(mixed.pctx.low.get('symblk'))})")
@@ -207,7 +209,7 @@ def main():
         # same sizes and draws, L scaled by 1/sqrt(N) so that walkers survive (see synthetic.make_synthetic)
         del st64, st_mix
         env_acc = dict(env, ham=HamChol(syn.h0, env["ham"].h1, syn.chol * (1.0 / np.sqrt(N))))
-        mixed_acc = Runner(env_acc, args.precision, args.taylor_precision)
+        mixed_acc = Runner(env_acc, args.precision, args.taylor_precision, args.taylor_hi_from)
         fp64_acc = Runner(env_acc, "f64", "f64")
         accuracy = paired_accuracy(env_acc, mixed_acc, fp64_acc, args.accuracy_steps)
         accuracy["system"] = "same sizes and seed, every L_gamma scaled by 1/sqrt(N)"
@@ -235,7 +237,7 @@ def main():
         # (measured) as on real ones.  The step itself times the same on both (21.7 vs 22.3 ms).
         env_blk = dict(env, ham=HamChol(syn.h0, env["ham"].h1, syn.chol * (1.0 / np.sqrt(N))))
         del st_mix
-        mixed_blk = Runner(env_blk, args.precision, args.taylor_precision)
+        mixed_blk = Runner(env_blk, args.precision, args.taylor_precision, args.taylor_hi_from)
         st_blk = mixed_blk.init()
         for _ in range(2):
             st_blk = mixed_blk.step(st_blk)
@@ -260,11 +262,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": f"mixed: {args.precision} MFMA Cholesky GEMMs (VHS, force bias, exchange), {args.taylor_precision} "
-                 f"Taylor and one-body products; f64 walkers, det, inverse, QR, weights and energy sums",
+                 f"one-body products and Taylor terms"
+                 + (f" 1-{args.taylor_hi_from - 1}, one-pass bf16 Taylor terms {args.taylor_hi_from}-6"
+                    if args.taylor_hi_from > 0 else "")
+                 + "; f64 walkers, det, inverse, QR, weights and energy sums",
         "data": "synthetic (seeded numpy PCG64 Hamiltonian, trial and walkers; on-device N(0,1) fields)",
         "config": {"workload": args.workload, "nbasis": N, "nocc": n, "nchol": G, "n_walkers": W, "dt": dt,
                    "n_exp_terms": env["params"].n_exp_terms, "trial": "RHF", "walker_kind": "restricted",
                    "precision": args.precision, "taylor_precision": args.taylor_precision,
+                   "taylor_hi_from": args.taylor_hi_from,
                    "vhs_layout": "sym-block" if sym else "dense"},
         "fp64_walker_steps_per_s": None if rate64 is None else round(rate64, 2),
         "fp64_ms_per_step": None if rate64 is None else round(1e3 * W / rate64, 3),

@@ -209,6 +209,8 @@ size_t trot_b3_planes_bytes(int rows, int K, int is_complex);
 int trot_split_bf16(const void* src, int src_f32, int is_complex, int transposed, long ld, int rows, int K,
                     void* dst, void* stream);
 size_t trot_gemm_b3_workspace(int M, int N, int a_complex, int ksplit);
+/* c_f64: 0 FP32 (complex64) output, 1 FP64 (complex128), 2 split complex64 for trot_trotter_b3v (complex A,
+ * beta 0, no split K): two planes of M x ldc words bf16(re) | bf16(im) << 16, hi then lo. */
 int trot_gemm_b3(const void* a_planes, int a_complex, const void* b_planes, int M, int N, int K, void* C, int c_f64,
                  long ldc, double alpha_re, double alpha_im, double beta, int ksplit, void* ws, size_t ws_bytes,
                  void* stream);
@@ -235,6 +237,17 @@ int trot_onebody_b3_planes(const double* expm, int N, void* planes, void* stream
  * _apply_one_body_half_array (:130-131), _apply_two_body_array (:152-162), _apply_one_body_half_array. */
 int trot_trotter_b3_symblk(const float* vhs_symblk, const void* onebody_planes, const double* w_in, double* w_out,
                            int W, int N, int n, double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, void* stream);
+
+/* trot_trotter_b3_symblk with V as split planes: v_planes = [hi | lo], each (W, trot_symblk_cols(N)) 32-bit
+ * words bf16(re) | bf16(im) << 16 with lo = bf16(x - hi), as trot_gemm_b3 writes with c_f64 = 2 (the
+ * same bf16 operands the complex64 form makes in LDS, so hi_from > n_terms gives identical results).
+ * Taylor terms k >= hi_from are one bf16 pass on the hi planes of V and of the term (half the V and term
+ * bytes, a quarter of the products): term k is smaller than the first by about (sqrt(dt) |V|)^(k-1) / k!,
+ * so from k = 3 on a 2^-8 relative product error stays at the size of the split-BF16 error of term 1.
+ * Replaces: as trot_trotter_b3_symblk (trot/prop/chol_afqmc_ops.py:198-222, trot/prop/utils.py:20-29). */
+int trot_trotter_b3v(const void* v_planes, int hi_from, const void* onebody_planes, const double* w_in,
+                     double* w_out, int W, int N, int n, double sqrt_dt, int n_terms, void* ws, size_t ws_bytes,
+                     void* stream);
 
 /* Split-BF16 planes of rot_chol (G, n, N) FP64 for trot_local_energy_b3, trot_rot_chol_b3_bytes(G, n, N) bytes;
  * made once per measurement context (trot/meas/rhf.py:308-319 builds rot_chol). */

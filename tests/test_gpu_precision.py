@@ -169,6 +169,69 @@ def test_trotter_b3_symblk(gpu, shape, n_terms):
         _lib.trotter_b3_symblk(v32, wt, _lib.onebody_b3_planes(_t(np.eye(N + 128))), sdt, n_terms)
 
 
+def _bf16_split_host(x: np.ndarray):
+    """hi = bf16(x), lo = bf16(x - hi), round to nearest even, as 16-bit words (x float32, finite)."""
+    def rne(v):
+        u = v.astype(np.float32).view(np.uint32).astype(np.uint64)
+        return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint32)
+
+    hi = rne(x)
+    lo = rne(x - (hi << 16).view(np.float32))
+    return hi, lo
+
+
+def _split_v_case(shape, seed):
+    from trot_amd import _b3, _lib
+
+    N, n, G, W = shape
+    p = problem(N, n, G, W, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    xbar = rng.standard_normal((W, G)) + 1j * rng.standard_normal((W, G))
+    chol_planes = _b3.split(_lib.pack_symblk(_t(p["chol"])), transposed=True)
+    xp = _b3.split(_t(xbar))
+    v64 = torch.empty((W, chol_planes.rows), dtype=torch.complex64, device="cuda")
+    _b3.gemm_nt(xp, chol_planes, v64)
+    vs = _b3.gemm_nt_split(xp, chol_planes)
+    v_ref = O.make_vhs(xbar, p["chol"].reshape(G, -1), N)
+    w = rng.standard_normal((W, N, n)) + 1j * rng.standard_normal((W, N, n))
+    return p, v_ref, v64, vs, w
+
+
+@pytest.mark.parametrize("shape", [(64, 16, 30, 7), (37, 5, 21, 3), (512, 64, 40, 9)])
+def test_split_v_planes_and_propagation(gpu, shape):
+    """VHS written as split planes: the planes are the bf16 hi / lo split of the complex64 VHS bit for bit;
+    with every term on both planes the propagation equals the complex64 form bit for bit; with terms
+    k >= 3 as one bf16 pass, the result moves by no more than 2^-7 of the size of those terms."""
+    import scipy.linalg
+    from trot_amd import _lib
+
+    N = shape[0]
+    p, v_ref, v64, vs, w = _split_v_case(shape, 41)
+    vc = torch.view_as_real(v64).cpu().numpy()
+    words = vs.cpu().numpy().view(np.uint32)
+    re_hi, re_lo = _bf16_split_host(vc[..., 0])
+    im_hi, im_lo = _bf16_split_host(vc[..., 1])
+    assert np.array_equal(words[0] & 0xFFFF, re_hi) and np.array_equal(words[0] >> 16, im_hi)
+    assert np.array_equal(words[1] & 0xFFFF, re_lo) and np.array_equal(words[1] >> 16, im_lo)
+
+    E = scipy.linalg.expm(-0.05 * p["h1"])
+    planes = _lib.onebody_b3_planes(_t(E))
+    sdt, K = 0.1, 6
+    wt = _t(w)
+    full = _lib.trotter_b3_symblk(v64, wt, planes, sdt, K)
+    same = _lib.trotter_b3v(vs, wt, planes, sdt, K, K + 1)
+    assert torch.equal(full, same)
+    hi3 = _lib.trotter_b3v(vs, wt, planes, sdt, K, 3)
+    assert not torch.equal(full, hi3)
+    ew = np.einsum("ij,wjp->wip", E, w)
+    late = np.einsum("ij,wjp->wip", E, O.taylor_expm_action(1j * sdt, v_ref, ew, K)
+                     - O.taylor_expm_action(1j * sdt, v_ref, ew, 2))  # E (T_3 + ... + T_6)
+    d = np.abs(hi3.cpu().numpy() - full.cpu().numpy()).max()
+    assert d <= 2.0**-7 * np.abs(late).max()
+    ref = np.einsum("ij,wjp->wip", E, O.taylor_expm_action(1j * sdt, v_ref, ew, K))
+    assert rel(hi3.cpu().numpy() - w, ref - w) < 4 * TOL["bf16x3"]
+
+
 @pytest.mark.parametrize("prec,tprec", [("f32", "f32"), ("bf16x3", "f32"), ("bf16x3", "bf16x3"), ("f32", "f64")])
 def test_mixed_steps_track_fp64(gpu, prec, tprec):
     """A few mixed-precision steps stay close to the FP64 oracle with identical fields."""

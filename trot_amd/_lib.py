@@ -62,6 +62,7 @@ _SIGS = {
     "trot_sr_comb": (_i, [_vp, _d, _i, _vp, _vp, _vp, _sz, _vp]),
     "trot_gather_rows": (_i, [_vp, _vp, _vp, _i, _l, _vp]),
     "trot_sr_comb_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _sz, _vp]),
+    "trot_trotter_b3v": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
     "trot_philox_raw": (_i, [_u64, _u64, _u64, _l, _vp, _vp]),
     "trot_philox_normal": (_i, [_vp, _u64, _u64, _l, _vp, _vp]),
     "trot_philox_uniform": (_i, [_vp, _u64, _u64, _vp, _vp]),
@@ -595,6 +596,26 @@ def trotter_b3_symblk(vpk32: torch.Tensor, w: torch.Tensor, onebody_planes: torc
     ws = _ws(L.trot_taylor_b3_workspace(W, N, n), w.device)
     _check(L.trot_trotter_b3_symblk(_p(vpk32), _p(onebody_planes), _p(w), _p(out), W, N, n, float(sqrt_dt),
                                     int(n_terms), _p(ws), ws.numel(), _stream()), "trot_trotter_b3_symblk")
+    return out
+
+
+def trotter_b3v(vsplit: torch.Tensor, w: torch.Tensor, onebody_planes: torch.Tensor, sqrt_dt: float, n_terms: int,
+                hi_from: int, out: torch.Tensor | None = None) -> torch.Tensor:
+    """trotter_b3_symblk with V as split planes (int32 (2, W, symblk_cols(N)), from _b3.gemm_nt_split);
+    Taylor terms k >= hi_from are one bf16 pass on the hi planes of V and of the term."""
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    _req(vsplit, "vhs (split planes)", torch.int32, (2, W, symblk_cols(N)))
+    L = lib()
+    if onebody_planes.dtype != torch.int16 or onebody_planes.numel() * 2 != L.trot_onebody_b3_planes_bytes(N):
+        raise ValueError("trotter_b3v: onebody_planes must come from onebody_b3_planes for this N")
+    if out is None:
+        out = torch.empty_like(w)
+    else:
+        _req(out, "out", C128, (W, N, n))
+    ws = _ws(L.trot_taylor_b3_workspace(W, N, n), w.device)
+    _check(L.trot_trotter_b3v(_p(vsplit), int(hi_from), _p(onebody_planes), _p(w), _p(out), W, N, n, float(sqrt_dt),
+                              int(n_terms), _p(ws), ws.numel(), _stream()), "trot_trotter_b3v")
     return out
 
 

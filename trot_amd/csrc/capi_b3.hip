@@ -50,7 +50,9 @@ size_t trot_gemm_b3_workspace(int M, int N, int a_complex, int ksplit) {
   return align256((size_t)ksplit * M * N * (a_complex ? 8 : 4));
 }
 
-// C (M x N, ldc; complex if a_complex, FP64 if c_f64 else FP32) = alpha * A . B^T (+ beta C)
+// C (M x N, ldc; complex if a_complex, FP64 if c_f64 == 1 else FP32) = alpha * A . B^T (+ beta C);
+// c_f64 == 2 (complex A, beta 0): C is split complex64, two planes of M x ldc 32-bit words
+// (bf16 re | bf16 im << 16), hi = bf16(x) then lo = bf16(x - hi).
 // A: split planes (M rows, K), B: split planes (N rows, K), both as made by trot_split_bf16.
 // ksplit > 1 splits K over blockIdx.y with FP32 partials in ws and an FP64 reduction
 // (complex FP64 output only).
@@ -60,6 +62,9 @@ int trot_gemm_b3(const void* a_planes, int a_complex, const void* b_planes, int 
   if (!a_planes || !b_planes || !C) return fail(TROT_ERR_INVALID, "gemm_b3: null pointer");
   if (M < 1 || N < 1 || K < 1 || ldc < N) return fail(TROT_ERR_INVALID, "gemm_b3: bad sizes");
   if (!a_complex && alpha_im != 0.0) return fail(TROT_ERR_INVALID, "gemm_b3: complex alpha needs complex A");
+  if (c_f64 < 0 || c_f64 > 2) return fail(TROT_ERR_INVALID, "gemm_b3: bad output kind");
+  if (c_f64 == 2 && (!a_complex || beta != 0.0 || ksplit > 1))
+    return fail(TROT_ERR_UNSUPPORTED, "gemm_b3: split complex64 output needs complex A, beta 0 and no split K");
   const int kp = (int)trot_b3_k_pad(K);
   if (ksplit < 1) ksplit = 1;
   int kslice = (int)pad_to((kp + ksplit - 1) / ksplit, 32);
@@ -94,7 +99,9 @@ int trot_gemm_b3(const void* a_planes, int a_complex, const void* b_planes, int 
     const long T2 = (long)n_mt * n_nt2;
     dim3 grid2((unsigned)((T2 + 7) / 8 * 8), (unsigned)ksplit);
     const int b_rows = (int)trot_b3_rows_pad(N);
-    if (c_f64)
+    if (c_f64 == 2)
+      hipLaunchKernelGGL((gemm_b3w_kernel<unsigned>), grid2, dim3(512), 0, s, p, n_mt, n_nt2, b_rows);
+    else if (c_f64)
       hipLaunchKernelGGL((gemm_b3w_kernel<double>), grid2, dim3(512), 0, s, p, n_mt, n_nt2, b_rows);
     else
       hipLaunchKernelGGL((gemm_b3w_kernel<float>), grid2, dim3(512), 0, s, p, n_mt, n_nt2, b_rows);

@@ -83,7 +83,15 @@ __device__ __forceinline__ void b3_epilogue(const B3P& p, const f4_t (&acc)[AC ?
         if constexpr (AC) {
           const float xr = acc[0][i][j][r], xi = acc[1][i][j][r];
           const float vr = p.ar * xr - p.ai * xi, vi = p.ar * xi + p.ai * xr;
-          if constexpr (sizeof(TC) == 8) {
+          if constexpr (__is_same(TC, unsigned)) {
+            // split complex64: hi plane at C, lo plane M ldc words further; word = bf16 re | bf16 im << 16
+            bf16raw rh, rl, ih, il;
+            split_bf16_raw(vr, rh, rl);
+            split_bf16_raw(vi, ih, il);
+            unsigned* c = reinterpret_cast<unsigned*>(p.C);
+            c[off] = (unsigned)rh | ((unsigned)ih << 16);
+            c[(long)p.M * p.ldc + off] = (unsigned)rl | ((unsigned)il << 16);
+          } else if constexpr (sizeof(TC) == 8) {
             double2* c = reinterpret_cast<double2*>(p.C) + off;
             double2 o = p.beta != 0.f ? *c : make_double2(0.0, 0.0);
             *c = make_double2((double)vr + (double)p.beta * o.x, (double)vi + (double)p.beta * o.y);

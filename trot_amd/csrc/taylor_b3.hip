@@ -34,6 +34,8 @@ constexpr int PB_BM = 128, PB_BN = 64, PB_BK = 32;
 
 struct PropB3P {
   const float* vsym;    // Taylor: V, sym-block complex64 (W, vcols)
+  const unsigned* vh;   // Taylor, split V: hi plane (W, vcols) of words bf16 re | bf16 im << 16
+  const unsigned* vl;   //   lo plane, same layout
   long vcols;           // complex numbers per walker in vsym
   const bf16raw* ep;    // one body: (E - I) planes [hi, lo], each (Np, Np)
   const bf16raw* tp;    // input planes (W, 4, np, Np)
@@ -55,10 +57,14 @@ __device__ __forceinline__ bf16x8_t neg_bf16x8(bf16x8_t x) {
 __device__ __forceinline__ unsigned pack2(bf16raw a, bf16raw b) { return (unsigned)a | ((unsigned)b << 16); }
 
 // REAL_A: product (E - I) . planes (one body); else V . planes (one Taylor term).
-template <bool REAL_A>
+// VM (Taylor only): 0 V as complex64, split here; 1 V as split planes (hi, lo), three passes; 2 the hi
+// planes of V and of the term only, one pass (bf16 products for the late, small terms).
+template <bool REAL_A, int VM = 0>
 __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
   constexpr int BM = PB_BM, BN = PB_BN, BK = PB_BK;
   constexpr int PA = REAL_A ? 2 : 4;
+  constexpr bool HIONLY = !REAL_A && VM == 2;  // lo planes neither loaded nor multiplied
+  constexpr bool BHI = HIONLY;
   // 64-byte LDS rows (32 bf16 of K), 16-byte segments XOR-swizzled by b3_lds: 48 KB.  Two workgroups
   // per CU (a third, possible for the one-body form, measured no faster)
   __shared__ __attribute__((aligned(16))) bf16raw As[PA][BM * BK];
@@ -88,7 +94,8 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
   const bf16raw* Eb = REAL_A ? p.ep + (long)m0 * p.Np : nullptr;
   bool up = true;
 
-  f4_t ast[8];
+  f4_t ast[VM == 0 ? 8 : 1];
+  u32x4_t vhw[VM != 0 ? 4 : 1], vlw[VM == 1 ? 4 : 1];
   u32x4_t bst[4];
   auto load = [&](int k0) {
     if constexpr (REAL_A) {
@@ -103,19 +110,31 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
     } else {
       const int kb = (k0 >> 3) + kbl;
       up = rb <= kb;
-      if (rb < p.nb && kb < p.nb) {
-        const long s = up ? symblk_index(rb, kb, p.nb) : symblk_index(kb, rb, p.nb);
+      const bool inb = rb < p.nb && kb < p.nb;
+      const long s = inb ? (up ? symblk_index(rb, kb, p.nb) : symblk_index(kb, rb, p.nb)) : 0;
+      if constexpr (VM == 0) {
         const f4_t* src = reinterpret_cast<const f4_t*>(Vb + s * 128 + 32 * aq);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) ast[j] = src[j];
+        for (int j = 0; j < 8; ++j) ast[j] = inb ? src[j] : f4_t{0.f, 0.f, 0.f, 0.f};
       } else {
+        const long o = walker * p.vcols + s * 64 + 16 * aq;
+        const u32x4_t* hsrc = reinterpret_cast<const u32x4_t*>(p.vh + o);
+        const u32x4_t z4 = {0u, 0u, 0u, 0u};
 #pragma unroll
-        for (int j = 0; j < 8; ++j) ast[j] = f4_t{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < 4; ++j) vhw[j] = inb ? hsrc[j] : z4;
+        if constexpr (VM == 1) {
+          const u32x4_t* lsrc = reinterpret_cast<const u32x4_t*>(p.vl + o);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) vlw[j] = inb ? lsrc[j] : z4;
+        }
       }
     }
     const long offb = (long)row0 * p.Np + k0 + kq;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bst[q] = *reinterpret_cast<const u32x4_t*>(Bb + q * tplane + offb);
+    for (int q = 0; q < 4; ++q) {
+      if (BHI && (q & 1)) continue;
+      bst[q] = *reinterpret_cast<const u32x4_t*>(Bb + q * tplane + offb);
+    }
   };
   auto store = [&]() {
     if constexpr (REAL_A) {
@@ -129,14 +148,28 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
       bf16raw h[4][16];
 #pragma unroll
       for (int c = 0; c < 16; ++c) {
-        split_bf16_raw(ast[c >> 1][2 * (c & 1)], h[0][c], h[1][c]);
-        split_bf16_raw(ast[c >> 1][2 * (c & 1) + 1], h[2][c], h[3][c]);
+        if constexpr (VM == 0) {
+          split_bf16_raw(ast[c >> 1][2 * (c & 1)], h[0][c], h[1][c]);
+          split_bf16_raw(ast[c >> 1][2 * (c & 1) + 1], h[2][c], h[3][c]);
+        } else {
+          const unsigned wh = vhw[c >> 2][c & 3];
+          h[0][c] = (bf16raw)(wh & 0xffffu);
+          h[2][c] = (bf16raw)(wh >> 16);
+          if constexpr (VM == 1) {
+            const unsigned wl = vlw[c >> 2][c & 3];
+            h[1][c] = (bf16raw)(wl & 0xffffu);
+            h[3][c] = (bf16raw)(wl >> 16);
+          } else {
+            h[1][c] = h[3][c] = 0;
+          }
+        }
       }
       if (up) {  // stored row 2 aq + srl is tile row 8 rbl + 2 aq + srl, k = 8 kbl + sc
 #pragma unroll
         for (int srl = 0; srl < 2; ++srl)
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
+            if (HIONLY && (q & 1)) continue;
             const bf16raw* v = &h[q][8 * srl];
             *reinterpret_cast<u32x4_t*>(&As[q][b3_lds(8 * rbl + 2 * aq + srl, kbl)]) =
                 (u32x4_t){pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7])};
@@ -145,12 +178,17 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
 #pragma unroll
         for (int sc = 0; sc < 8; ++sc)
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
+          for (int q = 0; q < 4; ++q) {
+            if (HIONLY && (q & 1)) continue;
             *reinterpret_cast<unsigned*>(&As[q][b3_lds(8 * rbl + sc, kbl) + 2 * aq]) = pack2(h[q][sc], h[q][8 + sc]);
+          }
       }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) *reinterpret_cast<u32x4_t*>(&Bs[q][b3_lds(row0, t & 3)]) = bst[q];
+    for (int q = 0; q < 4; ++q) {
+      if (BHI && (q & 1)) continue;
+      *reinterpret_cast<u32x4_t*>(&Bs[q][b3_lds(row0, t & 3)]) = bst[q];
+    }
   };
 
   f4_t acc[2][4][2];
@@ -175,14 +213,20 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
     for (int j = 0; j < 2; ++j) {
       const int col = wn * 32 + j * 16 + (lane & 15);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) bf[j][q] = *reinterpret_cast<const bf16x8_t*>(&Bs[q][b3_lds(col, seg)]);
+      for (int q = 0; q < 4; ++q) {
+        if (BHI && (q & 1)) continue;
+        bf[j][q] = *reinterpret_cast<const bf16x8_t*>(&Bs[q][b3_lds(col, seg)]);
+      }
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int row = wm * 64 + i * 16 + (lane & 15);
       bf16x8_t a[PA];
 #pragma unroll
-      for (int q = 0; q < PA; ++q) a[q] = *reinterpret_cast<const bf16x8_t*>(&As[q][b3_lds(row, seg)]);
+      for (int q = 0; q < PA; ++q) {
+        if (HIONLY && (q & 1)) continue;
+        a[q] = *reinterpret_cast<const bf16x8_t*>(&As[q][b3_lds(row, seg)]);
+      }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         f4_t cr = acc[0][i][j], ci = acc[1][i][j];
@@ -194,20 +238,20 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
           ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][3], ci, 0, 0, 0);
           ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][2], ci, 0, 0, 0);
         } else {
-          const bf16x8_t nh = neg_bf16x8(bf[j][2]), nl = neg_bf16x8(bf[j][3]);
+          const bf16x8_t nh = neg_bf16x8(bf[j][2]);
           // re += Ar.Br - Ai.Bi
-          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][0], cr, 0, 0, 0);
-          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][1], cr, 0, 0, 0);
+          if constexpr (!HIONLY) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][0], cr, 0, 0, 0);
+          if constexpr (!BHI) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][1], cr, 0, 0, 0);
           cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][0], cr, 0, 0, 0);
-          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], nh, cr, 0, 0, 0);
-          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], nl, cr, 0, 0, 0);
+          if constexpr (!HIONLY) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], nh, cr, 0, 0, 0);
+          if constexpr (!BHI) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], neg_bf16x8(bf[j][3]), cr, 0, 0, 0);
           cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], nh, cr, 0, 0, 0);
           // im += Ar.Bi + Ai.Br
-          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][2], ci, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][3], ci, 0, 0, 0);
+          if constexpr (!HIONLY) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][2], ci, 0, 0, 0);
+          if constexpr (!BHI) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][3], ci, 0, 0, 0);
           ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][2], ci, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], bf[j][0], ci, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][1], ci, 0, 0, 0);
+          if constexpr (!HIONLY) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], bf[j][0], ci, 0, 0, 0);
+          if constexpr (!BHI) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][1], ci, 0, 0, 0);
           ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][0], ci, 0, 0, 0);
         }
         acc[0][i][j] = cr;
@@ -330,18 +374,26 @@ size_t b3_workspace(int W, int N, int n) {
   return 2 * align256((size_t)W * 4 * np_ * Np * 2) + align256((size_t)W * Np * np_ * 8);
 }
 
-template <bool REAL_A>
+template <bool REAL_A, int VM = 0>
 int launch_prop(const PropB3P& P, hipStream_t s) {
   const int tpw = (P.Np / PB_BM) * (P.np_ / PB_BN);
   const long nwg = (long)((P.W + 7) / 8) * 8 * tpw;
   if (nwg > 0x7fffffffL) return fail(TROT_ERR_INVALID, "propagation b3: too many walkers");
-  hipLaunchKernelGGL(prop_b3_kernel<REAL_A>, dim3((unsigned)nwg), dim3(256), 0, s, P);
+  hipLaunchKernelGGL((prop_b3_kernel<REAL_A, VM>), dim3((unsigned)nwg), dim3(256), 0, s, P);
   HIPCHK(hipGetLastError());
   return TROT_OK;
 }
 
+// V of one call: complex64 (c64), or split planes (vh, vl) with terms k >= hi_from on the hi plane only.
+struct VSrc {
+  const float* c64;
+  const unsigned* vh;
+  const unsigned* vl;
+  int hi_from;  // terms k >= hi_from: one bf16 pass on the hi planes of V and of the term
+};
+
 // w_out <- [E .] T_K(i sqrt(dt) V) [. E] w_in for W walkers (one chunk); buffers sized for them.
-int run_b3_chunk(const float* vsym, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
+int run_b3_chunk(const VSrc& v, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
                  double sqrt_dt, int n_terms, bf16raw* const bufs[2], float* S, hipStream_t s) {
   const int Np = pad_up(N, PB_BM), np_ = pad_up(n, PB_BN), nb = (N + 7) / 8;
   if (!ep && n_terms == 0) {
@@ -353,7 +405,9 @@ int run_b3_chunk(const float* vsym, const bf16raw* ep, const double* w_in, doubl
   HIPCHK(hipGetLastError());
   int cur = 0;
   PropB3P P{};
-  P.vsym = vsym;
+  P.vsym = v.c64;
+  P.vh = v.vh;
+  P.vl = v.vl;
   P.vcols = symblk_count(nb) * 64;
   P.ep = ep;
   P.S = S;
@@ -381,7 +435,9 @@ int run_b3_chunk(const float* vsym, const bf16raw* ep, const double* w_in, doubl
     P.win = w_out;
     P.w = w_out;
     P.c = (float)(sqrt_dt / k);
-    if (int rc = launch_prop<false>(P, s)) return rc;
+    const int rc = v.c64 ? launch_prop<false, 0>(P, s)
+                         : (k >= v.hi_from ? launch_prop<false, 2>(P, s) : launch_prop<false, 1>(P, s));
+    if (rc) return rc;
     cur ^= 1;
   }
   if (ep) {  // second one-body factor on the planes of the propagated walker
@@ -395,7 +451,7 @@ int run_b3_chunk(const float* vsym, const bf16raw* ep, const double* w_in, doubl
 }
 
 // w_out <- [E .] T_K(i sqrt(dt) V) [. E] w_in; one-body factors when ep is non-null.
-int run_b3(const float* vsym, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
+int run_b3(const VSrc& v, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
            double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, hipStream_t s) {
   const int Np = pad_up(N, PB_BM), np_ = pad_up(n, PB_BN), nb = (N + 7) / 8;
   Carve cv{(char*)ws, ws_bytes, 0};
@@ -417,8 +473,12 @@ int run_b3(const float* vsym, const bf16raw* ep, const double* w_in, double* w_o
     const int wc = W - c0 < chunk ? W - c0 : chunk;
     bf16raw* cb[2] = {bufs[0] + (size_t)c0 * 4 * np_ * Np, bufs[1] + (size_t)c0 * 4 * np_ * Np};
     const size_t wo = (size_t)c0 * N * n * 2;
-    if (int rc = run_b3_chunk(vsym + (size_t)c0 * vcols * 2, ep, w_in + wo, w_out + wo, wc, N, n, sqrt_dt, n_terms,
-                              cb, S + (size_t)c0 * Np * np_ * 2, s))
+    VSrc vc = v;
+    if (v.c64) vc.c64 = v.c64 + (size_t)c0 * vcols * 2;
+    if (v.vh) vc.vh = v.vh + (size_t)c0 * vcols;
+    if (v.vl) vc.vl = v.vl + (size_t)c0 * vcols;
+    if (int rc = run_b3_chunk(vc, ep, w_in + wo, w_out + wo, wc, N, n, sqrt_dt, n_terms, cb,
+                              S + (size_t)c0 * Np * np_ * 2, s))
       return rc;
   }
   return TROT_OK;
@@ -446,7 +506,8 @@ int trot_taylor_apply_b3_symblk(const float* vhs_symblk, double* w, int W, int N
   if (!vhs_symblk || !w) return fail(TROT_ERR_INVALID, "taylor_apply_b3: null pointer");
   if (int rc = check_sizes("taylor_apply_b3", W, N, n, n_terms)) return rc;
   if (ws_bytes < b3_workspace(W, N, n)) return fail(TROT_ERR_WORKSPACE, "taylor_apply_b3: workspace");
-  return run_b3(vhs_symblk, nullptr, w, w, W, N, n, sqrt_dt, n_terms, ws, ws_bytes, (hipStream_t)stream);
+  return run_b3(VSrc{vhs_symblk, nullptr, nullptr, 0}, nullptr, w, w, W, N, n, sqrt_dt, n_terms, ws, ws_bytes,
+                (hipStream_t)stream);
 }
 
 size_t trot_onebody_b3_planes_bytes(int N) {
@@ -469,8 +530,24 @@ int trot_trotter_b3_symblk(const float* vhs_symblk, const void* onebody_planes, 
   if (!vhs_symblk || !onebody_planes || !w_in || !w_out) return fail(TROT_ERR_INVALID, "trotter_b3: null pointer");
   if (int rc = check_sizes("trotter_b3", W, N, n, n_terms)) return rc;
   if (ws_bytes < b3_workspace(W, N, n)) return fail(TROT_ERR_WORKSPACE, "trotter_b3: workspace");
-  return run_b3(vhs_symblk, (const bf16raw*)onebody_planes, w_in, w_out, W, N, n, sqrt_dt, n_terms, ws, ws_bytes,
-                (hipStream_t)stream);
+  return run_b3(VSrc{vhs_symblk, nullptr, nullptr, 0}, (const bf16raw*)onebody_planes, w_in, w_out, W, N, n,
+                sqrt_dt, n_terms, ws, ws_bytes, (hipStream_t)stream);
+}
+
+// As trot_trotter_b3_symblk with V as split planes [hi | lo], each (W, trot_symblk_cols(N)) words of
+// bf16 re | bf16 im << 16 (trot_gemm_b3 output kind 2); terms k >= hi_from are one bf16 pass on the hi
+// planes of V and of the term.
+int trot_trotter_b3v(const void* v_planes, int hi_from, const void* onebody_planes, const double* w_in,
+                     double* w_out, int W, int N, int n, double sqrt_dt, int n_terms, void* ws, size_t ws_bytes,
+                     void* stream) {
+  if (!v_planes || !onebody_planes || !w_in || !w_out) return fail(TROT_ERR_INVALID, "trotter_b3v: null pointer");
+  if (int rc = check_sizes("trotter_b3v", W, N, n, n_terms)) return rc;
+  if (hi_from < 1) return fail(TROT_ERR_INVALID, "trotter_b3v: hi_from must be >= 1");
+  if (ws_bytes < b3_workspace(W, N, n)) return fail(TROT_ERR_WORKSPACE, "trotter_b3v: workspace");
+  const long vcols = symblk_count((N + 7) / 8) * 64;
+  const unsigned* vh = (const unsigned*)v_planes;
+  return run_b3(VSrc{nullptr, vh, vh + (size_t)W * vcols, hi_from}, (const bf16raw*)onebody_planes, w_in, w_out, W,
+                N, n, sqrt_dt, n_terms, ws, ws_bytes, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -147,6 +147,7 @@ class CholAfqmcCtx:
     precision: str = "f64"  # VHS GEMM precision
     taylor_precision: str = "f64"
     walker_chunk: int = 0
+    taylor_hi_from: int = 0  # split-BF16 propagation: first Taylor term as one bf16 pass (0: none)
     low: dict = field(default_factory=dict)  # FP32 operand copies, kernel timer
 
 
@@ -162,7 +163,8 @@ def _total_dm(rdm1) -> torch.Tensor:
 
 
 def build_prop_ctx(ham_data, rdm1, dt: float, precision: str = "f64", walker_chunk: int = 0,
-                   taylor_precision: str | None = None, vhs_layout: str = "auto") -> CholAfqmcCtx:
+                   taylor_precision: str | None = None, vhs_layout: str = "auto",
+                   taylor_hi_from: int = 0) -> CholAfqmcCtx:
     """trot/prop/chol_afqmc_ops.py:103-127 (and the host twin trot/runtime_layout.py:125-197).
 
     mf, sum_g mf_g L_g and 1/2 sum_g L_g L_g run on the GPU; the N x N
@@ -193,7 +195,10 @@ def build_prop_ctx(ham_data, rdm1, dt: float, precision: str = "f64", walker_chu
         precision=precision,
         taylor_precision=taylor_precision or ("f64" if precision == "f64" else "f32"),
         walker_chunk=walker_chunk,
+        taylor_hi_from=int(taylor_hi_from),
     )
+    if taylor_hi_from < 0:
+        raise ValueError("taylor_hi_from must be >= 0")
     if vhs_layout not in ("auto", "dense", "symblk"):
         raise ValueError(f"unknown vhs_layout: {vhs_layout}")
     # Cholesky vectors of real-orbital ERIs are symmetric; after a rotation to the MO basis only to
@@ -221,6 +226,8 @@ def build_prop_ctx(ham_data, rdm1, dt: float, precision: str = "f64", walker_chu
     if ctx.low["symblk"] and ctx.taylor_precision == "bf16x3":
         # whole propagation on split-BF16 MFMA: (E - I) planes for the one-body factors
         ctx.low["onebody_b3"] = _lib.onebody_b3_planes(ctx.exp_h1_half)
+        # the VHS GEMM then writes V already split ([hi | lo] planes) for the propagation kernels
+        ctx.low["split_v"] = precision == "bf16x3"
     return ctx
 
 
@@ -305,6 +312,8 @@ def _vhs(ctx: CholAfqmcCtx, xbar: torch.Tensor) -> torch.Tensor:
         dtype = torch.complex128 if ctx.taylor_precision == "f64" else torch.complex64
 
         def fn(x):
+            if ctx.low.get("split_v"):
+                return _b3.gemm_nt_split(_b3.split(x), planes)
             out = torch.empty((x.shape[0], planes.rows), dtype=dtype, device=x.device)
             return _b3.gemm_nt(_b3.split(x), planes, out)
 
@@ -349,8 +358,13 @@ def _taylor(ctx: CholAfqmcCtx, v: torch.Tensor, t: torch.Tensor, n_terms: int) -
 
 def _trotter_b3(ctx: CholAfqmcCtx, v: torch.Tensor, w: torch.Tensor, n_terms: int) -> torch.Tensor:
     """One spin set through E T_K E on split-BF16 MFMA (one-body factors as w + (E - I) w)."""
-    fn = _lib.trotter_b3_symblk
-    args = (v, w.contiguous(), ctx.low["onebody_b3"], ctx.sqrt_dt, n_terms)
+    if v.dtype == torch.int32:  # split V planes
+        hi_from = ctx.taylor_hi_from if ctx.taylor_hi_from > 0 else n_terms + 1
+        fn = _lib.trotter_b3v
+        args = (v, w.contiguous(), ctx.low["onebody_b3"], ctx.sqrt_dt, n_terms, hi_from)
+    else:
+        fn = _lib.trotter_b3_symblk
+        args = (v, w.contiguous(), ctx.low["onebody_b3"], ctx.sqrt_dt, n_terms)
     timer = ctx.low.get("timer")
     if timer is not None:
         return timer.wrap("trotter", fn, *args)
@@ -438,7 +452,7 @@ class PropOps:
 
 def make_prop_ops(ham_basis: str, walker_kind: str, mixed_precision: bool = False, *,
                   precision: str | None = None, taylor_precision: str | None = None,
-                  walker_chunk: int = 0, vhs_layout: str = "auto") -> PropOps:
+                  walker_chunk: int = 0, vhs_layout: str = "auto", taylor_hi_from: int = 0) -> PropOps:
     """trot/prop/afqmc.py:147-180.
 
     ``mixed_precision=True`` selects the FP32 VHS GEMM (the reference's mixed
@@ -467,6 +481,7 @@ def make_prop_ops(ham_basis: str, walker_kind: str, mixed_precision: bool = Fals
 
     def _build(ham_data, rdm1, params):
         return build_prop_ctx(ham_data, rdm1, params.dt, precision=prec, walker_chunk=walker_chunk,
-                              taylor_precision=taylor_precision, vhs_layout=vhs_layout)
+                              taylor_precision=taylor_precision, vhs_layout=vhs_layout,
+                              taylor_hi_from=taylor_hi_from)
 
     return PropOps(init_prop_state=init_prop_state, build_prop_ctx=_build, step=step)

@@ -19,25 +19,19 @@
 // ties, as LAPACK's izamax), det = prod(pivots) * (-1)^swaps.  One workgroup of
 // four waves per walker, n <= 64; the matrix is padded to 64 x 64 with an
 // identity block and held in registers: element (i, j) in lane i of wave j / 16,
-// register j % 16.  Per elimination step the pivot search is a shuffle
-// reduction inside the wave that owns column k, the row swap is a lane shuffle
-// inside every wave, and row k reaches the other lanes of its wave by shuffle:
-// two barriers per step (pivot index; column k).
+// register j % 16.  Rows are never moved: each lane keeps the position its row
+// would have after LAPACK's swaps (lp), the pivot search runs over lp >= k with
+// ties to the smallest lp, and the pivot row and column reach every lane through
+// LDS (double-buffered by step parity).  Two barriers per step.  The arithmetic
+// per row is the row-swapping algorithm's; only the storage differs.
 // ---------------------------------------------------------------------------
-// Value of x in lane l (l wave-uniform), through the scalar unit.
-__device__ __forceinline__ double readlane_d(double x, int l) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
 __global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__ m, double* __restrict__ minv,
                                                       double* __restrict__ det, int n) {
-  __shared__ cplx colk[64];
-  __shared__ int perm[64];
+  __shared__ cplx colk[2][64];  // column k of every (physical) row, before the step
+  __shared__ cplx rowp[2][64];  // the pivot row, every column
+  __shared__ int s_piv[2];
+  __shared__ int perm[64];      // position swapped with k at step k (LAPACK ipiv)
   __shared__ int outcol[64];
-  __shared__ int s_piv;
   const int t = threadIdx.x, lane = t & 63, wq = t >> 6;
   const long w = blockIdx.x;
   const double* mw = m + 2 * w * n * n;
@@ -47,70 +41,65 @@ __global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__
     const int j = wq * 16 + r;
     a[r] = (lane < n && j < n) ? cload(mw, (long)lane * n + j) : cplx{lane == j ? 1.0 : 0.0, 0.0};
   }
+  int lp = lane;  // position of this row after the swaps so far (identical in all four waves)
   cplx dacc = {1.0, 0.0};
   for (int k = 0; k < n; ++k) {
-    const int kw = k >> 4, kr = k & 15;
+    const int kw = k >> 4, kr = k & 15, pb = k & 1;
     if (wq == kw) {
       cplx x = a[0];
 #pragma unroll
       for (int r = 1; r < 16; ++r)
         if (r == kr) x = a[r];
+      colk[pb][lane] = x;
       const double c = cabs1(x);
-      double v = (lane >= k && lane < n && c == c) ? c : -1.0;  // NaN never wins
-      int bi = lane;
+      double v = (lp >= k && lane < n && c == c) ? c : -1.0;  // NaN never wins
+      int key = lp, bi = lane;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
         const double v2 = __shfl_xor(v, off);
+        const int k2 = __shfl_xor(key, off);
         const int b2 = __shfl_xor(bi, off);
-        if (v2 > v || (v2 == v && b2 < bi)) {
+        if (v2 > v || (v2 == v && k2 < key)) {
           v = v2;
+          key = k2;
           bi = b2;
         }
       }
-      if (lane == 0) {
-        const int pr = v < 0.0 ? k : bi;  // column of NaNs: keep the diagonal
-        s_piv = pr;
-        perm[k] = pr;
+      if (v < 0.0) {  // column of NaNs: keep the row at position k
+        const unsigned long long at_k = __ballot(lp == k);
+        bi = __ffsll((long long)at_k) - 1;
       }
+      if (lane == 0) s_piv[pb] = bi;
     }
     __syncthreads();
-    const int pr = s_piv;
-    if (pr != k) {  // swap rows k and pr: lanes k and pr of every wave (selects, no branches)
-      const bool is_k = lane == k, is_p = lane == pr;
+    const int pr = s_piv[pb];
+    const int lpp = __shfl(lp, pr);  // position of the pivot row (>= k)
+    if (lane == pr) {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const double kre = readlane_d(a[r].re, k), kim = readlane_d(a[r].im, k);
-        const double pre = readlane_d(a[r].re, pr), pim = readlane_d(a[r].im, pr);
-        a[r].re = is_k ? pre : (is_p ? kre : a[r].re);
-        a[r].im = is_k ? pim : (is_p ? kim : a[r].im);
-      }
+      for (int r = 0; r < 16; ++r) rowp[pb][wq * 16 + r] = a[r];
     }
-    if (wq == kw) {
-      cplx x = a[0];
-#pragma unroll
-      for (int r = 1; r < 16; ++r)
-        if (r == kr) x = a[r];
-      colk[lane] = x;
-    }
-    __syncthreads();
-    const cplx piv = colk[k];
     if (t == 0) {
-      dacc = cmul(dacc, piv);
-      if (pr != k) dacc = {-dacc.re, -dacc.im};
+      perm[k] = lpp;
+      dacc = cmul(dacc, colk[pb][pr]);
+      if (lpp != k) dacc = {-dacc.re, -dacc.im};
     }
+    if (lp == k) lp = lpp;  // the swap of positions k and lpp
+    if (lane == pr) lp = k;
+    __syncthreads();
+    const cplx piv = colk[pb][pr];
     const cplx ipiv = cinv(piv);
-    // with u_j = a[k][j] (1 at j = k): row k becomes u ipiv, every other row i becomes
+    // with u_j = a[pr][j] (1 at j = k): the pivot row becomes u ipiv, every other row i becomes
     // a[i][j] - a[i][k] ipiv u_j with column k taken as 0.  One update for all lanes:
-    // a <- x - g u, (x, g) = (0, -ipiv) on lane k and (a with column k zeroed, a[i][k] ipiv) elsewhere.
-    const bool is_k = lane == k;
-    const cplx fi = cmul(colk[lane], ipiv);
-    const cplx g = {is_k ? -ipiv.re : fi.re, is_k ? -ipiv.im : fi.im};
+    // a <- x - g u, (x, g) = (0, -ipiv) on the pivot row and (a with column k zeroed, a[i][k] ipiv) elsewhere.
+    const bool is_p = lane == pr;
+    const cplx fi = cmul(colk[pb][lane], ipiv);
+    const cplx g = {is_p ? -ipiv.re : fi.re, is_p ? -ipiv.im : fi.im};
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int j = wq * 16 + r;
-      cplx u = {readlane_d(a[r].re, k), readlane_d(a[r].im, k)};  // row k, this column
+      cplx u = rowp[pb][j];
       if (j == k) u = {1.0, 0.0};
-      const bool zero = is_k || j == k;
+      const bool zero = is_p || j == k;
       const cplx x = {zero ? 0.0 : a[r].re, zero ? 0.0 : a[r].im};
       a[r] = csub(x, cmul(g, u));
     }
@@ -126,13 +115,13 @@ __global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__
     }
   }
   __syncthreads();
-  if (t < 64) colk[outcol[t]].re = (double)t;  // inverse map: here column j -> output column
+  if (t < 64) colk[0][outcol[t]].re = (double)t;  // inverse map: here column j -> output column
   __syncthreads();
   double* ow = minv + 2 * w * n * n;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int j = wq * 16 + r;
-    if (lane < n && j < n) cstore(ow, (long)lane * n + (int)colk[j].re, a[r]);
+    if (lp < n && j < n) cstore(ow, (long)lp * n + (int)colk[0][j].re, a[r]);
   }
   if (t == 0) cstore(det, w, dacc);
 }

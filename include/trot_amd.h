@@ -139,6 +139,36 @@ int trot_block_energy(const double* energy, double* weights, double* scalars, do
 size_t trot_orthonormalize_workspace(int W, int N, int n);
 int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t ws_bytes, void* stream);
 
+/* trot_orthonormalize that also keeps the norm: det_r[w] = prod diag(R) with diag(R) real positive, the
+ * product of the two CholQR passes' Cholesky diagonals (workspace trot_orthonormalize_workspace).
+ * Replaces: _qr / orthogonalize (trot/walkers.py:137-172), used by block_fp (trot/prop/blocks.py:451-458). */
+int trot_orthogonalize(double* walkers, int W, int N, int n, double* det_r, void* ws, size_t ws_bytes, void* stream);
+
+/* det_r[w] = prod_k L_kk for the Cholesky factor L of w^H w: |det R| of w = QR, which is the phase-fixed
+ * diag(R) product of the reference (LAPACK's geqrf returns a real diagonal, so d^2/|d| = |d|).
+ * Replaces: qr_norm (trot/walkers.py:182-205) per spin. */
+size_t trot_qr_norm_workspace(int W, int N, int n);
+int trot_qr_norm(const double* walkers, int W, int N, int n, double* det_r, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- free projection (trot/prop/afqmc_fp.py:18-72, trot/prop/blocks.py:415-490) ----
+ * trot_fp_prepare: per walker z = -sqrt(dt) sum_g x_g mf_g + dt h0p, cst = exp(z) (optional),
+ *   scale = cst^(1/n_total) (principal power), xbar = x + 0i (optional, (W, G) complex128).
+ *   Replaces: shift_term / constants (afqmc_fp.py:95-96) and multiply_constants (trot/walkers.py:208-226).
+ * trot_scale_rows: a[b, :] *= scale[b] for rows of row_complex complex numbers.
+ * trot_fp_weights: norm = det_a^2 (det_b NULL, restricted walkers) or det_a det_b; weights_new = weights norm
+ *   (afqmc_fp.py:107-108).
+ * trot_fp_unnorm: weights[i] /= norms[idx[i]] after the comb (afqmc_fp.py:113-119).
+ * trot_block_energy_fp: E_L -> ene0 where |E_L - ene0| > sqrt(2/dt); out[8] = complex pairs
+ *   [sum w ov E / sum w ov, sum w ov, sum ov, sum |ov|] (blocks.py:466-488). */
+int trot_fp_prepare(const double* fields, const double* mf, int W, int G, double sqrt_dt, double dt, double h0p_re,
+                    double h0p_im, int n_total, double* xbar, double* cst, double* scale, void* stream);
+int trot_scale_rows(double* a, const double* scale, int W, long row_complex, void* stream);
+int trot_fp_weights(const double* weights, const double* det_a, const double* det_b, int W, double* norms,
+                    double* weights_new, void* stream);
+int trot_fp_unnorm(double* weights, const double* norms, const int* idx, int W, void* stream);
+int trot_block_energy_fp(const double* energy, const double* weights, const double* overlaps, int W, double dt,
+                         double ene0, double* out, void* stream);
+
 /* Comb reconfiguration: idx[i] = searchsorted(cumsum|w|, tot (i + zeta)/W, 'left'), weights_new = tot/W.
  * Replaces: _sr_indices / stochastic_reconfiguration (trot/walkers.py:249-293). */
 size_t trot_sr_workspace(int W);

@@ -553,6 +553,115 @@ def block(state: PropState, fields_seq, zeta: float, ctx: PropCtx, ops: RhfOps, 
     return state, {"energy": float(e_block), "weight": float(w_sum), "e_samples": e_samples, "sr_idx": idx}
 
 
+# ---------------------------------------------------------------------------
+# free projection (trot/prop/afqmc_fp.py:18-72, chol_afqmc_ops_fp.py:49-74, walkers.py:182-226,
+# blocks.py:415-490)
+# ---------------------------------------------------------------------------
+
+
+@dataclass(frozen=True)
+class QmcParamsFp:
+    """trot/prop/types.py:55-62 (with the base fields this restatement uses)."""
+
+    dt: float = 0.05
+    n_prop_steps: int = 20
+    n_qr_blocks: int = 10
+    n_blocks: int = 5
+    ene0: float | None = None
+    n_traj: int = 10
+    n_walkers: int = 200
+
+
+def build_prop_ctx_fp(h0: float, h1: np.ndarray, chol: np.ndarray, rdm1: np.ndarray, dt: float,
+                      ene0: float) -> PropCtx:
+    """trot/prop/chol_afqmc_ops_fp.py:49-74: h0_prop = -h0 - 1/2 sum mf^2 + ene0, the rest as the phaseless ctx."""
+    ctx = build_prop_ctx(h0, h1, chol, rdm1, dt)
+    ctx.h0_prop = complex(ctx.h0_prop + ene0)
+    return ctx
+
+
+def qr_norm(w, walker_kind: str) -> np.ndarray:
+    """trot/walkers.py:182-205: R of the QR only; diag(R) multiplied by its phase; det_r = prod diag;
+    restricted det_r^2, unrestricted det_u det_d."""
+    def det_r(mat):
+        r = np.linalg.qr(mat, mode="r")
+        d = np.diagonal(r, axis1=-2, axis2=-1)
+        abs_d = np.abs(d)
+        phase = d / np.where(abs_d == 0, 1.0, abs_d)
+        r = phase[..., :, None] * r
+        return np.prod(np.diagonal(r, axis1=-2, axis2=-1), axis=-1)
+
+    if walker_kind == "unrestricted":
+        return det_r(w[0]) * det_r(w[1])
+    d = det_r(w)
+    return d * d
+
+
+def orthogonalize(w, walker_kind: str):
+    """trot/walkers.py:151-172: (Q with real positive diag R, norm = det_r^2 or det_u det_d)."""
+    if walker_kind == "unrestricted":
+        qu, du = qr_phase(w[0])
+        qd, dd = qr_phase(w[1])
+        return (qu, qd), du * dd
+    q, d = qr_phase(w)
+    return q, d * d
+
+
+def multiply_constants(w, constants: np.ndarray, walker_kind: str):
+    """trot/walkers.py:208-226: each walker's constant spread over its columns as c^(1/n_total)."""
+    if walker_kind == "unrestricted":
+        n_total = w[0].shape[-1] + w[1].shape[-1]
+        c = (constants ** (1.0 / n_total)).reshape(-1, 1, 1)
+        return (w[0] * c, w[1] * c)
+    n_total = 2 * w.shape[-1]
+    c = (constants ** (1.0 / n_total)).reshape(-1, 1, 1)
+    return w * c
+
+
+def afqmc_step_fp(state: PropState, fields: np.ndarray, zeta: float, ctx: PropCtx, walker_kind: str) -> PropState:
+    """trot/prop/afqmc_fp.py:18-72 with the fields (W, G) and the comb offset given instead of drawn."""
+    shift_term = fields @ ctx.mf_shifts  # afqmc_fp.py:35
+    constants = np.exp(-ctx.sqrt_dt * shift_term + ctx.dt * ctx.h0_prop)  # :36
+    if walker_kind == "unrestricted":
+        w_new = apply_trotter_u(state.walkers, fields, ctx, 10)  # :38-40
+    else:
+        w_new = apply_trotter_r(state.walkers, fields, ctx, 10)
+    w_new = multiply_constants(w_new, constants, walker_kind)  # :42
+    norms = qr_norm(w_new, walker_kind)  # :51
+    weights_new = state.weights * norms.real  # :52
+    n = weights_new.shape[0]
+    idx = np.minimum(sr_indices(weights_new, zeta, n), n - 1)  # :59 (a gather clamps)
+    w_sr, weight_sr, _ = stochastic_reconfiguration(w_new, weights_new, zeta, walker_kind)  # :60
+    weight_sr = weight_sr / norms[idx].real  # :61-62
+    return PropState(walkers=w_sr, weights=weight_sr, overlaps=state.overlaps,
+                     pop_control_ene_shift=state.pop_control_ene_shift, e_estimate=state.e_estimate,
+                     node_encounters=state.node_encounters)
+
+
+def block_fp(state: PropState, fields_seq, zetas, ctx: PropCtx, ops: RhfOps, params: QmcParamsFp):
+    """trot/prop/blocks.py:415-490 with pre-drawn fields and comb offsets (n_prop_steps each)."""
+    assert params.n_prop_steps % params.n_qr_blocks == 0
+    per = params.n_prop_steps // params.n_qr_blocks
+    s = 0
+    for _ in range(params.n_qr_blocks):
+        for _ in range(per):
+            state = afqmc_step_fp(state, fields_seq[s], zetas[s], ctx, ops.walker_kind)
+            s += 1
+        q, norms = orthogonalize(state.walkers, ops.walker_kind)  # blocks.py:451-458
+        state.weights = state.weights * norms.real
+        state.walkers = q
+    state.overlaps = ops.overlap(state.walkers)  # blocks.py:463-466
+    e_samples = ops.energy(state.walkers)  # :467-470 (complex)
+    thresh = np.sqrt(2.0 / params.dt)
+    ene0 = params.ene0
+    e_samples = np.where(np.abs(e_samples - ene0) > thresh, ene0, e_samples)  # :474-476
+    wov = state.weights * state.overlaps
+    w_sum = np.sum(wov)
+    e_block = np.sum(wov * e_samples) / w_sum
+    return state, {"energy": e_block, "weight": w_sum, "overlap": np.sum(state.overlaps),
+                   "abs_overlap": np.sum(np.abs(state.overlaps))}
+
+
 def init_prop_state(walkers, ops: RhfOps, n_walkers: int, initial_e_estimate: float | None = None) -> PropState:
     """trot/prop/afqmc.py:19-76: weights 1, overlaps, E_est = Re E_L(walker 0), shift = E_est."""
     overlaps = ops.overlap(walkers)

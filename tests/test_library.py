@@ -106,6 +106,17 @@ def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
     assert h.trot_rng_advance(None, 1, None) == -1
     assert h.trot_sr_comb_dev(d, None, 4, d, d, d, big, None) == -1
     assert h.trot_sr_comb_dev(d, d, 4, d, d, d, 0, None) == -4
+    # free projection
+    assert h.trot_fp_prepare(None, d, 4, 8, 0.1, 0.01, 0.0, 0.0, 4, None, None, d, None) == -1
+    assert h.trot_fp_prepare(d, d, 4, 8, 0.1, 0.01, 0.0, 0.0, 0, None, None, d, None) == -1
+    assert h.trot_scale_rows(None, d, 4, 8, None) == -1
+    assert h.trot_scale_rows(d, d, 0, 8, None) == 0  # nothing to scale
+    assert h.trot_fp_weights(d, None, None, 4, d, d, None) == -1
+    assert h.trot_fp_unnorm(d, d, None, 4, None) == -1
+    assert h.trot_block_energy_fp(d, d, d, 4, 0.0, 0.0, d, None) == -1  # dt must be positive
+    assert h.trot_qr_norm(d, 4, 8, 65, d, d, big, None) == -2
+    assert h.trot_qr_norm(d, 4, 8, 2, d, d, 0, None) == -4
+    assert h.trot_orthogonalize(None, 4, 8, 2, None, d, big, None) == -1
 
 
 def test_workspace_queries_are_sizes():

@@ -58,6 +58,14 @@ _SIGS = {
     "trot_block_energy": (_i, [_vp, _vp, _vp, _vp, _i, _d, _d, _vp]),
     "trot_orthonormalize_workspace": (_sz, [_i, _i, _i]),
     "trot_orthonormalize": (_i, [_vp, _i, _i, _i, _vp, _sz, _vp]),
+    "trot_orthogonalize": (_i, [_vp, _i, _i, _i, _vp, _vp, _sz, _vp]),
+    "trot_qr_norm_workspace": (_sz, [_i, _i, _i]),
+    "trot_qr_norm": (_i, [_vp, _i, _i, _i, _vp, _vp, _sz, _vp]),
+    "trot_fp_prepare": (_i, [_vp, _vp, _i, _i, _d, _d, _d, _d, _i, _vp, _vp, _vp, _vp]),
+    "trot_scale_rows": (_i, [_vp, _vp, _i, _l, _vp]),
+    "trot_fp_weights": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "trot_fp_unnorm": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "trot_block_energy_fp": (_i, [_vp, _vp, _vp, _i, _d, _d, _vp, _vp]),
     "trot_sr_workspace": (_sz, [_i]),
     "trot_sr_comb": (_i, [_vp, _d, _i, _vp, _vp, _vp, _sz, _vp]),
     "trot_gather_rows": (_i, [_vp, _vp, _vp, _i, _l, _vp]),
@@ -337,6 +345,94 @@ def orthonormalize(w: torch.Tensor) -> torch.Tensor:
     L = lib()
     ws = _ws(L.trot_orthonormalize_workspace(W, N, n), w.device)
     _check(L.trot_orthonormalize(_p(out), W, N, n, _p(ws), ws.numel(), _stream()), "trot_orthonormalize")
+    return out
+
+
+def orthogonalize(w: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """(Q, det_r): Q with real positive diag(R) and det_r = prod diag(R) (real, (W,))."""
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    if n > MAX_NOCC:
+        raise ValueError(f"nocc={n} > {MAX_NOCC} is not supported")
+    out = w.clone()
+    det_r = torch.empty((W,), dtype=F64, device=w.device)
+    L = lib()
+    ws = _ws(L.trot_orthonormalize_workspace(W, N, n), w.device)
+    _check(L.trot_orthogonalize(_p(out), W, N, n, _p(det_r), _p(ws), ws.numel(), _stream()), "trot_orthogonalize")
+    return out, det_r
+
+
+def qr_norm(w: torch.Tensor) -> torch.Tensor:
+    """|det R| of w = QR per walker (W,) FP64, from the Cholesky factor of w^H w."""
+    W, N, n = w.shape
+    _req(w, "walkers", C128, (W, N, n))
+    if n > MAX_NOCC:
+        raise ValueError(f"nocc={n} > {MAX_NOCC} is not supported")
+    det_r = torch.empty((W,), dtype=F64, device=w.device)
+    L = lib()
+    ws = _ws(L.trot_qr_norm_workspace(W, N, n), w.device)
+    _check(L.trot_qr_norm(_p(w), W, N, n, _p(det_r), _p(ws), ws.numel(), _stream()), "trot_qr_norm")
+    return det_r
+
+
+def fp_prepare(fields: torch.Tensor, mf: torch.Tensor, sqrt_dt: float, dt: float, h0p: complex, n_total: int):
+    """(xbar (W, G) complex128 = fields + 0i, constants (W,), per-orbital scale constants^(1/n_total) (W,))."""
+    W, G = fields.shape
+    _req(fields, "fields", F64, (W, G))
+    _req(mf, "mf_shifts", C128, (G,))
+    xbar = torch.empty((W, G), dtype=C128, device=fields.device)
+    cst = torch.empty((W,), dtype=C128, device=fields.device)
+    scale = torch.empty((W,), dtype=C128, device=fields.device)
+    h0p = complex(h0p)
+    _check(lib().trot_fp_prepare(_p(fields), _p(mf), W, G, float(sqrt_dt), float(dt), h0p.real, h0p.imag,
+                                 int(n_total), _p(xbar), _p(cst), _p(scale), _stream()), "trot_fp_prepare")
+    return xbar, cst, scale
+
+
+def scale_rows(a: torch.Tensor, scale: torch.Tensor) -> torch.Tensor:
+    """In place a[b] *= scale[b] for a complex128 (W, ...) tensor."""
+    W = a.shape[0]
+    if a.dtype != C128 or not a.is_contiguous() or a.device.type != "cuda":
+        raise ValueError("scale_rows: a must be a contiguous complex128 GPU tensor")
+    _req(scale, "scale", C128, (W,))
+    _check(lib().trot_scale_rows(_p(a), _p(scale), W, a[0].numel() if W else 0, _stream()), "trot_scale_rows")
+    return a
+
+
+def fp_weights(weights: torch.Tensor, det_a: torch.Tensor, det_b: torch.Tensor | None):
+    """(norms, weights * norms) with norms = det_a^2 or det_a det_b."""
+    W = weights.shape[0]
+    _req(weights, "weights", F64, (W,))
+    _req(det_a, "det_a", F64, (W,))
+    if det_b is not None:
+        _req(det_b, "det_b", F64, (W,))
+    norms = torch.empty_like(weights)
+    w_new = torch.empty_like(weights)
+    _check(lib().trot_fp_weights(_p(weights), _p(det_a), _p(det_b), W, _p(norms), _p(w_new), _stream()),
+           "trot_fp_weights")
+    return norms, w_new
+
+
+def fp_unnorm(weights: torch.Tensor, norms: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+    """In place weights[i] /= norms[idx[i]]."""
+    W = weights.shape[0]
+    _req(weights, "weights", F64, (W,))
+    _req(norms, "norms", F64, (W,))
+    _req(idx, "idx", torch.int32, (W,))
+    _check(lib().trot_fp_unnorm(_p(weights), _p(norms), _p(idx), W, _stream()), "trot_fp_unnorm")
+    return weights
+
+
+def block_energy_fp(energy: torch.Tensor, weights: torch.Tensor, overlaps: torch.Tensor, dt: float,
+                    ene0: float) -> torch.Tensor:
+    """(4,) complex128: [energy, weight, overlap, abs_overlap] of the free-projection block."""
+    W = energy.shape[0]
+    _req(energy, "energy", C128, (W,))
+    _req(weights, "weights", F64, (W,))
+    _req(overlaps, "overlaps", C128, (W,))
+    out = torch.empty((4,), dtype=C128, device=energy.device)
+    _check(lib().trot_block_energy_fp(_p(energy), _p(weights), _p(overlaps), W, float(dt), float(ene0), _p(out),
+                                      _stream()), "trot_block_energy_fp")
     return out
 
 

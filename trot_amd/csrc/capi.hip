@@ -461,6 +461,10 @@ size_t trot_orthonormalize_workspace(int W, int N, int n) {
 }
 
 int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t ws_bytes, void* stream) {
+  return trot_orthogonalize(walkers, W, N, n, nullptr, ws, ws_bytes, stream);
+}
+
+int trot_orthogonalize(double* walkers, int W, int N, int n, double* det_r, void* ws, size_t ws_bytes, void* stream) {
   if (!walkers) return fail(TROT_ERR_INVALID, "orthonormalize: null pointer");
   if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "orthonormalize: bad sizes");
   if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "orthonormalize: nocc > 64");
@@ -479,7 +483,7 @@ int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t w
     p.C = gram; p.ldc = n; p.sC = (long)n * n;
     p.M = n; p.N = n; p.K = N; p.ar = 1.0;
     HIPCHK((launch_gemm_f64<64, 64, true, true, false, true, true, EPI_STORE>(p, W, s)));
-    HIPCHK(launch_chol_rinv(gram, rinv, n, W, s));
+    HIPCHK(launch_chol_rinv(gram, rinv, n, W, s, det_r, pass));
     GemmP q{};
     q.A = src; q.lda = n; q.sA = (long)N * n;
     q.B = rinv; q.ldb = n; q.sB = (long)n * n;
@@ -489,6 +493,26 @@ int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t w
     src = dst;
     dst = walkers;
   }
+  return TROT_OK;
+}
+
+size_t trot_qr_norm_workspace(int W, int N, int n) { return align256((size_t)W * n * n * 16); }
+
+int trot_qr_norm(const double* walkers, int W, int N, int n, double* det_r, void* ws, size_t ws_bytes, void* stream) {
+  if (!walkers || !det_r) return fail(TROT_ERR_INVALID, "qr_norm: null pointer");
+  if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "qr_norm: bad sizes");
+  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "qr_norm: nocc > 64");
+  Carve c{(char*)ws, ws_bytes, 0};
+  double* gram = c.take<double>((size_t)W * n * n * 2);
+  if (!gram) return fail(TROT_ERR_WORKSPACE, "qr_norm: workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  GemmP p{};
+  p.A = walkers; p.lda = n; p.sA = (long)N * n;
+  p.B = walkers; p.ldb = n; p.sB = (long)N * n;
+  p.C = gram; p.ldc = n; p.sC = (long)n * n;
+  p.M = n; p.N = n; p.K = N; p.ar = 1.0;
+  HIPCHK((launch_gemm_f64<64, 64, true, true, false, true, true, EPI_STORE>(p, W, s)));
+  HIPCHK(launch_chol_rinv(gram, nullptr, n, W, s, det_r, 0));
   return TROT_OK;
 }
 

@@ -13,7 +13,8 @@ hipError_t launch_weight_update(double* weights, const double* ov_old, const dou
                                 const double* fbt, double* scal, long long* nodes, int W, double dt, double sqrt_dt,
                                 double h0p_re, double h0p_im, double w_floor, double w_cap, double damping,
                                 hipStream_t s);
-hipError_t launch_chol_rinv(const double* gram, double* rinv, int n, int W, hipStream_t s);
+hipError_t launch_chol_rinv(const double* gram, double* rinv, int n, int W, hipStream_t s, double* dprod = nullptr,
+                            int accumulate = 0);
 // zeta_dev: comb offset read on the device when non-null (else zeta)
 hipError_t launch_sr_index(const double* weights, double* cw, int* idx, double* w_new, int W, double zeta,
                            const double* zeta_dev, hipStream_t s);

@@ -233,8 +233,10 @@ __global__ __launch_bounds__(1024) void weight_update_kernel(
 // Cholesky of the Hermitian Gram matrix S = R^H R (R upper, real positive
 // diagonal) and R^-1.  One workgroup per walker.  rinv gets R^-1 (n x n).
 // ---------------------------------------------------------------------------
+// dprod (optional): product of the Cholesky diagonal, prod_k L_kk = |det R| of w = Q R, written
+// (accumulate = 0) or multiplied in (accumulate = 1, the second CholQR pass).  rinv null: diagonal only.
 __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict__ s, double* __restrict__ rinv,
-                                                        int n) {
+                                                        int n, double* __restrict__ dprod, int accumulate) {
   extern __shared__ double sh[];
   cplx* a = reinterpret_cast<cplx*>(sh);
   cplx* x = a + n * n;
@@ -243,8 +245,10 @@ __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict
   const double* sw = s + 2 * w * n * n;
   for (int e = t; e < n * n; e += 256) a[e] = cload(sw, e);
   __syncthreads();
+  double dp = 1.0;
   for (int k = 0; k < n; ++k) {
     const double d = sqrt(a[k * n + k].re);
+    dp *= d;
     __syncthreads();
     if (t == 0) a[k * n + k] = {d, 0.0};
     const double id = 1.0 / d;
@@ -258,6 +262,8 @@ __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict
     }
     __syncthreads();
   }
+  if (dprod && t == 0) dprod[w] = accumulate ? dprod[w] * dp : dp;
+  if (!rinv) return;
   // R X = I, X upper: column j by thread j (back substitution)
   for (int e = t; e < n * n; e += 256) x[e] = {0.0, 0.0};
   __syncthreads();
@@ -381,13 +387,14 @@ hipError_t launch_weight_update(double* weights, const double* ov_old, const dou
   return hipGetLastError();
 }
 
-hipError_t launch_chol_rinv(const double* gram, double* rinv, int n, int W, hipStream_t s) {
+hipError_t launch_chol_rinv(const double* gram, double* rinv, int n, int W, hipStream_t s, double* dprod,
+                            int accumulate) {
   if (W <= 0) return hipSuccess;
   if (n < 1 || n > TROT_MAX_NOCC) return hipErrorInvalidValue;
   const size_t lds = (size_t)2 * n * n * 16;
   hipError_t e = set_lds((const void*)chol_rinv_kernel, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(chol_rinv_kernel, dim3(W), dim3(256), lds, s, gram, rinv, n);
+  hipLaunchKernelGGL(chol_rinv_kernel, dim3(W), dim3(256), lds, s, gram, rinv, n, dprod, accumulate);
   return hipGetLastError();
 }
 

@@ -114,7 +114,7 @@ def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
     assert h.trot_fp_weights(d, None, None, 4, d, d, None) == -1
     assert h.trot_fp_unnorm(d, d, None, 4, None) == -1
     assert h.trot_block_energy_fp(d, d, d, 4, 0.0, 0.0, d, None) == -1  # dt must be positive
-    assert h.trot_qr_norm(d, 4, 8, 65, d, d, big, None) == -2
+    assert h.trot_qr_norm(d, 4, 100, 65, d, d, big, None) == -2
     assert h.trot_qr_norm(d, 4, 8, 2, d, d, 0, None) == -4
     assert h.trot_orthogonalize(None, 4, 8, 2, None, d, big, None) == -1
 

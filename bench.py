@@ -245,8 +245,10 @@ def main():
         block_graph = graph_block_rate(env_blk, mixed_blk, st_blk)
         del mixed_blk, st_blk, env_blk
 
+    fp_rates = None
     if not args.no_block:
         small = launch_bound_case()
+        fp_rates = free_projection_rates(syn, env, args)
 
     cpu = None if args.no_cpu else cpu_baseline(syn, mixed, args, dt)
 
@@ -280,6 +282,7 @@ def main():
         "block_inclusive_walker_steps_per_s": None if block_rate is None else round(block_rate, 2),
         "block_graph": block_graph,
         "launch_bound_case": small,
+        "free_projection": fp_rates,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -446,6 +449,45 @@ def launch_bound_case() -> dict:
            "graph_ms_per_block": round(1e3 * tb_graph, 3), "n_prop_steps": 50,
            "graph_block_walker_steps_per_s": round(W * 50 / tb_graph, 1)}
     log(f"launch-bound case: {out}")
+    return out
+
+
+def free_projection_rates(syn, env, args) -> dict:
+    """Free-projection step (trot_amd.fp, ten Taylor terms, QR norm and comb every step) at the workload's
+    sizes, mixed and FP64, on the scaled system (every L_gamma / sqrt(N)) so that the walkers stay finite."""
+    import torch
+
+    from trot_amd import fp, meas
+    from trot_amd.core import HamChol
+
+    N = env["sys"].norb
+    ham = HamChol(syn.h0, env["ham"].h1, syn.chol * (1.0 / np.sqrt(N)))
+    W = env["params"].n_walkers
+    params = fp.QmcParamsFp(dt=env["params"].dt, n_walkers=W, seed=11, ene0=0.0)
+    out = {"workload": args.workload, "n_exp_terms": fp.FP_TAYLOR_TERMS}
+    for tag, prec, tprec, hi in (("mixed", args.precision, args.taylor_precision, args.taylor_hi_from),
+                                 ("fp64", "f64", "f64", 0)):
+        mops = meas.make_rhf_meas_ops(env["sys"], precision=prec)
+        pops = fp.make_prop_ops_fp("restricted", "restricted", env["sys"], precision=prec, taylor_precision=tprec,
+                                   taylor_hi_from=hi)
+        pctx = pops.build_prop_ctx(ham, env["tops"].get_rdm1(env["tr"]), params)
+        mctx = mops.build_meas_ctx(ham, env["tr"])
+        st = pops.init_prop_state(sys=env["sys"], ham_data=ham, trial_ops=env["tops"], trial_data=env["tr"],
+                                  meas_ops=mops, params=params, initial_walkers=env["w0"])
+        kw = dict(params=params, ham_data=ham, trial_data=env["tr"], trial_ops=env["tops"], meas_ops=mops,
+                  meas_ctx=mctx, prop_ctx=pctx)
+        st = pops.step(st, **kw)
+        torch.cuda.synchronize()
+        steps = 3
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = pops.step(st, **kw)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[f"{tag}_ms_per_step"] = round(1e3 * dt, 3)
+        out[f"{tag}_walker_steps_per_s"] = round(W / dt, 1)
+        log(f"free projection {tag}: {1e3 * dt:.2f} ms/step")
+        del st, pctx, mctx, pops, mops
     return out
 
 

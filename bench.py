@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-block", action="store_true", help="skip the block-tail timing")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (profiling runs)")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replayed step (profiling runs)")
     ap.add_argument("--cpu-walkers", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -193,8 +194,10 @@ def main():
     st_mix, t_mix, vhs_ms = mixed.timed(st_mix, args.warmup, args.steps, f"mixed {args.precision}")
     value = W * args.steps / t_mix
     # the same step replayed from one HIP graph per step (fields drawn on the device)
-    st_mix, t_mix_graph = mixed.timed_graph(st_mix, args.warmup, args.steps, f"mixed {args.precision} graph")
-    rate_graph = W * args.steps / t_mix_graph
+    rate_graph = None
+    if not args.no_graph:
+        st_mix, t_mix_graph = mixed.timed_graph(st_mix, args.warmup, args.steps, f"mixed {args.precision} graph")
+        rate_graph = W * args.steps / t_mix_graph
 
     fp64 = None
     rate64 = None
@@ -278,7 +281,7 @@ def main():
         "fp64_ms_per_step": None if rate64 is None else round(1e3 * W / rate64, 3),
         "speedup_vs_fp64": None if rate64 is None else round(value / rate64, 3),
         "accuracy_vs_fp64": accuracy,
-        "graph_step_walker_steps_per_s": round(rate_graph, 2),
+        "graph_step_walker_steps_per_s": None if rate_graph is None else round(rate_graph, 2),
         "block_inclusive_walker_steps_per_s": None if block_rate is None else round(block_rate, 2),
         "block_graph": block_graph,
         "launch_bound_case": small,

@@ -181,3 +181,54 @@ def test_run_qmc_graph_equals_eager(gpu):
     assert np.array_equal(r_e.block_energies, r_g.block_energies)
     assert np.array_equal(r_e.block_weights, r_g.block_weights)
     assert r_e.mean_energy == r_g.mean_energy
+
+
+def test_graph_block_free_projection(gpu):
+    """GraphBlock(block_fp) and run_qmc_fp with it against the eager free-projection block and driver."""
+    from trot_amd import fp, meas, trial
+    from trot_amd.core import HamChol, System
+    from trot_amd.graph import GraphBlock
+
+    N, n, G, W = 24, 5, 30, 12
+    p = problem(N, n, G, W, seed=61)
+    sys = System(N, (n, n), "restricted")
+    ham = HamChol(p["h0"], p["h1"], p["chol"]).to_device()
+    tr = trial.RhfTrial(p["ca"])
+    mops = meas.make_rhf_meas_ops(sys)
+    tops = trial.make_rhf_trial_ops(sys)
+    params = fp.QmcParamsFp(dt=0.01, n_walkers=W, seed=5, n_prop_steps=4, n_qr_blocks=2, n_blocks=3, n_traj=2,
+                            ene0=-1.0)
+    pops = fp.make_prop_ops_fp("restricted", "restricted", sys)
+    pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
+    mctx = mops.build_meas_ctx(ham, tr)
+    kw = dict(sys=sys, params=params, ham_data=ham, trial_data=tr, trial_ops=tops, meas_ops=mops, meas_ctx=mctx,
+              prop_ops=pops, prop_ctx=pctx)
+    w0 = torch.from_numpy(p["wa"]).cuda()
+
+    def init():
+        return pops.init_prop_state(sys=sys, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops,
+                                    params=params, initial_walkers=w0)
+
+    s_e = init()
+    eager = []
+    for _ in range(3):
+        s_e, o = fp.block_fp(s_e, **kw)
+        eager.append((s_e, o))
+    gb = GraphBlock(fp.block_fp, fp.draws_per_block)
+    s_g = init()
+    for b in range(3):
+        s_g, o = gb(s_g, **kw)
+        torch.cuda.synchronize()
+        se, oe = eager[b]
+        assert s_g.rng_key == se.rng_key
+        for x, y in zip(_flat(s_g), _flat(se)):
+            assert torch.equal(x, y), b
+        for k in ("energy", "weight", "overlap", "abs_overlap"):
+            assert torch.equal(o.scalars[k], oe.scalars[k]), (b, k)
+        torch.randn(4096, device="cuda").sum().item()
+    args = dict(sys=sys, params=params, ham_data=ham, trial_data=tr, meas_ops=mops, trial_ops=tops, prop_ops=pops,
+                meas_ctx=mctx, prop_ctx=pctx, verbose=False)
+    r_e = fp.run_qmc_fp(state=init(), **args)
+    r_g = fp.run_qmc_fp(state=init(), block_fn=GraphBlock(fp.block_fp, fp.draws_per_block), **args)
+    assert np.array_equal(r_e.block_energies, r_g.block_energies)
+    assert np.array_equal(r_e.mean_energy, r_g.mean_energy)

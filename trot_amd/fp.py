@@ -115,6 +115,12 @@ def make_prop_ops_fp(ham_basis: str, walker_kind: str, sys: System, mixed_precis
     return PropOps(init_prop_state=init_prop_state, build_prop_ctx=_build, step=step_fp)
 
 
+def draws_per_block(params) -> int:
+    """Device-stream draws of one free-projection block: fields and a comb offset per step
+    (for trot_amd.graph.GraphBlock(block_fp, draws_per_block))."""
+    return 2 * params.n_prop_steps
+
+
 def block_fp(state: PropState, *, sys: System, params, ham_data, trial_data, trial_ops, meas_ops: MeasOps, meas_ctx,
              prop_ops, prop_ctx, sr_fn: Callable | None = None, observable_names: tuple = (), fields_seq=None,
              zetas=None) -> tuple[PropState, BlockObs]:

@@ -21,13 +21,40 @@ import dataclasses
 import torch
 
 from . import _lib
-from .batch import _static_copy, _tensors
+from .batch import _static_copy
 from .blocks import BlockObs, block as eager_block
-from .prop import DeviceFields, StateFields
+from .prop import DeviceFields, PropState, StateFields
+
+
+_FIELDS = ("walkers", "weights", "overlaps", "pop_control_ene_shift", "e_estimate", "node_encounters", "green")
+
+
+def _copy_state(dst: PropState, src: PropState) -> None:
+    """dst's tensors <- src's, field by field; fields that dst does not hold are skipped."""
+    for name in _FIELDS:
+        d, s = getattr(dst, name), getattr(src, name)
+        if d is None:
+            continue
+        if s is None:
+            raise ValueError(f"the state has no {name}, which the recording holds")
+        d = d if isinstance(d, tuple) else (d,)
+        s = s if isinstance(s, tuple) else (s,)
+        if len(d) != len(s):
+            raise ValueError(f"the state's {name} does not match the recorded structure")
+        for a, b in zip(d, s):
+            if a is None:
+                continue
+            if b is None or a.shape != b.shape:
+                raise ValueError(f"the state's {name} does not match the recorded shapes")
+            a.copy_(b)
 
 
 class _Recorded:
-    """Static state buffers, a device counter and one recorded HIP graph of ``run(static) -> (state, scalars)``."""
+    """Static state buffers, a device counter and one recorded HIP graph of ``run(static) -> (state, scalars)``.
+
+    The first call runs ``run`` eagerly (on a side stream, as graph capture requires) on a copy of the
+    given state; its result fixes the structure of the static buffers and is the first call's result.
+    Then ``run`` is recorded on the static buffers, writing back into them."""
 
     def __init__(self):
         self.graph = None
@@ -38,19 +65,12 @@ class _Recorded:
         self.replays = 0
 
     def _store(self, state_out, scalars):
-        for dst, src in zip(_tensors(self.static), _tensors(state_out)):
-            dst.copy_(src)
-        if scalars is not None:
+        _copy_state(self.static, state_out)
+        if scalars:
             self.obs.copy_(torch.stack([x.reshape(()) for x in scalars]))
 
     def _load(self, state):
-        if state.green is None:
-            raise ValueError("the state needs its Green's-function cache (PropState.green)")
-        dst, new = _tensors(self.static), _tensors(state)
-        if len(dst) != len(new) or any(a.shape != b.shape for a, b in zip(dst, new)):
-            raise ValueError("the state does not match the recorded shapes")
-        for a, b in zip(dst, new):
-            a.copy_(b)
+        _copy_state(self.static, state)
         self.rng.copy_(_lib.rng_state(*state.rng_key))
 
     def _device_ctx(self, kw):
@@ -61,23 +81,22 @@ class _Recorded:
         out["prop_ctx"] = dataclasses.replace(pctx, fields=StateFields(self.rng))
         return out
 
-    def _run(self, state, kw, run, n_draws, n_scalars):
+    def _run(self, state, kw, run, n_draws):
         seed, ctr = state.rng_key
         if self.graph is None:
-            if state.green is None:
-                _, g = kw["meas_ops"].engine.overlap_green(state.walkers, kw["trial_data"])
-                state = state._replace(green=g)
-            self.static = _static_copy(state)
             self.rng = _lib.rng_state(seed, ctr)
-            self.obs = torch.zeros((max(n_scalars, 1),), dtype=torch.float64, device="cuda")
             args = self._device_ctx(kw)
-            # warm-up: eager, on a side stream (graph capture requires it); it is the first call's result
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
-                self._store(*run(self.static, args))
+                out, scalars = run(_static_copy(state), args)
+                self.static = _static_copy(out)
+                if scalars:
+                    self.obs = torch.stack([x.reshape(()) for x in scalars]).to(torch.complex128 if any(
+                        x.is_complex() for x in scalars) else torch.float64).contiguous()
             torch.cuda.current_stream().wait_stream(side)
             torch.cuda.synchronize()
+            del out, scalars
             self.graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(self.graph):
                 self._store(*run(self.static, args))
@@ -96,9 +115,12 @@ class GraphStep(_Recorded):
     """``prop_ops.step`` recorded as one HIP graph: callable with the step's signature (no ``fields``:
     they are drawn on the device).  The first call runs one step eagerly and records the next."""
 
-    def __init__(self, prop_ops):
+    def __init__(self, prop_ops, draws_per_step: int = 1):
+        """``draws_per_step``: device-stream draws of one step (1 for the phaseless step, 2 for free
+        projection: fields and the comb offset)."""
         super().__init__()
         self.prop_ops = prop_ops
+        self.draws_per_step = draws_per_step
 
     def __call__(self, state, **kw):
         if kw.pop("fields", None) is not None:
@@ -107,7 +129,7 @@ class GraphStep(_Recorded):
         def run(st, args):
             return self.prop_ops.step(st, **args), None
 
-        return self._run(state, kw, run, 1, 0)
+        return self._run(state, kw, run, self.draws_per_step)
 
 
 class GraphBlock(_Recorded):
@@ -119,9 +141,13 @@ class GraphBlock(_Recorded):
     counter.  Block scalars are returned as fresh GPU tensors (copied out of the graph's buffers).
     """
 
-    def __init__(self, block_fn=eager_block):
+    def __init__(self, block_fn=eager_block, draws_per_block=None):
+        """``draws_per_block(params)``: device-stream draws one block makes (default: a field draw per step
+        and one comb offset, the phaseless block; the free-projection block draws two per step)."""
         super().__init__()
         self.block_fn = block_fn
+        self.draws_per_block = draws_per_block or (lambda params: params.n_prop_steps + 1)
+        self.names = None
 
     @property
     def blocks_done(self) -> int:
@@ -133,9 +159,10 @@ class GraphBlock(_Recorded):
 
         def run(st, args):
             out, obs = self.block_fn(st, **args)
-            return out, (obs.scalars["energy"], obs.scalars["weight"])
+            if self.names is None:
+                self.names = list(obs.scalars)
+            return out, tuple(obs.scalars[k] for k in self.names)
 
-        n_draws = kw["params"].n_prop_steps + 1  # one field draw per step, one comb offset
-        st = self._run(state, kw, run, n_draws, 2)
-        obs = BlockObs(scalars={"energy": self.obs[0].clone(), "weight": self.obs[1].clone()}, observables={})
+        st = self._run(state, kw, run, self.draws_per_block(kw["params"]))
+        obs = BlockObs(scalars={k: self.obs[i].clone() for i, k in enumerate(self.names)}, observables={})
         return st, obs

@@ -326,12 +326,16 @@ __device__ __forceinline__ void b3w_mainloop(const B3P& p, int m0, int n0, int b
         }
       }
     };
+    __builtin_amdgcn_s_setprio(1);
     mfma_rows(0);
+    __builtin_amdgcn_s_setprio(0);
     if (kc + 1 < nk) {
       store(cur ^ 1);
       if (kc + 2 < nk) load(kbeg + (kc + 2) * BK);
     }
+    __builtin_amdgcn_s_setprio(1);
     mfma_rows(2);
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
   }
 }

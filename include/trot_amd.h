@@ -329,6 +329,35 @@ int trot_local_energy_lp(const float* rot_chol_a, const float* rot_chol_b, const
                          int na, int nb, int G, double h0, double* energy, int prec, void* ws, size_t ws_bytes,
                          void* stream);
 
+/* ---- grouped launches: one launch per operation over many walker sets (the batching form) ----
+ * Each operation of the FP64 step over many independent walker sets (LNO-style fragments) runs as
+ * one launch: trot_gemm_problem describes one walker set's problem (the same operands, strides and K
+ * order as the operation's own entry point: trot_one_body_apply, one term of trot_taylor_apply_symblk,
+ * the two GEMMs of trot_overlap_green, trot_vhs_build) and returns its workgroup count; the caller
+ * gathers the descriptors of all sets into device memory with prefix sums of the counts and launches
+ * trot_gemm_f64_grouped.  Every workgroup computes exactly what it would in a launch of its own set, so
+ * results are bit-identical to launching the sets one at a time.  trot_inv_problem / trot_inv_det_grouped
+ * do the same for the inverse and determinant of trot_overlap_green, trot_copy_problem /
+ * trot_copy_grouped for device-to-device copies (sizes multiples of 16 bytes).
+ * Replaces: running trot/prop/afqmc.py:82-144 once per fragment (the reference batches nothing across
+ * fragments; its LNO driver loops over them). */
+#define TROT_GOP_ONE_BODY 0
+#define TROT_GOP_TAYLOR 1
+#define TROT_GOP_OVERLAP 2
+#define TROT_GOP_GREEN 3
+#define TROT_GOP_VHS 4
+#define TROT_GOP_TAYLOR_NARROW 5
+size_t trot_gemm_problem_bytes(void);
+long trot_gemm_problem(int op, const double* a, const double* b, double* c, double* d, int W, int N, int n,
+                       double coef, void* desc);
+int trot_gemm_f64_grouped(int op, const void* probs, const int* start, int nprob, int total_wg, void* stream);
+size_t trot_inv_problem_bytes(void);
+long trot_inv_problem(const double* m, double* minv, double* det, int W, int n, void* desc);
+int trot_inv_det_grouped(const void* probs, const int* start, int nprob, int total_w, void* stream);
+size_t trot_copy_problem_bytes(void);
+int trot_copy_problem(const void* src, void* dst, long bytes, void* desc);
+int trot_copy_grouped(const void* probs, int nprob, long max_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,12 +53,14 @@ def _snapshot(st):
         st.node_encounters.cpu().numpy()]
 
 
-def test_fragment_batch_bit_identical(gpu):
+@pytest.mark.parametrize("grouped", [True, False])
+def test_fragment_batch_bit_identical(gpu, grouped):
     from trot_amd import batch, synthetic
 
+    print(f"[batch] grouped launches: {grouped}", flush=True)
     frags = _fragments()
     init = [f.state for f in frags]
-    fb = batch.FragmentBatch(frags)
+    fb = batch.FragmentBatch(frags, grouped=grouped)
     tiles = fb.autotune()  # per (walkers, N, n): one of the four FP64 Taylor tile shapes, timed on the GPU
     fields = [[synthetic.fields(100 + i, s, W, f.n_fields) for i, f in enumerate(frags)] for s in range(STEPS)]
     checkpoints = {1, 10, STEPS}
@@ -93,7 +95,7 @@ def test_fragment_batch_bit_identical(gpu):
     torch.cuda.synchronize()
     t_graph = time.perf_counter() - t0
     n_finite = sum(bool(np.all(np.isfinite(ref[(i, STEPS)][-4]))) for i in range(len(frags)))
-    print(f"[batch] 32 fragments x {STEPS} steps: eager one-by-one {t_eager:.2f} s, "
+    print(f"[batch] grouped {grouped}: 32 fragments x {STEPS} steps: eager one-by-one {t_eager:.2f} s, "
           f"graph {t_graph:.2f} s for {STEPS - 1} steps ({1e3 * t_graph / (STEPS - 1):.2f} ms/step); "
           f"fragments with finite weights after {STEPS} steps: {n_finite}; Taylor tiles "
           f"{sorted(set(tiles.values()))} over {len(tiles)} shapes", flush=True)

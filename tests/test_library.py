@@ -117,6 +117,24 @@ def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
     assert h.trot_qr_norm(d, 4, 100, 65, d, d, big, None) == -2
     assert h.trot_qr_norm(d, 4, 8, 2, d, d, 0, None) == -4
     assert h.trot_orthogonalize(None, 4, 8, 2, None, d, big, None) == -1
+    # grouped launches: the descriptors are filled on the host, the launches refuse empty groups
+    desc = ctypes.create_string_buffer(h.trot_gemm_problem_bytes())
+    assert h.trot_gemm_problem(9, d, d, d, d, 4, 32, 8, 0.1, desc) == -1  # unknown operation
+    assert h.trot_gemm_problem(1, d, d, d, None, 4, 32, 8, 0.1, desc) == -1  # Taylor needs the running sum
+    assert h.trot_gemm_problem(0, d, d, d, None, 0, 32, 8, 0.0, desc) == -1  # no walkers
+    wide = h.trot_gemm_problem(1, d, d, d, d, 4, 128, 16, 0.1, desc)
+    narrow = h.trot_gemm_problem(5, d, d, d, d, 4, 128, 16, 0.1, desc)
+    assert 0 < wide <= narrow  # the same problem in 32 x 64 and 32 x 32 tiles
+    assert h.trot_gemm_f64_grouped(0, d, d, 0, 1, None) == -1
+    assert h.trot_gemm_f64_grouped(9, d, d, 1, 1, None) == -1
+    idesc = ctypes.create_string_buffer(h.trot_inv_problem_bytes())
+    assert h.trot_inv_problem(d, d, d, 4, 65, idesc) == -2  # nocc above the kernels' limit
+    assert h.trot_inv_problem(d, d, d, 5, 8, idesc) == 5  # one inverse per walker
+    assert h.trot_inv_det_grouped(None, d, 1, 1, None) == -1
+    cdesc = ctypes.create_string_buffer(h.trot_copy_problem_bytes())
+    assert h.trot_copy_problem(d, d, 17, cdesc) == -1  # sizes are multiples of 16 bytes
+    assert h.trot_copy_problem(d, d, 32, cdesc) == 0
+    assert h.trot_copy_grouped(d, 0, 32, None) == -1
 
 
 def test_workspace_queries_are_sizes():

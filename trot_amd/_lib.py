@@ -70,6 +70,15 @@ _SIGS = {
     "trot_sr_comb": (_i, [_vp, _d, _i, _vp, _vp, _vp, _sz, _vp]),
     "trot_gather_rows": (_i, [_vp, _vp, _vp, _i, _l, _vp]),
     "trot_sr_comb_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _sz, _vp]),
+    "trot_gemm_problem_bytes": (_sz, []),
+    "trot_gemm_problem": (_l, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _vp]),
+    "trot_gemm_f64_grouped": (_i, [_i, _vp, _vp, _i, _i, _vp]),
+    "trot_inv_problem_bytes": (_sz, []),
+    "trot_inv_problem": (_l, [_vp, _vp, _vp, _i, _i, _vp]),
+    "trot_inv_det_grouped": (_i, [_vp, _vp, _i, _i, _vp]),
+    "trot_copy_problem_bytes": (_sz, []),
+    "trot_copy_problem": (_i, [_vp, _vp, _l, _vp]),
+    "trot_copy_grouped": (_i, [_vp, _i, _l, _vp]),
     "trot_trotter_b3v": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
     "trot_philox_raw": (_i, [_u64, _u64, _u64, _l, _vp, _vp]),
     "trot_philox_normal": (_i, [_vp, _u64, _u64, _l, _vp, _vp]),
@@ -218,15 +227,22 @@ def force_bias(green_t: torch.Tensor, rot_chol_t: torch.Tensor, alpha: float, ou
     return out
 
 
-def field_shift(fields: torch.Tensor, fb: torch.Tensor, mf: torch.Tensor, sqrt_dt: float):
+def field_shift(fields: torch.Tensor, fb: torch.Tensor, mf: torch.Tensor, sqrt_dt: float, out=None):
+    """(xbar, shift_term, fb_term); ``out`` = three preallocated tensors of those shapes to write into."""
     W, G = fields.shape
     _req(fields, "fields", F64, (W, G))
     _req(fb, "force_bias", C128, (W, G))
     _req(mf, "mf_shifts", C128, (G,))
     dev = fields.device
-    xbar = torch.empty((W, G), dtype=C128, device=dev)
-    shift = torch.empty((W,), dtype=C128, device=dev)
-    fbt = torch.empty((W,), dtype=C128, device=dev)
+    if out is None:
+        xbar = torch.empty((W, G), dtype=C128, device=dev)
+        shift = torch.empty((W,), dtype=C128, device=dev)
+        fbt = torch.empty((W,), dtype=C128, device=dev)
+    else:
+        xbar, shift, fbt = out
+        _req(xbar, "xbar", C128, (W, G))
+        _req(shift, "shift_term", C128, (W,))
+        _req(fbt, "fb_term", C128, (W,))
     _check(lib().trot_field_shift(_p(fields), _p(fb), _p(mf), float(sqrt_dt), W, G, _p(xbar), _p(shift), _p(fbt),
                                   _stream()), "trot_field_shift")
     return xbar, shift, fbt
@@ -796,3 +812,78 @@ def taylor_apply_lp_symblk(vpk32: torch.Tensor, w: torch.Tensor, sqrt_dt: float,
     _check(L.trot_taylor_apply_lp_symblk(_p(vpk32), _p(out), W, N, n, float(sqrt_dt), int(n_terms), _prec(prec),
                                          _p(ws), ws.numel(), _stream()), "trot_taylor_apply_lp_symblk")
     return out
+
+
+# ---------------------------------------------------------------------------
+# grouped launches over many walker sets
+# ---------------------------------------------------------------------------
+GOP = {"one_body": 0, "taylor": 1, "overlap": 2, "green": 3, "vhs": 4, "taylor_narrow": 5}
+
+
+class Grouped:
+    """Descriptors of one operation over many walker sets, gathered into device memory, and its launch.
+
+    ``add(...)`` takes the operands of one set (tensors whose storage must outlive every launch; the
+    descriptors hold their addresses); ``finish()`` uploads the table; ``launch()`` runs all sets in one
+    kernel launch on the current stream."""
+
+    def __init__(self, kind: str, op: str | None = None):
+        self.kind, self.op = kind, op
+        L = lib()
+        self.size = {"gemm": L.trot_gemm_problem_bytes, "inv": L.trot_inv_problem_bytes,
+                     "copy": L.trot_copy_problem_bytes}[kind]()
+        self.descs, self.counts, self.keep = [], [], []
+        self.max_bytes = 0
+        self.probs = self.start = None
+
+    def add_gemm(self, a, b, c, d, W: int, N: int, n: int, coef: float = 0.0):
+        buf = ctypes.create_string_buffer(self.size)
+        wg = lib().trot_gemm_problem(GOP[self.op], _p(a), _p(b), _p(c), _p(d), int(W), int(N), int(n), float(coef),
+                                     buf)
+        _check(0 if wg > 0 else int(wg), "trot_gemm_problem")
+        self.descs.append(buf.raw)
+        self.counts.append(int(wg))
+        self.keep += [a, b, c, d]
+
+    def add_inv(self, m, minv, det, W: int, n: int):
+        buf = ctypes.create_string_buffer(self.size)
+        r = lib().trot_inv_problem(_p(m), _p(minv), _p(det), int(W), int(n), buf)
+        _check(0 if r > 0 else int(r), "trot_inv_problem")
+        self.descs.append(buf.raw)
+        self.counts.append(int(r))
+        self.keep += [m, minv, det]
+
+    def add_copy(self, src: torch.Tensor, dst: torch.Tensor):
+        if src.numel() != dst.numel() or src.dtype != dst.dtype or not (src.is_contiguous() and dst.is_contiguous()):
+            raise ValueError("copy: matching contiguous tensors needed")
+        nbytes = src.numel() * src.element_size()
+        buf = ctypes.create_string_buffer(self.size)
+        _check(lib().trot_copy_problem(_p(src), _p(dst), nbytes, buf), "trot_copy_problem")
+        self.descs.append(buf.raw)
+        self.counts.append(1)
+        self.max_bytes = max(self.max_bytes, nbytes)
+        self.keep += [src, dst]
+
+    def finish(self, device="cuda"):
+        raw = b"".join(self.descs)
+        self.probs = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+        starts = [0]
+        for c in self.counts:
+            starts.append(starts[-1] + c)
+        if starts[-1] >= 2**31:
+            raise ValueError("grouped launch too large")
+        self.start = torch.tensor(starts, dtype=torch.int32, device=device)
+        self.total = starts[-1]
+        return self
+
+    def launch(self):
+        L = lib()
+        n = len(self.descs)
+        if self.kind == "gemm":
+            _check(L.trot_gemm_f64_grouped(GOP[self.op], _p(self.probs), _p(self.start), n, self.total, _stream()),
+                   "trot_gemm_f64_grouped")
+        elif self.kind == "inv":
+            _check(L.trot_inv_det_grouped(_p(self.probs), _p(self.start), n, self.total, _stream()),
+                   "trot_inv_det_grouped")
+        else:
+            _check(L.trot_copy_grouped(_p(self.probs), n, self.max_bytes, _stream()), "trot_copy_grouped")

@@ -20,6 +20,7 @@ from typing import Any, List
 
 import torch
 
+from . import _lib
 from .prop import PropState
 
 
@@ -85,6 +86,88 @@ def _cost(f: Fragment) -> float:
     return float(f.n_walkers) * N * N * (f.n_fields + 8 * f.params.n_exp_terms * n)
 
 
+def _groupable(f: Fragment) -> bool:
+    """The FP64 step with sym-block V and all walkers in one chunk: the form the grouped launches cover."""
+    ctx = f.prop_ctx
+    eng = f.meas_ops.engine
+    return (ctx.precision == "f64" and ctx.taylor_precision == "f64" and bool(ctx.low.get("symblk"))
+            and 0 >= ctx.walker_chunk and eng.precision == "f64"
+            and f.sys.walker_kind in ("restricted", "unrestricted") and max(f.sys.nelec) <= _lib.MAX_NOCC)
+
+
+class _GroupedPlan:
+    """Static buffers of every fragment and the grouped launches of one step (FP64, sym-block V).
+
+    Per fragment, on the branches: force bias from the cached Green's functions and the shifted fields
+    (into static xbar / shift / bias buffers).  Then, one launch each over all fragments and spins: VHS,
+    one-body, copy of the first term, the Taylor terms, one-body, C^T w, inverse and determinant,
+    Green's functions (into the static state).  Then, per fragment: overlaps and the weight update.
+    Every launch runs, per fragment, the problem of the fragment's own entry point, so the step is
+    bit-identical to stepping the fragments one at a time."""
+
+    def __init__(self, frags: List[Fragment]):
+        from .trial import _coeffs
+
+        n_terms = {int(f.params.n_exp_terms) for f in frags}
+        if len(n_terms) != 1:
+            raise ValueError("grouped step: fragments must share n_exp_terms")
+        self.n_terms = n_terms.pop()
+        self.per = []
+        dev = "cuda"
+        vhs = _lib.Grouped("gemm", "vhs")
+        ob1 = _lib.Grouped("gemm", "one_body")
+        cp = _lib.Grouped("copy")
+        tay = [_lib.Grouped("gemm", "taylor") for _ in range(self.n_terms)]
+        tayn = [_lib.Grouped("gemm", "taylor_narrow") for _ in range(self.n_terms)]  # n <= 32: 32 x 32 tiles
+        ob2 = _lib.Grouped("gemm", "one_body")
+        ovl = _lib.Grouped("gemm", "overlap")
+        inv = _lib.Grouped("inv")
+        grn = _lib.Grouped("gemm", "green")
+        for f in frags:
+            ctx, st = f.prop_ctx, f.state
+            W = f.n_walkers
+            G = f.n_fields
+            N = ctx.norb
+            chol = ctx.low["chol_symblk"]
+            vcols = chol.shape[1]
+            b = dict(xbar=torch.empty((W, G), dtype=torch.complex128, device=dev),
+                     shift=torch.empty((W,), dtype=torch.complex128, device=dev),
+                     fbt=torch.empty((W,), dtype=torch.complex128, device=dev),
+                     v=torch.empty((W, vcols), dtype=torch.complex128, device=dev),
+                     det=[], spins=[])
+            vhs.add_gemm(b["xbar"], chol, b["v"], None, W, vcols, G)
+            walkers = st.walkers if isinstance(st.walkers, tuple) else (st.walkers,)
+            greens = [g for g in st.green if g is not None]
+            coeffs = _coeffs(f.trial)
+            for si, w in enumerate(walkers):
+                n = w.shape[2]
+                c = coeffs[si].contiguous()
+                t = torch.empty_like(w)
+                t0, t1 = torch.empty_like(w), torch.empty_like(w)
+                m = torch.empty((W, n, n), dtype=torch.complex128, device=dev)
+                minv = torch.empty_like(m)
+                det = torch.empty((W,), dtype=torch.complex128, device=dev)
+                ob1.add_gemm(ctx.exp_h1_half, w, t, None, W, N, n)
+                cp.add_copy(t, t0)
+                prev, nxt = t0, t1
+                for k in range(1, self.n_terms + 1):
+                    (tayn if n <= 32 else tay)[k - 1].add_gemm(b["v"], prev, nxt, t, W, N, n, ctx.sqrt_dt / k)
+                    prev, nxt = nxt, prev
+                ob2.add_gemm(ctx.exp_h1_half, t, w, None, W, N, n)
+                ovl.add_gemm(c, w, m, None, W, N, n)
+                inv.add_inv(m, minv, det, W, n)
+                grn.add_gemm(w, minv, greens[si], None, W, N, n)
+                b["det"].append(det)
+                b["spins"].append((t, t0, t1, m, minv, c))
+            self.per.append(b)
+        terms = [g for pair in zip(tay, tayn) for g in pair if g.descs]
+        self.ops = [g.finish() for g in (vhs, ob1, cp, *terms, ob2, ovl, inv, grn)]
+
+    def launch(self):
+        for g in self.ops:
+            g.launch()
+
+
 class FragmentBatch:
     """A list of fragments stepped together through one captured HIP graph.
 
@@ -92,9 +175,14 @@ class FragmentBatch:
     least-loaded stream (longest-processing-time first), each branch runs its fragments in order, and
     the graph joins the branches at the end, so fragments on different branches may run concurrently.
     Per fragment the kernels and their order are the same as in a one-by-one run.
+
+    ``grouped`` (default: when every fragment runs the FP64 step on sym-block V): the step's GEMMs,
+    inverses and copies run as one launch each over all fragments (trot_gemm_f64_grouped,
+    trot_inv_det_grouped, trot_copy_grouped) between two per-fragment phases on the branches; the
+    results stay bit-identical to the one-by-one run.
     """
 
-    def __init__(self, fragments: List[Fragment], n_streams: int = 4):
+    def __init__(self, fragments: List[Fragment], n_streams: int = 4, grouped: bool | None = None):
         if not fragments:
             raise ValueError("FragmentBatch needs at least one fragment")
         if n_streams < 1:
@@ -115,6 +203,11 @@ class FragmentBatch:
         self.fields = torch.zeros(self.offsets[-1], dtype=torch.float64, device="cuda")
         self.graph = None
         self.steps_done = 0
+        can = all(_groupable(f) for f in fragments)
+        if grouped and not can:
+            raise ValueError("grouped step: every fragment must run the FP64 step on sym-block V")
+        self.grouped = can if grouped is None else bool(grouped)
+        self.plan = None
 
     def autotune(self):
         """Pick the FP64 Taylor-term tile shape of every fragment's (walkers, N, n) per spin by timing
@@ -145,7 +238,54 @@ class FragmentBatch:
             dst.copy_(src)
         f.state = f.state._replace(rng_key=new.rng_key)
 
+    def _phase(self, views, fn):
+        main = torch.cuda.current_stream()
+        for st, idx in zip(self.streams, self.branch):
+            st.wait_stream(main)
+            with torch.cuda.stream(st):
+                for i in idx:
+                    fn(i, views)
+        for st in self.streams:
+            main.wait_stream(st)
+
+    def _pre(self, i, views):
+        f, b = self.fragments[i], self.plan.per[i]
+        fb = f.meas_ops.engine.force_bias_from_green(f.state.green, f.meas_ctx)  # afqmc.py:95-98
+        _lib.field_shift(views[i], fb, f.prop_ctx.mf_shifts, f.prop_ctx.sqrt_dt,
+                         out=(b["xbar"], b["shift"], b["fbt"]))  # afqmc.py:99-103
+
+    def _post(self, i, views):
+        f, b = self.fragments[i], self.plan.per[i]
+        st, ctx, params = f.state, f.prop_ctx, f.params
+        det = b["det"]
+        ov_new = _lib.overlap_combine(det[0], det[1] if len(det) > 1 else None)  # afqmc.py:108-110
+        _lib.phaseless_update(st.weights, st.overlaps, ov_new, b["shift"], b["fbt"], b["scal"], st.node_encounters,
+                              ctx.dt, ctx.h0_prop, getattr(params, "weight_floor", 1.0e-3),
+                              getattr(params, "weight_cap", 100.0), getattr(params, "pop_control_damping", 0.1))
+        st.overlaps.copy_(ov_new)
+
+    def _step_grouped(self, views):
+        self._phase(views, self._pre)
+        self.plan.launch()
+        self._phase(views, self._post)
+        for f in self.fragments:  # the fields came from the caller: one draw of the key per step
+            s, c = f.state.rng_key
+            f.state = f.state._replace(rng_key=(s, c + 1))
+
+    def _make_plan(self):
+        """Static state with the scalars as views of one (2,) buffer the weight update writes in place."""
+        for f in self.fragments:
+            scal = torch.stack([f.state.pop_control_ene_shift.reshape(()), f.state.e_estimate.reshape(())]).to(
+                torch.float64).contiguous()
+            f.state = f.state._replace(pop_control_ene_shift=scal[0], e_estimate=scal[1])
+            f._scal = scal
+        self.plan = _GroupedPlan(self.fragments)
+        for f, b in zip(self.fragments, self.plan.per):
+            b["scal"] = f._scal
+
     def _step_all(self, views):
+        if self.grouped:
+            return self._step_grouped(views)
         main = torch.cuda.current_stream()
         for st, idx in zip(self.streams, self.branch):
             st.wait_stream(main)
@@ -163,6 +303,8 @@ class FragmentBatch:
                 _, g = f.meas_ops.engine.overlap_green(f.state.walkers, f.trial)
                 f.state = f.state._replace(green=g)
             f.state = _static_copy(f.state)
+        if self.grouped:
+            self._make_plan()
         views = self.field_views()
         self.fields.copy_(warmup_fields)
         side = torch.cuda.Stream()

@@ -37,6 +37,53 @@ size_t trot_overlap_green_workspace(int W, int N, int n) {
   return 2 * align256((size_t)W * n * n * 16);
 }
 
+// ---------------------------------------------------------------------------
+// GEMM problems of the step, built in one place so that a grouped launch over many walker sets runs
+// exactly the problem a launch of its own would (same operands, strides and tile order of K).
+// ---------------------------------------------------------------------------
+static GemmP problem_overlap(const double* trial, const double* walkers, double* m, int N, int n) {
+  GemmP p{};  // m[b] = C^T w[b]  (n x n)
+  p.A = trial; p.lda = n; p.sA = 0;
+  p.B = walkers; p.ldb = n; p.sB = (long)N * n;
+  p.C = m; p.ldc = n; p.sC = (long)n * n;
+  p.M = n; p.N = n; p.K = N; p.ar = 1.0;
+  return p;
+}
+static GemmP problem_green(const double* walkers, const double* minv, double* green_t, int N, int n) {
+  GemmP q{};  // GT[b] = w[b] m[b]^-1  (N x n)
+  q.A = walkers; q.lda = n; q.sA = (long)N * n;
+  q.B = minv; q.ldb = n; q.sB = (long)n * n;
+  q.C = green_t; q.ldc = n; q.sC = (long)N * n;
+  q.M = N; q.N = n; q.K = n; q.ar = 1.0;
+  return q;
+}
+static GemmP problem_vhs(const double* xbar, const double* chol, double* vhs, int G, long N2, int W) {
+  GemmP p{};
+  p.A = xbar; p.lda = G;
+  p.B = chol; p.ldb = N2;
+  p.C = vhs; p.ldc = N2;
+  p.M = W; p.N = (int)N2; p.K = G; p.ar = 1.0;
+  return p;
+}
+static GemmP problem_one_body(const double* expm, const double* w_in, double* w_out, int N, int n) {
+  GemmP p{};
+  p.A = expm; p.lda = N; p.sA = 0;
+  p.B = w_in; p.ldb = n; p.sB = (long)N * n;
+  p.C = w_out; p.ldc = n; p.sC = (long)N * n;
+  p.M = N; p.N = n; p.K = N; p.ar = 1.0;
+  return p;
+}
+static GemmP problem_taylor_symblk(const double* vhs_symblk, const double* prev, double* next, double* acc, int N,
+                                   int n, double coef) {
+  GemmP p{};
+  p.A = vhs_symblk; p.lda = (N + 7) / 8; p.sA = symblk_count((N + 7) / 8) * 64;
+  p.B = prev; p.ldb = n; p.sB = (long)N * n;
+  p.C = next; p.ldc = n; p.sC = (long)N * n;
+  p.C2 = acc; p.ldc2 = n; p.sC2 = (long)N * n;
+  p.M = N; p.N = n; p.K = N; p.ar = 0.0; p.ai = coef;
+  return p;
+}
+
 int trot_overlap_green(const double* trial, const double* walkers, int W, int N, int n, double* det,
                        double* green_t, void* ws, size_t ws_bytes, void* stream) {
   if (!trial || !walkers || !det) return fail(TROT_ERR_INVALID, "overlap_green: null pointer");
@@ -47,23 +94,12 @@ int trot_overlap_green(const double* trial, const double* walkers, int W, int N,
   double* minv = c.take<double>((size_t)W * n * n * 2);
   if (!m || !minv) return fail(TROT_ERR_WORKSPACE, "overlap_green: workspace too small");
   hipStream_t s = (hipStream_t)stream;
-  // m[b] = C^T w[b]  (n x n)
-  GemmP p{};
-  p.A = trial; p.lda = n; p.sA = 0;
-  p.B = walkers; p.ldb = n; p.sB = (long)N * n;
-  p.C = m; p.ldc = n; p.sC = (long)n * n;
-  p.M = n; p.N = n; p.K = N; p.ar = 1.0;
-  HIPCHK((launch_gemm_f64<64, 64, false, true, false, true, false, EPI_STORE>(p, W, s)));
+  HIPCHK((launch_gemm_f64<64, 64, false, true, false, true, false, EPI_STORE>(
+      problem_overlap(trial, walkers, m, N, n), W, s)));
   HIPCHK(launch_inv_det(m, minv, det, n, W, s));
-  if (green_t) {
-    // GT[b] = w[b] m[b]^-1  (N x n)
-    GemmP q{};
-    q.A = walkers; q.lda = n; q.sA = (long)N * n;
-    q.B = minv; q.ldb = n; q.sB = (long)n * n;
-    q.C = green_t; q.ldc = n; q.sC = (long)N * n;
-    q.M = N; q.N = n; q.K = n; q.ar = 1.0;
-    HIPCHK((launch_gemm_f64<64, 64, true, true, true, true, false, EPI_STORE>(q, W, s)));
-  }
+  if (green_t)
+    HIPCHK((launch_gemm_f64<64, 64, true, true, true, true, false, EPI_STORE>(
+        problem_green(walkers, minv, green_t, N, n), W, s)));
   return TROT_OK;
 }
 
@@ -145,12 +181,8 @@ int trot_field_shift(const double* fields, const double* fb, const double* mf, d
 int trot_vhs_build(const double* xbar, const double* chol_flat, int W, int G, long N2, double* vhs, void* stream) {
   if (!xbar || !chol_flat || !vhs) return fail(TROT_ERR_INVALID, "vhs_build: null pointer");
   if (W < 1 || G < 1 || N2 < 1) return fail(TROT_ERR_INVALID, "vhs_build: bad sizes");
-  GemmP p{};
-  p.A = xbar; p.lda = G;
-  p.B = chol_flat; p.ldb = N2;
-  p.C = vhs; p.ldc = N2;
-  p.M = W; p.N = (int)N2; p.K = G; p.ar = 1.0;
-  HIPCHK((launch_gemm_f64<64, 64, true, false, true, true, false, EPI_STORE>(p, 1, (hipStream_t)stream)));
+  HIPCHK((launch_gemm_f64<64, 64, true, false, true, true, false, EPI_STORE>(problem_vhs(xbar, chol_flat, vhs, G, N2, W),
+                                                                            1, (hipStream_t)stream)));
   return TROT_OK;
 }
 
@@ -161,12 +193,8 @@ int trot_one_body_apply(const double* expm, const double* w_in, int W, int N, in
   if (!expm || !w_in || !w_out) return fail(TROT_ERR_INVALID, "one_body_apply: null pointer");
   if (w_in == w_out) return fail(TROT_ERR_INVALID, "one_body_apply: in-place not supported");
   if (W < 1 || N < 1 || n < 1) return fail(TROT_ERR_INVALID, "one_body_apply: bad sizes");
-  GemmP p{};
-  p.A = expm; p.lda = N; p.sA = 0;
-  p.B = w_in; p.ldb = n; p.sB = (long)N * n;
-  p.C = w_out; p.ldc = n; p.sC = (long)N * n;
-  p.M = N; p.N = n; p.K = N; p.ar = 1.0;
-  HIPCHK((launch_gemm_f64<64, 64, false, true, true, true, false, EPI_STORE>(p, W, (hipStream_t)stream)));
+  HIPCHK((launch_gemm_f64<64, 64, false, true, true, true, false, EPI_STORE>(problem_one_body(expm, w_in, w_out, N, n),
+                                                                             W, (hipStream_t)stream)));
   return TROT_OK;
 }
 
@@ -260,14 +288,9 @@ int trot_taylor_apply_symblk(const double* vhs_symblk, double* w, int W, int N, 
   const int nb = (N + 7) / 8;
   double* prev = t0;
   double* next = t1;
+  (void)nb;
   for (int k = 1; k <= n_terms; ++k) {
-    GemmP p{};
-    p.A = vhs_symblk; p.lda = nb; p.sA = trot_symblk_cols(N);
-    p.B = prev; p.ldb = n; p.sB = (long)N * n;
-    p.C = next; p.ldc = n; p.sC = (long)N * n;
-    p.C2 = w; p.ldc2 = n; p.sC2 = (long)N * n;
-    p.M = N; p.N = n; p.K = N; p.ar = 0.0; p.ai = sqrt_dt / k;
-    HIPCHK(launch_taylor_term<true>(shape, p, W, s));
+    HIPCHK(launch_taylor_term<true>(shape, problem_taylor_symblk(vhs_symblk, prev, next, w, N, n, sqrt_dt / k), W, s));
     double* tmp = prev;
     prev = next;
     next = tmp;
@@ -608,6 +631,104 @@ int trot_build_prop_tensors(const double* chol, const double* dm, int G, int N, 
     HIPCHK((launch_gemm_f64<64, 64, false, false, true, true, false, EPI_STORE>(p, nb, s)));
     HIPCHK(launch_sum_batches(tmp, v0m, nb, (long)N * N, 0.5, g0 > 0, s));
   }
+  return TROT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// grouped launches: one launch per operation over many walker sets (the batching form)
+// ---------------------------------------------------------------------------
+size_t trot_gemm_problem_bytes(void) { return sizeof(GemmP); }
+size_t trot_inv_problem_bytes(void) { return sizeof(InvProb); }
+size_t trot_copy_problem_bytes(void) { return sizeof(CopyProb); }
+
+// Fills desc (trot_gemm_problem_bytes()) with the problem of one operation over W walkers and returns the
+// number of workgroups it takes in trot_gemm_f64_grouped, or a negative status.  Arguments, by op:
+//   TROT_GOP_ONE_BODY  a = exp(-dt/2 h1_eff) (N x N), b = w_in, c = w_out
+//   TROT_GOP_TAYLOR    a = V (sym-block), b = previous term, c = next term, d = running sum, coef = sqrt(dt)/k
+//                      (32 x 64 tiles; TROT_GOP_TAYLOR_NARROW the same with 32 x 32 tiles, for n <= 32)
+//   TROT_GOP_OVERLAP   a = trial (N x n), b = walkers, c = C^T w (W, n, n)
+//   TROT_GOP_GREEN     a = walkers, b = (C^T w)^-1, c = green_t (W, N, n)
+//   TROT_GOP_VHS       a = xbar (W, G), b = chol (G, N2), c = vhs (W, N2); n = G, N = N2 (N2 < 2^31)
+long trot_gemm_problem(int op, const double* a, const double* b, double* c, double* d, int W, int N, int n,
+                       double coef, void* desc) {
+  if (!a || !b || !c || !desc || W < 1 || N < 1 || n < 1) return fail(TROT_ERR_INVALID, "gemm_problem: bad arguments");
+  GemmP p{};
+  long wg = 0;
+  switch (op) {
+    case TROT_GOP_ONE_BODY: p = problem_one_body(a, b, c, N, n); wg = gemm_f64_workgroups<64, 64>(p, W); break;
+    case TROT_GOP_TAYLOR:
+      if (!d) return fail(TROT_ERR_INVALID, "gemm_problem: Taylor needs the running sum");
+      p = problem_taylor_symblk(a, b, c, d, N, n, coef);
+      wg = gemm_f64_workgroups<32, 64>(p, W);
+      break;
+    case TROT_GOP_TAYLOR_NARROW:
+      if (!d) return fail(TROT_ERR_INVALID, "gemm_problem: Taylor needs the running sum");
+      p = problem_taylor_symblk(a, b, c, d, N, n, coef);
+      wg = gemm_f64_workgroups<32, 32>(p, W);
+      break;
+    case TROT_GOP_OVERLAP: p = problem_overlap(a, b, c, N, n); wg = gemm_f64_workgroups<64, 64>(p, W); break;
+    case TROT_GOP_GREEN: p = problem_green(a, b, c, N, n); wg = gemm_f64_workgroups<64, 64>(p, W); break;
+    case TROT_GOP_VHS: p = problem_vhs(a, b, c, n, N, W); wg = gemm_f64_workgroups<64, 64>(p, 1); break;
+    default: return fail(TROT_ERR_INVALID, "gemm_problem: unknown operation");
+  }
+  *reinterpret_cast<GemmP*>(desc) = p;
+  return wg;
+}
+
+// probs: device array of nprob problems from trot_gemm_problem (all of operation op); start: device
+// prefix sums of their workgroup counts (nprob + 1 ints, start[0] = 0, start[nprob] = total_wg).
+int trot_gemm_f64_grouped(int op, const void* probs, const int* start, int nprob, int total_wg, void* stream) {
+  if (!probs || !start || nprob < 1 || total_wg < 1) return fail(TROT_ERR_INVALID, "gemm_f64_grouped: bad arguments");
+  const GemmP* P = (const GemmP*)probs;
+  hipStream_t s = (hipStream_t)stream;
+  switch (op) {
+    case TROT_GOP_ONE_BODY:
+      HIPCHK((launch_gemm_f64_grouped<64, 64, false, true, true, true, false, EPI_STORE>(P, start, nprob, total_wg, s)));
+      break;
+    case TROT_GOP_TAYLOR:
+      HIPCHK((launch_gemm_f64_grouped<32, 64, true, true, true, true, false, EPI_TAYLOR, true>(P, start, nprob, total_wg,
+                                                                                           s)));
+      break;
+    case TROT_GOP_TAYLOR_NARROW:
+      HIPCHK((launch_gemm_f64_grouped<32, 32, true, true, true, true, false, EPI_TAYLOR, true>(P, start, nprob, total_wg,
+                                                                                           s)));
+      break;
+    case TROT_GOP_OVERLAP:
+      HIPCHK((launch_gemm_f64_grouped<64, 64, false, true, false, true, false, EPI_STORE>(P, start, nprob, total_wg, s)));
+      break;
+    case TROT_GOP_GREEN:
+      HIPCHK((launch_gemm_f64_grouped<64, 64, true, true, true, true, false, EPI_STORE>(P, start, nprob, total_wg, s)));
+      break;
+    case TROT_GOP_VHS:
+      HIPCHK((launch_gemm_f64_grouped<64, 64, true, false, true, true, false, EPI_STORE>(P, start, nprob, total_wg, s)));
+      break;
+    default: return fail(TROT_ERR_INVALID, "gemm_f64_grouped: unknown operation");
+  }
+  return TROT_OK;
+}
+
+long trot_inv_problem(const double* m, double* minv, double* det, int W, int n, void* desc) {
+  if (!m || !minv || !det || !desc || W < 1 || n < 1) return fail(TROT_ERR_INVALID, "inv_problem: bad arguments");
+  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "inv_problem: nocc > 64");
+  *reinterpret_cast<InvProb*>(desc) = InvProb{m, minv, det, n};
+  return W;
+}
+
+int trot_inv_det_grouped(const void* probs, const int* start, int nprob, int total_w, void* stream) {
+  if (!probs || !start || nprob < 1 || total_w < 1) return fail(TROT_ERR_INVALID, "inv_det_grouped: bad arguments");
+  HIPCHK(launch_inv_det_grouped((const InvProb*)probs, start, nprob, total_w, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+int trot_copy_problem(const void* src, void* dst, long bytes, void* desc) {
+  if (!src || !dst || !desc || bytes < 0 || bytes % 16) return fail(TROT_ERR_INVALID, "copy_problem: bad arguments");
+  *reinterpret_cast<CopyProb*>(desc) = CopyProb{src, dst, bytes};
+  return TROT_OK;
+}
+
+int trot_copy_grouped(const void* probs, int nprob, long max_bytes, void* stream) {
+  if (!probs || nprob < 1 || max_bytes < 0) return fail(TROT_ERR_INVALID, "copy_grouped: bad arguments");
+  HIPCHK(launch_copy_grouped((const CopyProb*)probs, nprob, max_bytes, (hipStream_t)stream));
   return TROT_OK;
 }
 

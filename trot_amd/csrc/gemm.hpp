@@ -195,17 +195,15 @@ struct TileF64 {
   }
 };
 
-template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
+// One output tile (m0, n0) of batch b of problem p (nbatch batches): the whole work of a workgroup.
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM>
+__device__ __forceinline__ void gemm_f64_tile(const GemmP& p, int b, int nbatch, int m0, int n0,
+                                              typename TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM>::Smem& sm) {
   typedef TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM> T;
-  __shared__ typename T::Smem sm;
-  const int b = blockIdx.z;
-  const int m0 = blockIdx.y * BM;
-  const int n0 = blockIdx.x * BN;
   const double* A = p.A + (AC ? 2 : 1) * (long)b * p.sA;
   const double* B = p.B + (BC ? 2 : 1) * (long)b * p.sB;
   typename T::Acc acc;
-  const int K = (p.Klast > 0 && b == (int)gridDim.z - 1) ? p.Klast : p.K;
+  const int K = (p.Klast > 0 && b == nbatch - 1) ? p.Klast : p.K;
   T::run(A, p.lda, B, p.ldb, p.M, p.N, K, m0, n0, sm, acc);
 
   constexpr int PC = T::PC;
@@ -245,6 +243,47 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
           C[off] = v;
         }
       }
+}
+
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
+  __shared__ typename TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM>::Smem sm;
+  gemm_f64_tile<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM>(p, blockIdx.z, gridDim.z, blockIdx.y * BM,
+                                                            blockIdx.x * BN, sm);
+}
+
+// Many independent problems in one launch: problem i owns workgroups start[i] .. start[i+1] - 1, laid out
+// (batch, row tile, column tile) with the column tile fastest, as gemm_f64_kernel's grid.  Each workgroup
+// computes exactly what it would in a launch of its own problem.
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false>
+__global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const GemmP* __restrict__ probs,
+                                                               const int* __restrict__ start, int nprob) {
+  __shared__ typename TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM>::Smem sm;
+  const int g = blockIdx.x;
+  int lo = 0, hi = nprob;  // start[lo] <= g < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= g) lo = mid; else hi = mid;
+  }
+  const GemmP p = probs[lo];
+  const int local = g - start[lo];
+  const int ntn = (p.N + BN - 1) / BN, ntm = (p.M + BM - 1) / BM;
+  const int tn = local % ntn, tm = (local / ntn) % ntm, b = local / (ntn * ntm);
+  gemm_f64_tile<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM>(p, b, 1 << 30, tm * BM, tn * BN, sm);
+}
+
+template <int BM, int BN>
+inline int gemm_f64_workgroups(const GemmP& p, int batch) {
+  return ((p.N + BN - 1) / BN) * ((p.M + BM - 1) / BM) * batch;
+}
+
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false>
+static hipError_t launch_gemm_f64_grouped(const GemmP* probs, const int* start, int nprob, int total_wg,
+                                          hipStream_t s) {
+  if (nprob <= 0 || total_wg <= 0) return hipSuccess;
+  hipLaunchKernelGGL((gemm_f64_grouped_kernel<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM>), dim3(total_wg), dim3(256),
+                     0, s, probs, start, nprob);
+  return hipGetLastError();
 }
 
 template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false>

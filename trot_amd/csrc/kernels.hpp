@@ -6,6 +6,21 @@
 #define TROT_MAX_NOCC 64
 
 hipError_t launch_inv_det(const double* m, double* minv, double* det, int n, int W, hipStream_t s);
+
+// grouped launches over many walker sets (layouts shared with the C ABI, include/trot_amd.h)
+struct InvProb {
+  const double* m;
+  double* minv;
+  double* det;
+  long n;
+};
+struct CopyProb {
+  const void* src;
+  void* dst;
+  long bytes;
+};
+hipError_t launch_inv_det_grouped(const InvProb* probs, const int* start, int nprob, int total_w, hipStream_t s);
+hipError_t launch_copy_grouped(const CopyProb* probs, int nprob, long max_bytes, hipStream_t s);
 hipError_t launch_overlap_combine(const double* det_a, const double* det_b, double* ovlp, int W, hipStream_t s);
 hipError_t launch_shift(const double* fields, const double* fb, const double* mf, double sqrt_dt, int W, int G,
                         double* xbar, float* xbar32, double* shift, double* fbt, hipStream_t s);

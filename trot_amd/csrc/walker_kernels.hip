@@ -25,16 +25,23 @@
 // LDS (double-buffered by step parity).  Two barriers per step.  The arithmetic
 // per row is the row-swapping algorithm's; only the storage differs.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__ m, double* __restrict__ minv,
-                                                      double* __restrict__ det, int n) {
-  __shared__ cplx colk[2][64];  // column k of every (physical) row, before the step
-  __shared__ cplx rowp[2][64];  // the pivot row, every column
-  __shared__ int s_piv[2];
-  __shared__ int perm[64];      // position swapped with k at step k (LAPACK ipiv)
-  __shared__ int outcol[64];
+struct InvShared {
+  cplx colk[2][64];  // column k of every (physical) row, before the step
+  cplx rowp[2][64];  // the pivot row, every column
+  int s_piv[2];
+  int perm[64];      // position swapped with k at step k (LAPACK ipiv)
+  int outcol[64];
+};
+
+// One walker's n x n matrix at mw -> inverse at ow, determinant at detw (one complex).
+__device__ __forceinline__ void inv_det_one(const double* __restrict__ mw, double* __restrict__ ow,
+                                            double* __restrict__ detw, int n, InvShared& sh) {
+  cplx(&colk)[2][64] = sh.colk;
+  cplx(&rowp)[2][64] = sh.rowp;
+  int(&s_piv)[2] = sh.s_piv;
+  int(&perm)[64] = sh.perm;
+  int(&outcol)[64] = sh.outcol;
   const int t = threadIdx.x, lane = t & 63, wq = t >> 6;
-  const long w = blockIdx.x;
-  const double* mw = m + 2 * w * n * n;
   cplx a[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -117,13 +124,45 @@ __global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__
   __syncthreads();
   if (t < 64) colk[0][outcol[t]].re = (double)t;  // inverse map: here column j -> output column
   __syncthreads();
-  double* ow = minv + 2 * w * n * n;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int j = wq * 16 + r;
     if (lp < n && j < n) cstore(ow, (long)lp * n + (int)colk[0][j].re, a[r]);
   }
-  if (t == 0) cstore(det, w, dacc);
+  if (t == 0) cstore(detw, 0, dacc);
+}
+
+__global__ __launch_bounds__(256) void inv_det_kernel(const double* __restrict__ m, double* __restrict__ minv,
+                                                      double* __restrict__ det, int n) {
+  __shared__ InvShared sh;
+  const long w = blockIdx.x;
+  inv_det_one(m + 2 * w * n * n, minv + 2 * w * n * n, det + 2 * w, n, sh);
+}
+
+// Many walker sets in one launch: problem i owns workgroups start[i] .. start[i+1] - 1, one per walker.
+__global__ __launch_bounds__(256) void inv_det_grouped_kernel(const InvProb* __restrict__ probs,
+                                                              const int* __restrict__ start, int nprob) {
+  __shared__ InvShared sh;
+  const int g = blockIdx.x;
+  int lo = 0, hi = nprob;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= g) lo = mid; else hi = mid;
+  }
+  const InvProb q = probs[lo];
+  const long w = g - start[lo];
+  inv_det_one(q.m + 2 * w * q.n * q.n, q.minv + 2 * w * q.n * q.n, q.det + 2 * w, q.n, sh);
+}
+
+// Many copies in one launch: grid (chunks of 64 KiB, problems); sizes in bytes, multiples of 16.
+__global__ __launch_bounds__(256) void copy_grouped_kernel(const CopyProb* __restrict__ probs) {
+  const CopyProb q = probs[blockIdx.y];
+  const long n16 = q.bytes / 16;
+  const long beg = (long)blockIdx.x * 4096;
+  const long end = beg + 4096 < n16 ? beg + 4096 : n16;
+  const double2* src = reinterpret_cast<const double2*>(q.src);
+  double2* dst = reinterpret_cast<double2*>(q.dst);
+  for (long e = beg + threadIdx.x; e < end; e += 256) dst[e] = src[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -355,6 +394,20 @@ __global__ void sum_batches_kernel(const double* __restrict__ src, double* __res
 static hipError_t set_lds(const void* f, size_t bytes) {
   if (bytes <= 65536) return hipSuccess;
   return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_inv_det_grouped(const InvProb* probs, const int* start, int nprob, int total_w, hipStream_t s) {
+  if (nprob <= 0 || total_w <= 0) return hipSuccess;
+  hipLaunchKernelGGL(inv_det_grouped_kernel, dim3(total_w), dim3(256), 0, s, probs, start, nprob);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_grouped(const CopyProb* probs, int nprob, long max_bytes, hipStream_t s) {
+  if (nprob <= 0 || max_bytes <= 0) return hipSuccess;
+  const long chunks = (max_bytes / 16 + 4095) / 4096;
+  if (nprob > 65535 || chunks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(copy_grouped_kernel, dim3((unsigned)chunks, (unsigned)nprob), dim3(256), 0, s, probs);
+  return hipGetLastError();
 }
 
 hipError_t launch_inv_det(const double* m, double* minv, double* det, int n, int W, hipStream_t s) {

@@ -358,6 +358,42 @@ size_t trot_copy_problem_bytes(void);
 int trot_copy_problem(const void* src, void* dst, long bytes, void* desc);
 int trot_copy_grouped(const void* probs, int nprob, long max_bytes, void* stream);
 
+/* ---- CISD trial, restricted walkers (FP64) ----
+ * The reference determinant occupies the first nocc orbitals; the active block is occupied orbitals
+ * core .. nocc-1 (na = nocc - core) and virtuals nocc .. nocc+nvir-1; ci1 (na, nvir) and ci2 (na, nvir,
+ * na, nvir) are real.  GT = w w[:nocc]^-1 and det = det(w[:nocc]) come from trot_overlap_green with the
+ * trial I[:, :nocc].  Cholesky vectors and h1 must be symmetric (real orbitals).
+ *
+ * trot_cisd_doubles_matrix: D[(p,t),(q,u)] = 2 ci2[p,t,q,u] - ci2[p,u,q,t], (na nvir)^2 doubles: both
+ *   doubles contractions of trot/trial/cisd.py:99-145 and trot/meas/cisd.py:179-198 in one matrix.
+ * trot_pack_sym: upper triangles of `count` symmetric N x N matrices, row-major, N (N+1) / 2 each
+ *   (the K index of the CISD force bias).
+ * trot_cisd_overlap: Omega = 1 + 2 <ci1, x> + x^T D x, overlap = det^2 Omega (ovl may be NULL), and
+ *   Bs = (ci1 + x D) / Omega (W, na, nvir).  Replaces trot/trial/cisd.py:148-166 (overlap_r).
+ * trot_cisd_ps: Ps = greenp Bs^T (W, N, na), greenp of trot/meas/cisd.py:24-44.
+ * trot_cisd_gamma: gamma = G0 - Ps G_act (W, N, N): the mixed 1-RDM of trot/meas/cisd.py:772-820
+ *   (rdm1_kernel_rw, one spin); packed != 0 writes gamma[i,j] + gamma[j,i] (i < j) and gamma[i,i] in the
+ *   trot_pack_sym order instead, so that the force bias of trot/meas/cisd.py:548-564
+ *   (force_bias_kernel_rw_rh_high) is fb = 2 gamma_packed L_packed^T (trot_force_bias_splitk with
+ *   Nn = N (N+1) / 2).
+ * trot_cisd_energy: local energies (W complex) of trot/meas/cisd.py:608-769 (energy_kernel_rw_rh);
+ *   lx = the G Cholesky vectors followed by h1, (G+1, N, N); walkers in chunks of wc, the workspace
+ *   from trot_cisd_energy_workspace(wc, ...). */
+int trot_cisd_doubles_matrix(const double* ci2, int na, int nvir, double* D, void* stream);
+int trot_pack_sym(const double* m, int count, int N, double* out, void* stream);
+size_t trot_cisd_overlap_workspace(int W, int na, int nvir);
+int trot_cisd_overlap(const double* green_t, const double* det, const double* ci1, const double* D, int W, int N,
+                      int nocc, int core, int na, int nvir, double* omega, double* ovl, double* bs, void* ws,
+                      size_t ws_bytes, void* stream);
+int trot_cisd_ps(const double* green_t, const double* bs, int W, int N, int nocc, int core, int na, int nvir,
+                 double* ps, void* stream);
+int trot_cisd_gamma(const double* green_t, const double* ps, int W, int N, int nocc, int core, int na, int packed,
+                    double* out, void* stream);
+size_t trot_cisd_energy_workspace(int wc, int N, int nocc, int G, int na, int nvir);
+int trot_cisd_energy(const double* green_t, const double* bs, const double* ps, const double* omega,
+                     const double* lx, const double* D, int W, int N, int nocc, int G, int core, int na, int nvir,
+                     double h0, int wc, double* e, void* ws, size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,15 @@ _SIGS = {
     "trot_copy_problem_bytes": (_sz, []),
     "trot_copy_problem": (_i, [_vp, _vp, _l, _vp]),
     "trot_copy_grouped": (_i, [_vp, _i, _l, _vp]),
+    "trot_cisd_doubles_matrix": (_i, [_vp, _i, _i, _vp, _vp]),
+    "trot_pack_sym": (_i, [_vp, _i, _i, _vp, _vp]),
+    "trot_cisd_overlap_workspace": (_sz, [_i, _i, _i]),
+    "trot_cisd_overlap": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "trot_cisd_ps": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp]),
+    "trot_cisd_gamma": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp]),
+    "trot_cisd_energy_workspace": (_sz, [_i, _i, _i, _i, _i, _i]),
+    "trot_cisd_energy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _d, _i, _vp, _vp, _sz,
+                              _vp]),
     "trot_trotter_b3v": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
     "trot_philox_raw": (_i, [_u64, _u64, _u64, _l, _vp, _vp]),
     "trot_philox_normal": (_i, [_vp, _u64, _u64, _l, _vp, _vp]),
@@ -225,6 +234,106 @@ def force_bias(green_t: torch.Tensor, rot_chol_t: torch.Tensor, alpha: float, ou
                                     _p(out), _p(ws), 0 if ws is None else ws.numel(), _stream()),
            "trot_force_bias_splitk")
     return out
+
+
+def force_bias_flat(a: torch.Tensor, r: torch.Tensor, alpha: float) -> torch.Tensor:
+    """fb[w, g] = alpha sum_k r[g, k] a[w, k]: a (W, K) c128, r (G, K) f64 (split K for small W)."""
+    W, K = a.shape
+    G = r.shape[0]
+    _req(a, "a", C128, (W, K))
+    _req(r, "r", F64, (G, K))
+    out = torch.empty((W, G), dtype=C128, device=a.device)
+    L = lib()
+    nbytes = L.trot_force_bias_workspace(W, G, K)
+    ws = _ws(nbytes, a.device) if nbytes else None
+    _check(L.trot_force_bias_splitk(_p(a), _p(r), W, G, K, float(alpha), 0, _p(out), _p(ws),
+                                    0 if ws is None else ws.numel(), _stream()), "trot_force_bias_splitk")
+    return out
+
+
+def cisd_doubles_matrix(ci2: torch.Tensor) -> torch.Tensor:
+    """D[(p,t),(q,u)] = 2 ci2[p,t,q,u] - ci2[p,u,q,t] as an (na nvir, na nvir) f64 matrix."""
+    na, va = int(ci2.shape[0]), int(ci2.shape[1])
+    _req(ci2, "ci2", F64, (na, va, na, va))
+    out = torch.empty((na * va, na * va), dtype=F64, device=ci2.device)
+    _check(lib().trot_cisd_doubles_matrix(_p(ci2), na, va, _p(out), _stream()), "trot_cisd_doubles_matrix")
+    return out
+
+
+def pack_sym(m: torch.Tensor) -> torch.Tensor:
+    """Upper triangles (row-major) of (count, N, N) symmetric f64 matrices -> (count, N (N+1) / 2)."""
+    cnt, N, _ = m.shape
+    _req(m, "m", F64, (cnt, N, N))
+    out = torch.empty((cnt, N * (N + 1) // 2), dtype=F64, device=m.device)
+    _check(lib().trot_pack_sym(_p(m), cnt, N, _p(out), _stream()), "trot_pack_sym")
+    return out
+
+
+def cisd_overlap(gt: torch.Tensor, det: torch.Tensor | None, ci1: torch.Tensor, dmat: torch.Tensor, core: int):
+    """(omega, overlap or None, Bs (W, na, nvir)) of the CISD trial from GT = w w[:n]^-1 and det(w[:n])."""
+    W, N, n = gt.shape
+    na, va = int(ci1.shape[0]), int(ci1.shape[1])
+    _req(gt, "green_t", C128, (W, N, n))
+    _req(ci1, "ci1", F64, (na, va))
+    _req(dmat, "D", F64, (na * va, na * va))
+    if det is not None:
+        _req(det, "det", C128, (W,))
+    omega = torch.empty((W,), dtype=C128, device=gt.device)
+    ovl = torch.empty((W,), dtype=C128, device=gt.device) if det is not None else None
+    bs = torch.empty((W, na, va), dtype=C128, device=gt.device)
+    L = lib()
+    ws = _ws(L.trot_cisd_overlap_workspace(W, na, va), gt.device)
+    _check(L.trot_cisd_overlap(_p(gt), _p(det), _p(ci1), _p(dmat), W, N, n, core, na, va, _p(omega), _p(ovl), _p(bs),
+                               _p(ws), ws.numel(), _stream()), "trot_cisd_overlap")
+    return omega, ovl, bs
+
+
+def cisd_ps(gt: torch.Tensor, bs: torch.Tensor, core: int) -> torch.Tensor:
+    W, N, n = gt.shape
+    na, va = int(bs.shape[1]), int(bs.shape[2])
+    _req(gt, "green_t", C128, (W, N, n))
+    _req(bs, "bs", C128, (W, na, va))
+    ps = torch.empty((W, N, na), dtype=C128, device=gt.device)
+    _check(lib().trot_cisd_ps(_p(gt), _p(bs), W, N, n, core, na, va, _p(ps), _stream()), "trot_cisd_ps")
+    return ps
+
+
+def cisd_gamma(gt: torch.Tensor, ps: torch.Tensor, core: int, packed: bool) -> torch.Tensor:
+    """Mixed 1-RDM (W, N, N), or its packed symmetric sums (W, N (N+1) / 2)."""
+    W, N, n = gt.shape
+    na = int(ps.shape[2])
+    _req(gt, "green_t", C128, (W, N, n))
+    _req(ps, "ps", C128, (W, N, na))
+    shape = (W, N * (N + 1) // 2) if packed else (W, N, N)
+    out = torch.empty(shape, dtype=C128, device=gt.device)
+    _check(lib().trot_cisd_gamma(_p(gt), _p(ps), W, N, n, core, na, int(packed), _p(out), _stream()),
+           "trot_cisd_gamma")
+    return out
+
+
+CISD_ENERGY_WS_BYTES = 1 << 31  # walker chunks sized to keep the energy workspace near 2 GiB
+
+
+def cisd_energy(gt, bs, ps, omega, lx, dmat, core: int, h0: float, wc: int | None = None) -> torch.Tensor:
+    W, N, n = gt.shape
+    na, va = int(bs.shape[1]), int(bs.shape[2])
+    G = int(lx.shape[0]) - 1
+    _req(gt, "green_t", C128, (W, N, n))
+    _req(bs, "bs", C128, (W, na, va))
+    _req(ps, "ps", C128, (W, N, na))
+    _req(omega, "omega", C128, (W,))
+    _req(lx, "lx", F64, (G + 1, N, N))
+    _req(dmat, "D", F64, (na * va, na * va))
+    L = lib()
+    if wc is None:
+        per = L.trot_cisd_energy_workspace(1, N, n, G, na, va)
+        wc = max(1, min(W, CISD_ENERGY_WS_BYTES // max(per, 1)))
+    wc = int(min(wc, W))
+    e = torch.empty((W,), dtype=C128, device=gt.device)
+    ws = _ws(L.trot_cisd_energy_workspace(wc, N, n, G, na, va), gt.device)
+    _check(L.trot_cisd_energy(_p(gt), _p(bs), _p(ps), _p(omega), _p(lx), _p(dmat), W, N, n, G, core, na, va,
+                              float(h0), wc, _p(e), _p(ws), ws.numel(), _stream()), "trot_cisd_energy")
+    return e
 
 
 def field_shift(fields: torch.Tensor, fb: torch.Tensor, mf: torch.Tensor, sqrt_dt: float, out=None):

@@ -91,7 +91,7 @@ def _groupable(f: Fragment) -> bool:
     ctx = f.prop_ctx
     eng = f.meas_ops.engine
     return (ctx.precision == "f64" and ctx.taylor_precision == "f64" and bool(ctx.low.get("symblk"))
-            and 0 >= ctx.walker_chunk and eng.precision == "f64"
+            and 0 >= ctx.walker_chunk and eng.precision == "f64" and getattr(eng, "kind", "") in ("rr", "ru", "uu")
             and f.sys.walker_kind in ("restricted", "unrestricted") and max(f.sys.nelec) <= _lib.MAX_NOCC)
 
 

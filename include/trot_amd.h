@@ -378,7 +378,9 @@ int trot_copy_grouped(const void* probs, int nprob, long max_bytes, void* stream
  *   Nn = N (N+1) / 2).
  * trot_cisd_energy: local energies (W complex) of trot/meas/cisd.py:608-769 (energy_kernel_rw_rh);
  *   lx = the G Cholesky vectors followed by h1, (G+1, N, N); walkers in chunks of wc, the workspace
- *   from trot_cisd_energy_workspace(wc, ...). */
+ *   from trot_cisd_energy_workspace(wc, ..., b3).  d_planes NULL: the doubles term x_g^T D x_g in FP64;
+ *   else split-BF16 planes of D^T (trot_split_bf16, transposed, nvir na rows and columns) and that
+ *   term on split-BF16 MFMA (the reference computes it in float32 unless testing=True, :745-768). */
 int trot_cisd_doubles_matrix(const double* ci2, int na, int nvir, double* D, void* stream);
 int trot_pack_sym(const double* m, int count, int N, double* out, void* stream);
 size_t trot_cisd_overlap_workspace(int W, int na, int nvir);
@@ -389,10 +391,10 @@ int trot_cisd_ps(const double* green_t, const double* bs, int W, int N, int nocc
                  double* ps, void* stream);
 int trot_cisd_gamma(const double* green_t, const double* ps, int W, int N, int nocc, int core, int na, int packed,
                     double* out, void* stream);
-size_t trot_cisd_energy_workspace(int wc, int N, int nocc, int G, int na, int nvir);
+size_t trot_cisd_energy_workspace(int wc, int N, int nocc, int G, int na, int nvir, int b3);
 int trot_cisd_energy(const double* green_t, const double* bs, const double* ps, const double* omega,
-                     const double* lx, const double* D, int W, int N, int nocc, int G, int core, int na, int nvir,
-                     double h0, int wc, double* e, void* ws, size_t ws_bytes, void* stream);
+                     const double* lx, const double* D, const void* d_planes, int W, int N, int nocc, int G, int core,
+                     int na, int nvir, double h0, int wc, double* e, void* ws, size_t ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

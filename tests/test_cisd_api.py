@@ -29,6 +29,7 @@ def test_meas_ops_config():
     assert ops.has_observable("rdm1")
     ops64 = cisd.make_cisd_meas_ops(System(10, (3, 3), "restricted"), mixed_precision=False, testing=True)
     assert cisd.get_cisd_meas_cfg(ops64).force_bias_precision == "f64" and ops64.engine.precision == "f64"
+    assert ops.engine.doubles_precision == "bf16x3" and ops64.engine.doubles_precision == "f64"
     from trot_amd import meas
 
     assert cisd.get_cisd_meas_cfg(meas.make_rhf_meas_ops(System(10, (3, 3), "restricted"))) is None

@@ -2,8 +2,8 @@
 
 FP64 bar 1e-10 relative for the overlap, force bias, local energy and mixed 1-RDM, with and without
 frozen core / outer orbitals, with the energy's walker chunks smaller than the batch; the split-BF16
-force bias (mixed_precision=True, the reference's default) within the 2e-5 the reference allows its
-float32 variant (tests/test_cisd.py:245-287); one propagation step pair and one block (energy, QR,
+force bias and doubles term (mixed_precision=True, testing=False: the reference's defaults) within the
+tolerances the reference allows its float32 variants (tests/test_cisd.py:245-287, :426); one propagation step pair and one block (energy, QR,
 SR) through the generic step and block driver against the oracle's.
 """
 
@@ -39,7 +39,7 @@ def _case(N, nocc, core, outer, G, W, seed, s1=0.1, s2=0.05, mix=0.2):
     return dict(h0=0.37, h1=h1, chol=chol, trial=t, w=w, rng=rng)
 
 
-def _gpu(c, mixed=False):
+def _gpu(c, mixed=False, testing=None):
     from trot_amd import cisd
     from trot_amd.core import HamChol, System
 
@@ -49,7 +49,7 @@ def _gpu(c, mixed=False):
     ham = HamChol(c["h0"], c["h1"], c["chol"]).to_device()
     tr = cisd.make_cisd_trial_data(dict(ci1=t.ci1, ci2=t.ci2, nocc_t_core=t.nocc_t_core,
                                         nvir_t_outer=t.nvir_t_outer), sys)
-    mops = cisd.make_cisd_meas_ops(sys, mixed_precision=mixed)
+    mops = cisd.make_cisd_meas_ops(sys, mixed_precision=mixed, testing=(not mixed) if testing is None else testing)
     tops = cisd.make_cisd_trial_ops(sys)
     return sys, ham, tr, mops, tops, mops.build_meas_ctx(ham, tr)
 
@@ -86,15 +86,24 @@ def test_measurement_matches_oracle(gpu, shape):
     assert rel(e7.cpu().numpy(), e_ref) < TOL
 
 
-def test_mixed_precision_force_bias(gpu):
-    from trot_amd.core import k_force_bias
+def test_mixed_precision_force_bias_and_energy(gpu):
+    """The reference's defaults (mixed_precision=True, testing=False): split-BF16 force bias within the 2e-5
+    its float32 variant is held to; the energy with the doubles term on split-BF16 within the rtol 2e-5 /
+    atol 2e-6 the reference holds its default (float32) energy to."""
+    from trot_amd.core import k_energy, k_force_bias
 
     c = _case(30, 6, 1, 2, 40, 24, seed=77)
     sys, ham, tr, mops, tops, mctx = _gpu(c, mixed=True)
-    fb = mops.require_kernel(k_force_bias)(torch.from_numpy(c["w"]).cuda(), ham, mctx, tr).cpu().numpy()
-    ref = C.force_bias(c["w"], c["chol"], C.build_meas_ctx(c["chol"], c["trial"]), c["trial"])
+    assert mops.engine.precision == "bf16x3" and mops.engine.doubles_precision == "bf16x3"
+    wd = torch.from_numpy(c["w"]).cuda()
+    ctx = C.build_meas_ctx(c["chol"], c["trial"])
+    fb = mops.require_kernel(k_force_bias)(wd, ham, mctx, tr).cpu().numpy()
+    ref = C.force_bias(c["w"], c["chol"], ctx, c["trial"])
     assert np.allclose(fb, ref, rtol=2e-5, atol=2e-5)
     assert rel(fb, ref) > 0  # the split-BF16 path ran, not the FP64 one
+    e = mops.require_kernel(k_energy)(wd, ham, mctx, tr).cpu().numpy()
+    e_ref = C.energy(c["w"], c["h0"], c["h1"], c["chol"], ctx, c["trial"])
+    assert rel(e, e_ref) > 0 and np.allclose(e, e_ref, rtol=2e-5, atol=2e-6)  # tests/test_cisd.py:426
 
 
 def _prop(c, sys, ham, tr, mops, tops, dt=0.01, n_prop_steps=3):

@@ -85,9 +85,9 @@ _SIGS = {
     "trot_cisd_overlap": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
     "trot_cisd_ps": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp]),
     "trot_cisd_gamma": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp]),
-    "trot_cisd_energy_workspace": (_sz, [_i, _i, _i, _i, _i, _i]),
-    "trot_cisd_energy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _d, _i, _vp, _vp, _sz,
-                              _vp]),
+    "trot_cisd_energy_workspace": (_sz, [_i, _i, _i, _i, _i, _i, _i]),
+    "trot_cisd_energy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _d, _i, _vp, _vp,
+                              _sz, _vp]),
     "trot_trotter_b3v": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _vp]),
     "trot_philox_raw": (_i, [_u64, _u64, _u64, _l, _vp, _vp]),
     "trot_philox_normal": (_i, [_vp, _u64, _u64, _l, _vp, _vp]),
@@ -314,7 +314,10 @@ def cisd_gamma(gt: torch.Tensor, ps: torch.Tensor, core: int, packed: bool) -> t
 CISD_ENERGY_WS_BYTES = 1 << 31  # walker chunks sized to keep the energy workspace near 2 GiB
 
 
-def cisd_energy(gt, bs, ps, omega, lx, dmat, core: int, h0: float, wc: int | None = None) -> torch.Tensor:
+def cisd_energy(gt, bs, ps, omega, lx, dmat, core: int, h0: float, wc: int | None = None,
+                d_planes=None) -> torch.Tensor:
+    """Local energies; ``d_planes`` (split-BF16 planes of D^T, trot_amd._b3.Planes) puts the doubles term on
+    split-BF16 MFMA, else FP64."""
     W, N, n = gt.shape
     na, va = int(bs.shape[1]), int(bs.shape[2])
     G = int(lx.shape[0]) - 1
@@ -325,14 +328,18 @@ def cisd_energy(gt, bs, ps, omega, lx, dmat, core: int, h0: float, wc: int | Non
     _req(lx, "lx", F64, (G + 1, N, N))
     _req(dmat, "D", F64, (na * va, na * va))
     L = lib()
+    b3 = int(d_planes is not None)
+    if b3 and (d_planes.rows != na * va or d_planes.K != na * va or d_planes.complex):
+        raise ValueError("d_planes: split-BF16 planes of the real (na nvir)^2 matrix D^T expected")
     if wc is None:
-        per = L.trot_cisd_energy_workspace(1, N, n, G, na, va)
+        per = L.trot_cisd_energy_workspace(1, N, n, G, na, va, b3)
         wc = max(1, min(W, CISD_ENERGY_WS_BYTES // max(per, 1)))
     wc = int(min(wc, W))
     e = torch.empty((W,), dtype=C128, device=gt.device)
-    ws = _ws(L.trot_cisd_energy_workspace(wc, N, n, G, na, va), gt.device)
-    _check(L.trot_cisd_energy(_p(gt), _p(bs), _p(ps), _p(omega), _p(lx), _p(dmat), W, N, n, G, core, na, va,
-                              float(h0), wc, _p(e), _p(ws), ws.numel(), _stream()), "trot_cisd_energy")
+    ws = _ws(L.trot_cisd_energy_workspace(wc, N, n, G, na, va, b3), gt.device)
+    _check(L.trot_cisd_energy(_p(gt), _p(bs), _p(ps), _p(omega), _p(lx), _p(dmat),
+                              _p(d_planes.data) if b3 else None, W, N, n, G, core, na, va, float(h0), wc, _p(e),
+                              _p(ws), ws.numel(), _stream()), "trot_cisd_energy")
     return e
 
 

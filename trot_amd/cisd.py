@@ -7,11 +7,16 @@ nvir_act) and ``ci2`` (nocc_act, nvir_act, nocc_act, nvir_act) refer to the acti
 trial I[:, :nocc_full]; both doubles contractions of the reference are one FP64 GEMM with the matrix
 D[(p,t),(q,u)] = 2 ci2[p,t,q,u] - ci2[p,u,q,t], built once per trial.
 
-Precision (``make_cisd_meas_ops``): ``mixed_precision=True`` (the reference's default, whose
-Cholesky contractions then run in float32) computes the force-bias GEMM on split-BF16 MFMA (about
-FP32 accuracy, FP64 accumulation); ``mixed_precision=False`` keeps it FP64.  Overlaps, Green's
-functions and local energies are FP64 in both, which is at least the reference's precision
-(its "testing" dtypes cover only the energy's doubles term).
+Precision (``make_cisd_meas_ops``), mirroring the reference's two switches:
+
+* ``mixed_precision=True`` (the reference's default, whose Cholesky contractions then run in float32)
+  computes the force-bias GEMM on split-BF16 MFMA (16-bit significands per operand, about 1e-5
+  relative, FP32 accumulation); ``mixed_precision=False`` keeps it FP64;
+* ``testing=False`` (the reference's default, whose doubles term of the local energy then runs in
+  float32, trot/meas/cisd.py:745-768) computes that term, sum_g x_g^T D x_g, on split-BF16 MFMA;
+  ``testing=True`` keeps it FP64.
+
+Overlaps, Green's functions and every other term of the local energy are FP64 in all modes.
 
 Amplitudes, h1 and the Cholesky vectors must be real and the latter two symmetric (real orbitals, as
 every Hamiltonian the reference builds).  Complex amplitudes raise ValueError.
@@ -195,6 +200,10 @@ class CisdMeasCfg:
     def force_bias_precision(self) -> str:
         return "f64" if self.mixed_real_dtype == torch.float64 else "bf16x3"
 
+    @property
+    def doubles_precision(self) -> str:
+        return "f64" if self.mixed_real_dtype_testing == torch.float64 else "bf16x3"
+
 
 def get_cisd_meas_cfg(meas_ops: MeasOps) -> CisdMeasCfg | None:
     """trot/meas/cisd.py:56-60."""
@@ -259,10 +268,11 @@ class CisdEngine:
 
     kind = "cisd"
 
-    def __init__(self, precision: str = "f64"):
-        if precision not in ("f64", "bf16x3"):
-            raise ValueError(f"unknown precision: {precision}")
+    def __init__(self, precision: str = "f64", doubles_precision: str = "f64"):
+        if precision not in ("f64", "bf16x3") or doubles_precision not in ("f64", "bf16x3"):
+            raise ValueError(f"unknown precision: {precision}, {doubles_precision}")
         self.precision = precision
+        self.doubles_precision = doubles_precision
 
     def overlap_green(self, walkers, trial_data: CisdTrial):
         if isinstance(walkers, tuple):
@@ -289,7 +299,13 @@ class CisdEngine:
     def energy_from_green(self, greens, h0: float, meas_ctx: CisdMeasCtx) -> torch.Tensor:
         gt, bs, omega = greens
         t = meas_ctx.trial
-        return _lib.cisd_energy(gt, bs, self._ps(greens, meas_ctx), omega, meas_ctx.lx, t.dmat(), t.nocc_t_core, h0)
+        planes = None
+        if self.doubles_precision == "bf16x3":
+            if "dmat_b3" not in t.cache:
+                t.cache["dmat_b3"] = _b3.split(t.dmat(), transposed=True)  # D^T: rows j, K index k
+            planes = t.cache["dmat_b3"]
+        return _lib.cisd_energy(gt, bs, self._ps(greens, meas_ctx), omega, meas_ctx.lx, t.dmat(), t.nocc_t_core, h0,
+                                d_planes=planes)
 
     def rdm1_from_green(self, greens, meas_ctx: CisdMeasCtx) -> torch.Tensor:
         gam = _lib.cisd_gamma(greens[0], self._ps(greens, meas_ctx), meas_ctx.trial.nocc_t_core, packed=False)
@@ -344,7 +360,7 @@ def make_cisd_meas_ops(sys: System, memory_mode: str = "high", mixed_precision: 
         mixed_real_dtype_testing=torch.float64 if testing else torch.float32,
         mixed_complex_dtype_testing=torch.complex128 if testing else torch.complex64,
     )
-    engine = CisdEngine(cfg.force_bias_precision)
+    engine = CisdEngine(cfg.force_bias_precision, cfg.doubles_precision)
     fb, en, dm = _kernels(engine)
     ops = MeasOps(
         overlap=overlap_r,

@@ -24,8 +24,8 @@
 //   pack_sym_kernel       upper triangles of symmetric matrices, row-major (the force-bias K index)
 //   cisd_gather_kernel    x from GT
 //   cisd_overlap_kernel   c1, c2, Omega, overlap, Bs (one workgroup per walker)
-//   cisd_ps_kernel        Ps
-//   cisd_gamma_kernel     gamma, full or as packed symmetric sums gamma[i,j] + gamma[j,i]
+//   cisd_ps_top_kernel    Ps rows < nocc; cisd_ps_rest_kernel the rest
+//   cisd_gamma_tile_kernel gamma, full or as packed symmetric sums gamma[i,j] + gamma[j,i]
 //   cisd_permute_kernel   GT of a walker chunk side by side (N x Wc n) for one R GEMM
 //   cisd_terms_kernel     per (walker, g): tr R_g, tr(R_g R_g), <glgp_g, H_g>, <Ps, R_g[:,act]>, x_g
 //   cisd_rowdot_kernel    x_g^T (x_g D)
@@ -109,51 +109,121 @@ __global__ __launch_bounds__(256) void cisd_overlap_kernel(const double* __restr
   }
 }
 
-__global__ __launch_bounds__(256) void cisd_ps_kernel(const double* __restrict__ gt, const double* __restrict__ bs,
-                                                     int N, int n, int na, int va, double* __restrict__ ps) {
-  const long w = blockIdx.y;
-  const long e = (long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (long)N * na) return;
-  const int i = (int)(e / na), a = (int)(e - (long)i * na);
+// Ps rows i < n: Ps[i,a] = sum_t GT[n+t, i] Bs[a,t].  One workgroup per walker; t in chunks of PS_TT
+// staged in LDS (GT rows n+t: PS_TT x n; Bs columns: na x PS_TT); (i, a) pairs in registers (n na <= 4096).
+constexpr int PS_TT = 32;
+__global__ __launch_bounds__(256) void cisd_ps_top_kernel(const double* __restrict__ gt, const double* __restrict__ bs,
+                                                         int N, int n, int na, int va, double* __restrict__ ps) {
+  extern __shared__ double sh[];
+  double* gv = sh;                                // [PS_TT][n + 1] complex
+  double* bc = sh + 2 * PS_TT * (n + 1);          // [na][PS_TT + 1] complex
+  const int tid = threadIdx.x;
+  const long w = blockIdx.x;
   const long ov = (long)na * va;
   const double* g = gt + 2 * w * N * n;
-  const double* b = bs + 2 * (w * ov + (long)a * va);
-  cplx acc = {0.0, 0.0};
-  if (i < n) {
-    for (int t = 0; t < va; ++t) acc = cadd(acc, cmul(cload(g, (long)(n + t) * n + i), cload(b, t)));
-  } else if (i < n + va) {
-    const cplx v = cload(b, i - n);
-    acc = {-v.re, -v.im};
+  const double* b = bs + 2 * w * ov;
+  const int npair = n * na;
+  cplx acc[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = {0.0, 0.0};
+  for (int t0 = 0; t0 < va; t0 += PS_TT) {
+    const int tt = va - t0 < PS_TT ? va - t0 : PS_TT;
+    for (int e = tid; e < tt * n; e += 256) {
+      const int t = e / n, i = e - t * n;
+      cstore(gv, (long)t * (n + 1) + i, cload(g, (long)(n + t0 + t) * n + i));
+    }
+    for (int e = tid; e < na * tt; e += 256) {
+      const int a = e / tt, t = e - a * tt;
+      cstore(bc, (long)a * (PS_TT + 1) + t, cload(b, (long)a * va + t0 + t));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int e = tid + k * 256;
+      if (e < npair) {
+        const int i = e / na, a = e - i * na;
+        cplx s = acc[k];
+        for (int t = 0; t < tt; ++t)
+          s = cadd(s, cmul(cload(gv, (long)t * (n + 1) + i), cload(bc, (long)a * (PS_TT + 1) + t)));
+        acc[k] = s;
+      }
+    }
+    __syncthreads();
   }
-  cstore(ps, w * N * na + e, acc);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int e = tid + k * 256;
+    if (e < npair) cstore(ps, w * N * na + e, acc[k]);
+  }
 }
 
-__device__ __forceinline__ cplx cisd_gamma_at(const double* g, const double* p, int i, int j, int n, int core,
-                                              int na) {
-  cplx acc = j < n ? cload(g, (long)i * n + j) : cplx{0.0, 0.0};
-  for (int a = 0; a < na; ++a) acc = csub(acc, cmul(cload(p, (long)i * na + a), cload(g, (long)j * n + core + a)));
-  return acc;
-}
-
-// packed: out[w, idx(i,j)] = gamma[i,j] + gamma[j,i] (i < j), gamma[i,i]; else out[w, i, j] = gamma[i,j].
-__global__ __launch_bounds__(256) void cisd_gamma_kernel(const double* __restrict__ gt, const double* __restrict__ ps,
-                                                        int N, int n, int core, int na, int packed,
-                                                        double* __restrict__ out) {
+// Ps rows i >= n: -Bs[a, i - n] for i < n + va, zero beyond.
+__global__ __launch_bounds__(256) void cisd_ps_rest_kernel(const double* __restrict__ bs, int N, int n, int na, int va,
+                                                          double* __restrict__ ps) {
   const long w = blockIdx.y;
   const long e = (long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (long)N * N) return;
-  const int i = (int)(e / N), j = (int)(e - (long)i * N);
+  if (e >= (long)(N - n) * na) return;
+  const int i = n + (int)(e / na), a = (int)(e % na);
+  cplx v = {0.0, 0.0};
+  if (i < n + va) {
+    const cplx x = cload(bs, (w * na + a) * va + (i - n));
+    v = {-x.re, -x.im};
+  }
+  cstore(ps, w * N * na + (long)n * na + e, v);
+}
+
+// gamma = G0 - Ps G_act on (TS x TS) tiles (bi <= bj) of one walker: the tile and its mirror from four
+// LDS panels (Ps rows and GT active columns of both row blocks).  packed: gamma[i,j] + gamma[j,i] (i < j) and
+// gamma[i,i] at the trot_pack_sym index; else both gamma[i,j] and gamma[j,i] into the full matrix.
+template <int TS>
+__global__ __launch_bounds__(256) void cisd_gamma_tile_kernel(const double* __restrict__ gt,
+                                                             const double* __restrict__ ps, int N, int n, int core,
+                                                             int na, int nt, int packed, double* __restrict__ out) {
+  extern __shared__ double sh[];
+  const int ld = na + 1;
+  double* Pi = sh;
+  double* Pj = Pi + 2 * TS * ld;
+  double* Ai = Pj + 2 * TS * ld;
+  double* Aj = Ai + 2 * TS * ld;
+  const int tid = threadIdx.x;
+  const long w = blockIdx.y;
+  int bi = 0, rem = blockIdx.x;
+  while (rem >= nt - bi) {
+    rem -= nt - bi;
+    ++bi;
+  }
+  const int bj = bi + rem;
   const double* g = gt + 2 * w * N * n;
   const double* p = ps + 2 * w * N * na;
-  if (!packed) {
-    cstore(out, w * N * N + e, cisd_gamma_at(g, p, i, j, n, core, na));
-    return;
+  for (int e = tid; e < TS * na; e += 256) {
+    const int r = e / na, a = e - r * na;
+    const int i = bi * TS + r, j = bj * TS + r;
+    const cplx z = {0.0, 0.0};
+    cstore(Pi, (long)r * ld + a, i < N ? cload(p, (long)i * na + a) : z);
+    cstore(Pj, (long)r * ld + a, j < N ? cload(p, (long)j * na + a) : z);
+    cstore(Ai, (long)r * ld + a, i < N ? cload(g, (long)i * n + core + a) : z);
+    cstore(Aj, (long)r * ld + a, j < N ? cload(g, (long)j * n + core + a) : z);
   }
-  if (j < i) return;
-  cplx v = cisd_gamma_at(g, p, i, j, n, core, na);
-  if (j != i) v = cadd(v, cisd_gamma_at(g, p, j, i, n, core, na));
+  __syncthreads();
   const long np = (long)N * (N + 1) / 2;
-  cstore(out, w * np + (long)i * N - ((long)i * (i - 1)) / 2 + (j - i), v);
+  for (int e = tid; e < TS * TS; e += 256) {
+    const int ri = e / TS, rj = e - ri * TS;
+    const int i = bi * TS + ri, j = bj * TS + rj;
+    if (i >= N || j >= N || (bi == bj && rj < ri)) continue;
+    cplx gij = j < n ? cload(g, (long)i * n + j) : cplx{0.0, 0.0};
+    cplx gji = i < n ? cload(g, (long)j * n + i) : cplx{0.0, 0.0};
+    for (int a = 0; a < na; ++a) {
+      gij = csub(gij, cmul(cload(Pi, (long)ri * ld + a), cload(Aj, (long)rj * ld + a)));
+      gji = csub(gji, cmul(cload(Pj, (long)rj * ld + a), cload(Ai, (long)ri * ld + a)));
+    }
+    if (packed) {
+      const cplx v = i == j ? gij : cadd(gij, gji);
+      cstore(out, w * np + (long)i * N - ((long)i * (i - 1)) / 2 + (j - i), v);
+    } else {
+      cstore(out, (w * N + i) * N + j, gij);
+      cstore(out, (w * N + j) * N + i, gji);
+    }
+  }
 }
 
 // gtall[j, wl n + p] = gt[w0 + wl, j, p]
@@ -167,14 +237,21 @@ __global__ __launch_bounds__(256) void cisd_permute_kernel(const double* __restr
   cstore(gtall, e, cload(gt, (wl * N + j) * n + p));
 }
 
-// One workgroup per (g, walker of the chunk).  R: ((G+1) N) x (wc n), row (g N + i), column (wl n + p);
-// row block G holds h1 GT.  terms[(wl (G+1) + g) * 4 + k]: tr R_g, tr(R_g[:n] R_g[:n]), <glgp, H>, <Ps, R_g[:,act]>.
-// x_g (for g < G) goes to xg[(wl G + g) ov + a va + t].
+// One workgroup per (g, walker of the chunk).  R: ((G+1) N) x (wc n), row (g N + i), column (wl n + p); row
+// block G holds h1 GT.  terms[(wl (G+1) + g) * 4 + k]: tr R_g, tr(R_g[:n] R_g[:n]), <glgp, H>, <Ps, R_g[:,act]>,
+// with H = R_g[:n, act] Bs.  x_g (g < G) goes to xg[(wl G + g) ov + a va + t].  LDS: f = R_g[:n] (n x n) for the
+// whole workgroup; GT rows n+t, R_g rows n+t and Bs columns t in chunks of tt.
 __global__ __launch_bounds__(256) void cisd_terms_kernel(const double* __restrict__ R, const double* __restrict__ gt,
                                                         const double* __restrict__ bs, const double* __restrict__ ps,
-                                                        int N, int n, int G, int core, int na, int va, int wc,
+                                                        int N, int n, int G, int core, int na, int va, int wc, int TT,
                                                         double* __restrict__ terms, double* __restrict__ xg) {
+  extern __shared__ double sh[];
   __shared__ double red[256];
+  const int ldn = n + 1;
+  double* f = sh;                           // [n][n + 1]
+  double* rv = f + 2 * n * ldn;             // [TT][n + 1]
+  double* gv = rv + 2 * TT * ldn;           // [TT][n + 1]
+  double* bc = gv + 2 * TT * ldn;           // [na][TT + 1]
   const int tid = threadIdx.x;
   const int g = blockIdx.x;
   const long wl = blockIdx.y;
@@ -185,37 +262,54 @@ __global__ __launch_bounds__(256) void cisd_terms_kernel(const double* __restric
   const double* b = bs + 2 * wl * ov;
   const double* pw = ps + 2 * wl * N * na;
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  // traces
   for (int e = tid; e < n * n; e += 256) {
     const int p = e / n, q = e - p * n;
-    const cplx rpq = cload(Rg, (long)p * ldc + q);
-    if (p == q) {
-      v[0] += rpq.re;
-      v[1] += rpq.im;
-    }
-    const cplx m = cmul(rpq, cload(Rg, (long)q * ldc + p));
-    v[2] += m.re;
-    v[3] += m.im;
+    cstore(f, (long)p * ldn + q, cload(Rg, (long)p * ldc + q));
   }
-  // glgp and its two uses
-  for (long e = tid; e < (long)n * va; e += 256) {
-    const int p = (int)(e / va), t = (int)(e - (long)p * va);
-    cplx gl = cload(Rg, (long)(n + t) * ldc + p);
-    gl = {-gl.re, -gl.im};
-    for (int i = 0; i < n; ++i) gl = cadd(gl, cmul(cload(Rg, (long)i * ldc + p), cload(gw, (long)(n + t) * n + i)));
-    cplx h = {0.0, 0.0};
-    for (int a = 0; a < na; ++a) h = cadd(h, cmul(cload(Rg, (long)p * ldc + core + a), cload(b, (long)a * va + t)));
-    const cplx m = cmul(gl, h);
-    v[4] += m.re;
-    v[5] += m.im;
-    if (g < G && p >= core && p < core + na) cstore(xg, (wl * G + g) * ov + (long)(p - core) * va + t, gl);
-  }
-  // <Ps, R_g[:, act]>
+  // <Ps, R_g[:, act]> straight from memory (each element used once)
   for (long e = tid; e < (long)N * na; e += 256) {
     const int i = (int)(e / na), a = (int)(e - (long)i * na);
     const cplx m = cmul(cload(pw, e), cload(Rg, (long)i * ldc + core + a));
     v[6] += m.re;
     v[7] += m.im;
+  }
+  __syncthreads();
+  for (int e = tid; e < n * n; e += 256) {
+    const int p = e / n, q = e - p * n;
+    const cplx rpq = cload(f, (long)p * ldn + q);
+    if (p == q) {
+      v[0] += rpq.re;
+      v[1] += rpq.im;
+    }
+    const cplx m = cmul(rpq, cload(f, (long)q * ldn + p));
+    v[2] += m.re;
+    v[3] += m.im;
+  }
+  for (int t0 = 0; t0 < va; t0 += TT) {
+    const int tt = va - t0 < TT ? va - t0 : TT;
+    for (int e = tid; e < tt * n; e += 256) {
+      const int t = e / n, i = e - t * n;
+      cstore(rv, (long)t * ldn + i, cload(Rg, (long)(n + t0 + t) * ldc + i));
+      cstore(gv, (long)t * ldn + i, cload(gw, (long)(n + t0 + t) * n + i));
+    }
+    for (int e = tid; e < na * tt; e += 256) {
+      const int a = e / tt, t = e - a * tt;
+      cstore(bc, (long)a * (TT + 1) + t, cload(b, (long)a * va + t0 + t));
+    }
+    __syncthreads();
+    for (int e = tid; e < n * tt; e += 256) {
+      const int p = e / tt, t = e - p * tt;
+      cplx gl = cload(rv, (long)t * ldn + p);
+      gl = {-gl.re, -gl.im};
+      for (int i = 0; i < n; ++i) gl = cadd(gl, cmul(cload(f, (long)i * ldn + p), cload(gv, (long)t * ldn + i)));
+      cplx h = {0.0, 0.0};
+      for (int a = 0; a < na; ++a) h = cadd(h, cmul(cload(f, (long)p * ldn + core + a), cload(bc, (long)a * (TT + 1) + t)));
+      const cplx m = cmul(gl, h);
+      v[4] += m.re;
+      v[5] += m.im;
+      if (g < G && p >= core && p < core + na) cstore(xg, (wl * G + g) * ov + (long)(p - core) * va + t0 + t, gl);
+    }
+    __syncthreads();
   }
   double s[8];
 #pragma unroll
@@ -275,6 +369,11 @@ __global__ __launch_bounds__(256) void cisd_combine_kernel(const double* __restr
 }
 
 inline unsigned blocks_of(long n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t set_lds(const void* f, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
 
 int check_shape(int W, int N, int n, int core, int na, int va) {
   if (W < 1 || n < 1 || na < 1 || va < 1 || core < 0 || core + na != n || n + va > N)
@@ -347,9 +446,16 @@ int trot_cisd_ps(const double* gt, const double* bs, int W, int N, int n, int co
   if (!gt || !bs || !ps) return fail(TROT_ERR_INVALID, "cisd_ps: null pointer");
   if (int rc = check_shape(W, N, n, core, na, va)) return rc;
   if (W > 65535) return fail(TROT_ERR_UNSUPPORTED, "cisd_ps: more than 65535 walkers per call");
-  hipLaunchKernelGGL(cisd_ps_kernel, dim3(blocks_of((long)N * na), W), dim3(256), 0, (hipStream_t)stream, gt, bs, N, n,
-                     na, va, ps);
+  hipStream_t s = (hipStream_t)stream;
+  const size_t lds = (size_t)(PS_TT * (n + 1) + na * (PS_TT + 1)) * 16;
+  HIPCHK(set_lds((const void*)cisd_ps_top_kernel, lds));
+  hipLaunchKernelGGL(cisd_ps_top_kernel, dim3(W), dim3(256), lds, s, gt, bs, N, n, na, va, ps);
   HIPCHK(hipGetLastError());
+  if (N > n) {
+    hipLaunchKernelGGL(cisd_ps_rest_kernel, dim3(blocks_of((long)(N - n) * na), W), dim3(256), 0, s, bs, N, n, na, va,
+                       ps);
+    HIPCHK(hipGetLastError());
+  }
   return TROT_OK;
 }
 
@@ -359,24 +465,37 @@ int trot_cisd_gamma(const double* gt, const double* ps, int W, int N, int n, int
   if (W < 1 || n < 1 || na < 1 || core < 0 || core + na != n || n > N)
     return fail(TROT_ERR_INVALID, "cisd_gamma: bad sizes");
   if (W > 65535) return fail(TROT_ERR_UNSUPPORTED, "cisd_gamma: more than 65535 walkers per call");
-  hipLaunchKernelGGL(cisd_gamma_kernel, dim3(blocks_of((long)N * N), W), dim3(256), 0, (hipStream_t)stream, gt, ps, N,
-                     n, core, na, packed, out);
+  if (na <= 32) {
+    const int nt = (N + 31) / 32;
+    const size_t lds = (size_t)4 * 32 * (na + 1) * 16;
+    HIPCHK(set_lds((const void*)cisd_gamma_tile_kernel<32>, lds));
+    hipLaunchKernelGGL(cisd_gamma_tile_kernel<32>, dim3(nt * (nt + 1) / 2, W), dim3(256), lds, (hipStream_t)stream, gt,
+                       ps, N, n, core, na, nt, packed, out);
+  } else {
+    const int nt = (N + 15) / 16;
+    const size_t lds = (size_t)4 * 16 * (na + 1) * 16;
+    HIPCHK(set_lds((const void*)cisd_gamma_tile_kernel<16>, lds));
+    hipLaunchKernelGGL(cisd_gamma_tile_kernel<16>, dim3(nt * (nt + 1) / 2, W), dim3(256), lds, (hipStream_t)stream, gt,
+                       ps, N, n, core, na, nt, packed, out);
+  }
   HIPCHK(hipGetLastError());
   return TROT_OK;
 }
 
-size_t trot_cisd_energy_workspace(int wc, int N, int n, int G, int na, int va) {
+size_t trot_cisd_energy_workspace(int wc, int N, int n, int G, int na, int va, int b3) {
   if (wc < 1 || N < 1 || n < 1 || G < 1 || na < 1 || va < 1) return 0;
   const size_t ov = (size_t)na * va;
-  return align256((size_t)N * wc * n * 16) + align256((size_t)(G + 1) * N * wc * n * 16) +
-         2 * align256((size_t)wc * G * ov * 16) + align256((size_t)wc * (G + 1) * 4 * 16) +
-         align256((size_t)wc * G * 16);
+  size_t bytes = align256((size_t)N * wc * n * 16) + align256((size_t)(G + 1) * N * wc * n * 16) +
+                 2 * align256((size_t)wc * G * ov * 16) + align256((size_t)wc * (G + 1) * 4 * 16) +
+                 align256((size_t)wc * G * 16);
+  if (b3) bytes += align256(trot_b3_planes_bytes(wc * G, (int)ov, 1));
+  return bytes;
 }
 
 // lx: (G+1, N, N) real, the Cholesky vectors followed by h1.  Walkers go through in chunks of wc.
 int trot_cisd_energy(const double* gt, const double* bs, const double* ps, const double* omega, const double* lx,
-                     const double* D, int W, int N, int n, int G, int core, int na, int va, double h0, int wc,
-                     double* e, void* ws, size_t ws_bytes, void* stream) {
+                     const double* D, const void* d_planes, int W, int N, int n, int G, int core, int na, int va,
+                     double h0, int wc, double* e, void* ws, size_t ws_bytes, void* stream) {
   if (!gt || !bs || !ps || !omega || !lx || !D || !e) return fail(TROT_ERR_INVALID, "cisd_energy: null pointer");
   if (int rc = check_shape(W, N, n, core, na, va)) return rc;
   if (G < 1 || wc < 1) return fail(TROT_ERR_INVALID, "cisd_energy: bad sizes");
@@ -384,7 +503,9 @@ int trot_cisd_energy(const double* gt, const double* bs, const double* ps, const
   const long ov = (long)na * va;
   if ((long)(G + 1) * N >= (1L << 31) || (long)wc * G >= (1L << 31) || G + 1 > 65535 * 256 || wc > 65535)
     return fail(TROT_ERR_UNSUPPORTED, "cisd_energy: sizes beyond the launch limits");
-  if (!ws || ws_bytes < trot_cisd_energy_workspace(wc, N, n, G, na, va))
+  const int b3 = d_planes != nullptr;
+  if (b3 && (long)wc * G * 4 > (1L << 31)) return fail(TROT_ERR_UNSUPPORTED, "cisd_energy: chunk too large for bf16");
+  if (!ws || ws_bytes < trot_cisd_energy_workspace(wc, N, n, G, na, va, b3))
     return fail(TROT_ERR_WORKSPACE, "cisd_energy: workspace");
   Carve cv{(char*)ws, ws_bytes, 0};
   double* gtall = cv.take<double>((size_t)N * wc * n * 2);
@@ -393,7 +514,11 @@ int trot_cisd_energy(const double* gt, const double* bs, const double* ps, const
   double* Z = cv.take<double>((size_t)wc * G * ov * 2);
   double* terms = cv.take<double>((size_t)wc * (G + 1) * 4 * 2);
   double* q = cv.take<double>((size_t)wc * G * 2);
+  void* xplanes = b3 ? (void*)cv.take<char>(trot_b3_planes_bytes(wc * G, (int)ov, 1)) : nullptr;
   hipStream_t s = (hipStream_t)stream;
+  const int TT = n <= 32 ? 32 : 16;
+  const size_t terms_lds = (size_t)(n * (n + 1) + 2 * TT * (n + 1) + na * (TT + 1)) * 16;
+  HIPCHK(set_lds((const void*)cisd_terms_kernel, terms_lds));
   for (int w0 = 0; w0 < W; w0 += wc) {
     const int c = W - w0 < wc ? W - w0 : wc;
     const long cols = (long)c * n;
@@ -406,10 +531,17 @@ int trot_cisd_energy(const double* gt, const double* bs, const double* ps, const
     p.C = R; p.ldc = cols; p.sC = 0;
     p.M = (G + 1) * N; p.N = (int)cols; p.K = N; p.ar = 1.0;
     HIPCHK((launch_gemm_f64<64, 64, false, true, true, true, false, EPI_STORE>(p, 1, s)));
-    hipLaunchKernelGGL(cisd_terms_kernel, dim3(G + 1, c), dim3(256), 0, s, R, gt + 2 * (long)w0 * N * n,
-                       bs + 2 * (long)w0 * ov, ps + 2 * (long)w0 * N * na, N, n, G, core, na, va, c, terms, X);
+    hipLaunchKernelGGL(cisd_terms_kernel, dim3(G + 1, c), dim3(256), terms_lds, s, R, gt + 2 * (long)w0 * N * n,
+                       bs + 2 * (long)w0 * ov, ps + 2 * (long)w0 * N * na, N, n, G, core, na, va, c, TT, terms, X);
     HIPCHK(hipGetLastError());
-    HIPCHK(times_d(X, D, Z, (long)c * G, ov, s));
+    if (b3) {  // Z = X D on split-BF16 planes (d_planes: D^T, ov rows of K = ov)
+      if (int rc = trot_split_bf16(X, 0, 1, 0, ov, c * G, (int)ov, xplanes, stream)) return rc;
+      if (int rc = trot_gemm_b3(xplanes, 1, d_planes, c * G, (int)ov, (int)ov, Z, 1, ov, 1.0, 0.0, 0.0, 1, nullptr, 0,
+                                stream))
+        return rc;
+    } else {
+      HIPCHK(times_d(X, D, Z, (long)c * G, ov, s));
+    }
     hipLaunchKernelGGL(cisd_rowdot_kernel, dim3((unsigned)((long)c * G)), dim3(256), 0, s, Z, X, ov, q);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(cisd_combine_kernel, dim3(c), dim3(256), 0, s, terms, q, omega + 2 * (long)w0, G, h0,

@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replayed step (profiling runs)")
     ap.add_argument("--cpu-walkers", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cisd-workload", default="cisd_n128_o16_g512_w1024")
+    ap.add_argument("--cisd-only", action="store_true",
+                    help="only the CISD-trial rates, printed as their own JSON line (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device copies of 1 GiB (scale reference for rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -170,6 +173,9 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.cisd_only:
+        print(json.dumps(cisd_rates(args)), flush=True)
+        return
     import torch
 
     from trot_amd import _lib, meas, prop, synthetic, trial
@@ -255,6 +261,8 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(syn, mixed, args, dt)
 
+    cisd = None if args.no_block else cisd_rates(args)
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -286,6 +294,7 @@ def main():
         "block_graph": block_graph,
         "launch_bound_case": small,
         "free_projection": fp_rates,
+        "cisd": cisd,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -491,6 +500,74 @@ def free_projection_rates(syn, env, args) -> dict:
         out[f"{tag}_walker_steps_per_s"] = round(W / dt, 1)
         log(f"free projection {tag}: {1e3 * dt:.2f} ms/step")
         del st, pctx, mctx, pops, mops
+    return out
+
+
+def cisd_rates(args) -> dict:
+    """CISD trial (trot_amd.cisd) on its own workload: step rate and local-energy time, FP64
+    (mixed_precision=False, testing=True) and the reference's default precision (mixed_precision=True,
+    testing=False: split-BF16 force bias and doubles term) with the mixed propagation of the main line;
+    the two energies compared on the same walkers."""
+    import torch
+
+    from trot_amd import cisd, prop, synthetic
+    from trot_amd.core import HamChol, System
+
+    cfg = synthetic.WORKLOADS[args.cisd_workload]
+    N, n, G, W, dt = cfg["norb"], cfg["nocc"], cfg["nchol"], cfg["n_walkers"], cfg["dt"]
+    dev = torch.device("cuda")
+    h0, h1, chol, ci1, ci2, w0 = synthetic.make_synthetic_cisd(N, n, G, W, cfg["seed"], cfg["tau"], dev)
+    sys_ = System(N, (n, n), "restricted")
+    ham = HamChol(h0, h1, chol)
+    tr = cisd.CisdTrial(ci1, ci2)
+    tops = cisd.make_cisd_trial_ops(sys_)
+    out = {"workload": args.cisd_workload, "nbasis": N, "nocc": n, "nvir": N - n, "nchol": G, "n_walkers": W,
+           "data": "synthetic (make_synthetic_cisd)"}
+    energies = {}
+    state = None
+    for tag, mixed in (("fp64", False), ("mixed", True)):
+        mops = cisd.make_cisd_meas_ops(sys_, mixed_precision=mixed, testing=not mixed)
+        if mixed:
+            pops = prop.make_prop_ops("restricted", "restricted", precision=args.precision,
+                                      taylor_precision=args.taylor_precision, taylor_hi_from=args.taylor_hi_from)
+        else:
+            pops = prop.make_prop_ops("restricted", "restricted")
+        params = prop.QmcParams(dt=dt, n_walkers=W, seed=cfg["seed"])
+        pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
+        mctx = mops.build_meas_ctx(ham, tr)
+        st = pops.init_prop_state(sys=sys_, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops,
+                                  params=params, initial_walkers=w0)
+        kw = dict(params=params, ham_data=ham, trial_data=tr, trial_ops=tops, meas_ops=mops, meas_ctx=mctx,
+                  prop_ctx=pctx)
+        for _ in range(2):
+            st = pops.step(st, **kw)
+        torch.cuda.synchronize()
+        steps = 5
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = pops.step(st, **kw)
+        torch.cuda.synchronize()
+        t_step = (time.perf_counter() - t0) / steps
+        if state is None:
+            state = st.walkers.clone()  # both precisions measure the energy on the FP64 run's walkers
+        _, greens = mops.engine.overlap_green(state, tr)
+        mops.engine.energy_from_green(greens, h0, mctx)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e = mops.engine.energy_from_green(greens, h0, mctx)
+        torch.cuda.synchronize()
+        t_e = time.perf_counter() - t0
+        energies[tag] = e.cpu().numpy()
+        out[f"{tag}_ms_per_step"] = round(1e3 * t_step, 3)
+        out[f"{tag}_walker_steps_per_s"] = round(W / t_step, 1)
+        out[f"{tag}_energy_ms"] = round(1e3 * t_e, 2)
+        log(f"cisd {tag}: {1e3 * t_step:.2f} ms/step, energy {1e3 * t_e:.1f} ms")
+        del st, pctx, mctx, pops, mops, greens
+    d = energies["mixed"] - energies["fp64"]
+    out["mixed_vs_fp64_energy"] = {"max_abs_Ha": float(np.max(np.abs(d))),
+                                   "mean_diff_Ha": float(abs(np.mean(d.real))),
+                                   "mean_fp64_Ha": float(np.mean(energies["fp64"].real))}
+    out["speedup_mixed_vs_fp64"] = round(out["fp64_ms_per_step"] / out["mixed_ms_per_step"], 3)
     return out
 
 

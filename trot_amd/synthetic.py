@@ -103,6 +103,32 @@ def make_synthetic(norb: int, nocc: int, nchol: int, n_walkers: int, seed: int, 
                            trial_b=trial_b, walkers_b=walkers_b)
 
 
+def make_synthetic_cisd(norb: int, nocc: int, nchol: int, n_walkers: int, seed: int, tau: float, device,
+                        ci1_scale: float = 0.02, ci2_scale: float = 0.005, noise: float = 1e-2):
+    """A CISD workload: the recipe's Hamiltonian (every L_gamma scaled by 1/sqrt(N) so that walkers
+    survive) rotated to the eigenbasis of h1, where the reference determinant occupies the first nocc
+    orbitals (the basis the CISD trial assumes); random real amplitudes ci1 (nocc, nvir) and ci2 with
+    pair symmetry ci2[i,a,j,b] = ci2[j,b,i,a]; walkers near [I; 0].  Returns (h0, h1, chol, ci1, ci2,
+    walkers) with h1, chol and walkers as GPU tensors."""
+    import torch
+
+    syn = make_synthetic(norb, nocc, nchol, 1, seed, tau, device=device, chol_scale=1.0 / np.sqrt(norb))
+    evals, evecs = np.linalg.eigh(syn.h1)
+    c = torch.from_numpy(evecs).to(device)
+    chol = torch.matmul(torch.matmul(c.T, syn.chol), c)  # C^T L_g C for every g
+    chol = 0.5 * (chol + chol.transpose(1, 2))
+    h1 = torch.from_numpy(np.diag(evals)).to(device)
+    rng = np.random.default_rng(seed + 1000)
+    nvir = norb - nocc
+    ci1 = ci1_scale * rng.standard_normal((nocc, nvir))
+    ci2 = ci2_scale * rng.standard_normal((nocc, nvir, nocc, nvir))
+    ci2 = 0.5 * (ci2 + ci2.transpose(2, 3, 0, 1))
+    w = np.zeros((n_walkers, norb, nocc), complex)
+    w[:, :nocc] = np.eye(nocc)
+    w = w + noise * (rng.standard_normal(w.shape) + 1j * rng.standard_normal(w.shape))
+    return syn.h0, h1, chol.contiguous(), ci1, ci2, torch.from_numpy(w).to(device)
+
+
 def fields(seed: int, step: int, n_walkers: int, nchol: int) -> np.ndarray:
     """Pre-drawn N(0,1) auxiliary fields for one step, (W, G) float64."""
     return np.random.default_rng([seed, 7919, step]).standard_normal((n_walkers, nchol))
@@ -118,6 +144,8 @@ WORKLOADS = {
     "n256_o32_g1024_w1024": dict(norb=256, nocc=32, nchol=1024, n_walkers=1024, seed=1, tau=256.0, dt=0.01),
     "n512_o64_g2560_w2048": dict(norb=512, nocc=64, nchol=2560, n_walkers=2048, seed=2, tau=512.0, dt=0.005),
     "n128_o16_g640_w512": dict(norb=128, nocc=16, nchol=640, n_walkers=512, seed=3, tau=256.0, dt=0.005),
+    # CISD trial (make_synthetic_cisd): a mid-size molecule in a double-zeta basis
+    "cisd_n128_o16_g512_w1024": dict(norb=128, nocc=16, nchol=512, n_walkers=1024, seed=5, tau=256.0, dt=0.005),
 }
 
 

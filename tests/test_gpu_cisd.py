@@ -3,8 +3,10 @@
 FP64 bar 1e-10 relative for the overlap, force bias, local energy and mixed 1-RDM, with and without
 frozen core / outer orbitals, with the energy's walker chunks smaller than the batch; the split-BF16
 force bias and doubles term (mixed_precision=True, testing=False: the reference's defaults) within the
-tolerances the reference allows its float32 variants (tests/test_cisd.py:245-287, :426); one propagation step pair and one block (energy, QR,
-SR) through the generic step and block driver against the oracle's.
+tolerances the reference allows its float32 variants (tests/test_cisd.py:245-287, :426); one
+propagation step pair and one block (energy, QR, SR) through the generic step and block driver against
+the oracle's; one HIP graph per block (GraphBlock) bit-identical to the eager block in both precisions;
+a free-projection block with the CISD trial against the oracle's block_fp.
 """
 
 from __future__ import annotations
@@ -161,3 +163,78 @@ def test_block_matches_oracle(gpu):
     assert rel(st.walkers.cpu().numpy(), ref.walkers) < TOL
     assert rel(st.overlaps.cpu().numpy(), ref.overlaps) < TOL
     assert rel(st.weights.cpu().numpy(), ref.weights) < 1e-13
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_graph_block_cisd_matches_eager(gpu, mixed):
+    """One HIP graph per block (GraphBlock) with the CISD trial, bit-identical to the eager block."""
+    from trot_amd import prop
+    from trot_amd.batch import _tensors
+    from trot_amd.blocks import block
+    from trot_amd.graph import GraphBlock
+
+    print(f"[cisd graph block] mixed={mixed}", flush=True)
+    c = _case(24, 6, 1, 2, 30, 32, seed=12)
+    sys, ham, tr, mops, tops, mctx = _gpu(c, mixed=mixed)
+    W = c["w"].shape[0]
+    if mixed:
+        pops = prop.make_prop_ops("restricted", "restricted", precision="bf16x3", taylor_precision="bf16x3",
+                                  taylor_hi_from=3)
+    else:
+        pops = prop.make_prop_ops("restricted", "restricted")
+    params = prop.QmcParams(dt=0.01, n_walkers=W, seed=4, n_prop_steps=3)
+    pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
+    kw = dict(sys=sys, params=params, ham_data=ham, trial_data=tr, trial_ops=tops, meas_ops=mops, meas_ctx=mctx,
+              prop_ops=pops, prop_ctx=pctx)
+
+    def init():
+        return pops.init_prop_state(sys=sys, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops,
+                                    params=params, initial_walkers=torch.from_numpy(c["w"]).cuda())
+
+    s_e = init()
+    eager = []
+    for _ in range(3):
+        s_e, o = block(s_e, **kw)
+        eager.append((s_e, o))
+    gb = GraphBlock()
+    s_g = init()
+    for b in range(3):
+        s_g, o = gb(s_g, **kw)
+        torch.cuda.synchronize()
+        se, oe = eager[b]
+        for x, y in zip(_tensors(s_g), _tensors(se)):
+            assert torch.equal(x, y), b
+        assert torch.equal(o.scalars["energy"], oe.scalars["energy"]), b
+        torch.randn(4096, device="cuda").sum().item()  # unrelated eager work between replays
+
+
+def test_free_projection_block_cisd(gpu):
+    """Free projection (trot/prop/blocks.py:415-490) with the CISD trial's overlap and energy against the
+    oracle's block_fp with the CISD oracle's ops, pre-drawn fields and comb offsets."""
+    from trot_amd import fp
+
+    c = _case(20, 5, 0, 2, 24, 16, seed=21, mix=0.1)
+    sys, ham, tr, mops, tops, mctx = _gpu(c)
+    W = c["w"].shape[0]
+    params = fp.QmcParamsFp(dt=0.01, n_walkers=W, seed=5, n_prop_steps=4, n_qr_blocks=2, n_blocks=1, n_traj=1,
+                            ene0=-1.0)
+    pops = fp.make_prop_ops_fp("restricted", "restricted", sys)
+    pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
+    st = pops.init_prop_state(sys=sys, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops, params=params,
+                              initial_walkers=torch.from_numpy(c["w"]).cuda())
+    ref_ops = C.make_ops(c["h0"], c["h1"], c["chol"], c["trial"])
+    ref = O.init_prop_state(c["w"], ref_ops, W)
+    ref_ctx = O.build_prop_ctx_fp(c["h0"], c["h1"], c["chol"], C.get_rdm1(c["trial"]), 0.01, -1.0)
+    fields = [c["rng"].standard_normal((W, 24)) for _ in range(4)]
+    zetas = [float(z) for z in c["rng"].uniform(size=4)]
+    st, obs = fp.block_fp(st, sys=sys, params=params, ham_data=ham, trial_data=tr, trial_ops=tops, meas_ops=mops,
+                          meas_ctx=mctx, prop_ops=pops, prop_ctx=pctx,
+                          fields_seq=[torch.from_numpy(f).cuda() for f in fields], zetas=zetas)
+    rp = O.QmcParamsFp(dt=0.01, n_prop_steps=4, n_qr_blocks=2, ene0=-1.0, n_walkers=W)
+    ref, robs = O.block_fp(ref, fields, zetas, ref_ctx, ref_ops, rp)
+    assert rel(st.walkers.cpu().numpy(), ref.walkers) < TOL
+    assert rel(st.weights.cpu().numpy(), ref.weights) < TOL
+    assert rel(st.overlaps.cpu().numpy(), ref.overlaps) < TOL
+    for k in ("energy", "weight", "overlap", "abs_overlap"):
+        g = complex(obs.scalars[k].item())
+        assert abs(g - complex(robs[k])) <= TOL * max(abs(complex(robs[k])), 1e-300), k

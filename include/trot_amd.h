@@ -347,6 +347,7 @@ int trot_local_energy_lp(const float* rot_chol_a, const float* rot_chol_b, const
 #define TROT_GOP_GREEN 3
 #define TROT_GOP_VHS 4
 #define TROT_GOP_TAYLOR_NARROW 5
+#define TROT_GOP_FORCE_BIAS 6 /* problems from trot_force_bias_problem, not trot_gemm_problem */
 size_t trot_gemm_problem_bytes(void);
 long trot_gemm_problem(int op, const double* a, const double* b, double* c, double* d, int W, int N, int n,
                        double coef, void* desc);
@@ -357,6 +358,28 @@ int trot_inv_det_grouped(const void* probs, const int* start, int nprob, int tot
 size_t trot_copy_problem_bytes(void);
 int trot_copy_problem(const void* src, void* dst, long bytes, void* desc);
 int trot_copy_grouped(const void* probs, int nprob, long max_bytes, void* stream);
+/* The rest of the step in the same form.  trot_force_bias_problem: trot_force_bias_splitk's GEMM as one
+ * grouped problem (op TROT_GOP_FORCE_BIAS); with trot_force_bias_slices(W, G, Nn) > 1 slices `out` takes
+ * the slices' partial products and a trot_sum_problem adds them in slice order into fb.
+ * trot_shift_problem / trot_shift_grouped: trot_field_shift per set (trot/prop/afqmc.py:99-103).
+ * trot_update_problem / trot_update_grouped: trot_overlap_combine, trot_phaseless_update and the copy of
+ * the new overlaps over the old ones, one workgroup per set (afqmc.py:108-134).  All bit-identical to the
+ * per-set entry points. */
+int trot_force_bias_slices(int W, int G, long Nn);
+long trot_force_bias_problem(const double* green_t, const double* rot_chol_t, int W, int G, long Nn, double alpha,
+                             int accumulate, double* out, void* desc);
+size_t trot_sum_problem_bytes(void);
+long trot_sum_problem(const double* src, double* dst, int nb, long len, double scale, int accumulate, void* desc);
+int trot_sum_grouped(const void* probs, const int* start, int nprob, int total_blocks, void* stream);
+size_t trot_shift_problem_bytes(void);
+long trot_shift_problem(const double* fields, const double* fb, const double* mf, double sqrt_dt, int W, int G,
+                        double* xbar, double* shift_term, double* fb_term, void* desc);
+int trot_shift_grouped(const void* probs, const int* start, int nprob, int total_w, void* stream);
+size_t trot_update_problem_bytes(void);
+int trot_update_problem(const double* det_a, const double* det_b, double* overlaps, const double* shift_term,
+                        const double* fb_term, double* weights, double* scalars, long long* nodes, int W, double dt,
+                        double h0p_re, double h0p_im, double w_floor, double w_cap, double damping, void* desc);
+int trot_update_grouped(const void* probs, int nprob, void* stream);
 
 /* ---- CISD trial, restricted walkers (FP64) ----
  * The reference determinant occupies the first nocc orbitals; the active block is occupied orbitals

@@ -135,6 +135,24 @@ def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
     assert h.trot_copy_problem(d, d, 17, cdesc) == -1  # sizes are multiples of 16 bytes
     assert h.trot_copy_problem(d, d, 32, cdesc) == 0
     assert h.trot_copy_grouped(d, 0, 32, None) == -1
+    assert h.trot_force_bias_slices(128, 800, 8000) == 31  # ~1024 workgroups, slices of >= 256, <= 64 slices
+    assert h.trot_force_bias_slices(2048, 2560, 32768) == 1
+    assert h.trot_force_bias_slices(0, 8, 8) == -1
+    gdesc = ctypes.create_string_buffer(h.trot_gemm_problem_bytes())
+    assert h.trot_force_bias_problem(d, d, 128, 800, 8000, 1.0, 0, d, gdesc) == 26 * 31  # tiles x slices
+    assert h.trot_force_bias_problem(None, d, 128, 800, 8000, 1.0, 0, d, gdesc) == -1
+    sdesc = ctypes.create_string_buffer(h.trot_sum_problem_bytes())
+    assert h.trot_sum_problem(d, d, 0, 100, 1.0, 0, sdesc) == -1
+    assert h.trot_sum_problem(d, d, 3, 1000, 1.0, 0, sdesc) == 4  # blocks of 256 elements
+    assert h.trot_sum_grouped(d, d, 0, 1, None) == -1
+    hdesc = ctypes.create_string_buffer(h.trot_shift_problem_bytes())
+    assert h.trot_shift_problem(d, d, d, 0.1, 0, 8, d, d, d, hdesc) == -1
+    assert h.trot_shift_problem(d, d, d, 0.1, 5, 8, d, d, d, hdesc) == 5  # one workgroup per walker
+    assert h.trot_shift_grouped(None, d, 1, 1, None) == -1
+    udesc = ctypes.create_string_buffer(h.trot_update_problem_bytes())
+    assert h.trot_update_problem(d, None, d, d, d, d, d, d, 4, 0.0, 0.0, 0.0, 1e-3, 100.0, 0.1, udesc) == -1
+    assert h.trot_update_problem(d, None, d, d, d, d, d, d, 4, 0.01, 0.0, 0.0, 1e-3, 100.0, 0.1, udesc) == 0
+    assert h.trot_update_grouped(d, 0, None) == -1
 
 
 def test_workspace_queries_are_sizes():

@@ -79,6 +79,17 @@ _SIGS = {
     "trot_copy_problem_bytes": (_sz, []),
     "trot_copy_problem": (_i, [_vp, _vp, _l, _vp]),
     "trot_copy_grouped": (_i, [_vp, _i, _l, _vp]),
+    "trot_force_bias_slices": (_i, [_i, _i, _l]),
+    "trot_force_bias_problem": (_l, [_vp, _vp, _i, _i, _l, _d, _i, _vp, _vp]),
+    "trot_sum_problem_bytes": (_sz, []),
+    "trot_sum_problem": (_l, [_vp, _vp, _i, _l, _d, _i, _vp]),
+    "trot_sum_grouped": (_i, [_vp, _vp, _i, _i, _vp]),
+    "trot_shift_problem_bytes": (_sz, []),
+    "trot_shift_problem": (_l, [_vp, _vp, _vp, _d, _i, _i, _vp, _vp, _vp, _vp]),
+    "trot_shift_grouped": (_i, [_vp, _vp, _i, _i, _vp]),
+    "trot_update_problem_bytes": (_sz, []),
+    "trot_update_problem": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _d, _d, _vp]),
+    "trot_update_grouped": (_i, [_vp, _i, _vp]),
     "trot_cisd_doubles_matrix": (_i, [_vp, _i, _i, _vp, _vp]),
     "trot_pack_sym": (_i, [_vp, _i, _i, _vp, _vp]),
     "trot_cisd_overlap_workspace": (_sz, [_i, _i, _i]),
@@ -933,7 +944,7 @@ def taylor_apply_lp_symblk(vpk32: torch.Tensor, w: torch.Tensor, sqrt_dt: float,
 # ---------------------------------------------------------------------------
 # grouped launches over many walker sets
 # ---------------------------------------------------------------------------
-GOP = {"one_body": 0, "taylor": 1, "overlap": 2, "green": 3, "vhs": 4, "taylor_narrow": 5}
+GOP = {"one_body": 0, "taylor": 1, "overlap": 2, "green": 3, "vhs": 4, "taylor_narrow": 5, "force_bias": 6}
 
 
 class Grouped:
@@ -947,7 +958,8 @@ class Grouped:
         self.kind, self.op = kind, op
         L = lib()
         self.size = {"gemm": L.trot_gemm_problem_bytes, "inv": L.trot_inv_problem_bytes,
-                     "copy": L.trot_copy_problem_bytes}[kind]()
+                     "copy": L.trot_copy_problem_bytes, "sum": L.trot_sum_problem_bytes,
+                     "shift": L.trot_shift_problem_bytes, "update": L.trot_update_problem_bytes}[kind]()
         self.descs, self.counts, self.keep = [], [], []
         self.max_bytes = 0
         self.probs = self.start = None
@@ -968,6 +980,41 @@ class Grouped:
         self.descs.append(buf.raw)
         self.counts.append(int(r))
         self.keep += [m, minv, det]
+
+    def _add(self, buf, count: int, keep):
+        self.descs.append(buf.raw)
+        self.counts.append(int(count))
+        self.keep += list(keep)
+
+    def add_force_bias(self, green_t, rot_chol_t, W: int, G: int, Nn: int, alpha: float, accumulate: bool, out):
+        """trot_force_bias_splitk's GEMM: ``out`` is fb with one slice, else the partial products."""
+        buf = ctypes.create_string_buffer(self.size)
+        wg = lib().trot_force_bias_problem(_p(green_t), _p(rot_chol_t), int(W), int(G), int(Nn), float(alpha),
+                                           int(accumulate), _p(out), buf)
+        _check(0 if wg > 0 else int(wg), "trot_force_bias_problem")
+        self._add(buf, wg, (green_t, rot_chol_t, out))
+
+    def add_sum(self, src, dst, nb: int, length: int, scale: float, accumulate: bool):
+        buf = ctypes.create_string_buffer(self.size)
+        nblk = lib().trot_sum_problem(_p(src), _p(dst), int(nb), int(length), float(scale), int(accumulate), buf)
+        _check(0 if nblk > 0 else int(nblk), "trot_sum_problem")
+        self._add(buf, nblk, (src, dst))
+
+    def add_shift(self, fields, fb, mf, sqrt_dt: float, W: int, G: int, xbar, shift, fbt):
+        buf = ctypes.create_string_buffer(self.size)
+        r = lib().trot_shift_problem(_p(fields), _p(fb), _p(mf), float(sqrt_dt), int(W), int(G), _p(xbar), _p(shift),
+                                     _p(fbt), buf)
+        _check(0 if r > 0 else int(r), "trot_shift_problem")
+        self._add(buf, r, (fields, fb, mf, xbar, shift, fbt))
+
+    def add_update(self, det_a, det_b, overlaps, shift, fbt, weights, scal, nodes, W: int, dt: float, h0p: complex,
+                   w_floor: float, w_cap: float, damping: float):
+        buf = ctypes.create_string_buffer(self.size)
+        h0p = complex(h0p)
+        _check(lib().trot_update_problem(_p(det_a), _p(det_b), _p(overlaps), _p(shift), _p(fbt), _p(weights),
+                                         _p(scal), _p(nodes), int(W), float(dt), h0p.real, h0p.imag, float(w_floor),
+                                         float(w_cap), float(damping), buf), "trot_update_problem")
+        self._add(buf, 1, (det_a, det_b, overlaps, shift, fbt, weights, scal, nodes))
 
     def add_copy(self, src: torch.Tensor, dst: torch.Tensor):
         if src.numel() != dst.numel() or src.dtype != dst.dtype or not (src.is_contiguous() and dst.is_contiguous()):
@@ -1001,5 +1048,12 @@ class Grouped:
         elif self.kind == "inv":
             _check(L.trot_inv_det_grouped(_p(self.probs), _p(self.start), n, self.total, _stream()),
                    "trot_inv_det_grouped")
+        elif self.kind == "sum":
+            _check(L.trot_sum_grouped(_p(self.probs), _p(self.start), n, self.total, _stream()), "trot_sum_grouped")
+        elif self.kind == "shift":
+            _check(L.trot_shift_grouped(_p(self.probs), _p(self.start), n, self.total, _stream()),
+                   "trot_shift_grouped")
+        elif self.kind == "update":
+            _check(L.trot_update_grouped(_p(self.probs), n, _stream()), "trot_update_grouped")
         else:
             _check(L.trot_copy_grouped(_p(self.probs), n, self.max_bytes, _stream()), "trot_copy_grouped")

@@ -98,14 +98,14 @@ def _groupable(f: Fragment) -> bool:
 class _GroupedPlan:
     """Static buffers of every fragment and the grouped launches of one step (FP64, sym-block V).
 
-    Per fragment, on the branches: force bias from the cached Green's functions and the shifted fields
-    (into static xbar / shift / bias buffers).  Then, one launch each over all fragments and spins: VHS,
-    one-body, copy of the first term, the Taylor terms, one-body, C^T w, inverse and determinant,
-    Green's functions (into the static state).  Then, per fragment: overlaps and the weight update.
-    Every launch runs, per fragment, the problem of the fragment's own entry point, so the step is
-    bit-identical to stepping the fragments one at a time."""
+    One launch each over all fragments and spins: the force bias from the cached Green's functions (its
+    split-K partial products, then their slice sums), the shifted fields, VHS, one-body, copy of the
+    first term, the Taylor terms, one-body, C^T w, inverse and determinant, Green's functions (into the
+    static state), and the overlaps with the weight update.  Every launch runs, per fragment, the
+    problem of the fragment's own entry point, so the step is bit-identical to stepping the fragments
+    one at a time."""
 
-    def __init__(self, frags: List[Fragment]):
+    def __init__(self, frags: List[Fragment], views):
         from .trial import _coeffs
 
         n_terms = {int(f.params.n_exp_terms) for f in frags}
@@ -123,7 +123,13 @@ class _GroupedPlan:
         ovl = _lib.Grouped("gemm", "overlap")
         inv = _lib.Grouped("inv")
         grn = _lib.Grouped("gemm", "green")
-        for f in frags:
+        # the force bias as trot_force_bias_splitk runs it per fragment: spin a (alpha), then spin b added
+        # (alpha 1); one-slice problems write fb, split ones their partials, summed in slice order
+        fb_a, fb_b = _lib.Grouped("gemm", "force_bias"), _lib.Grouped("gemm", "force_bias")
+        sum_a, sum_b = _lib.Grouped("sum"), _lib.Grouped("sum")
+        shf, upd = _lib.Grouped("shift"), _lib.Grouped("update")
+        L = _lib.lib()
+        for fi, f in enumerate(frags):
             ctx, st = f.prop_ctx, f.state
             W = f.n_walkers
             G = f.n_fields
@@ -159,12 +165,47 @@ class _GroupedPlan:
                 grn.add_gemm(w, minv, greens[si], None, W, N, n)
                 b["det"].append(det)
                 b["spins"].append((t, t0, t1, m, minv, c))
+            eng = f.meas_ops.engine
+            sa, sb = eng.spins(f.meas_ctx)
+            b["fb"] = torch.empty((W, G), dtype=torch.complex128, device=dev)
+            b["part"] = []
+            spin_fb = [(greens[0], sa, 2.0 if eng.kind == "rr" else 1.0, False)]
+            if eng.kind != "rr":
+                spin_fb.append((greens[1], sb, 1.0, True))
+            for si, (gt, sc, alpha, acc) in enumerate(spin_fb):
+                Nn = gt.shape[1] * gt.shape[2]
+                S = L.trot_force_bias_slices(W, G, Nn)
+                _lib._check(min(S, 0), "trot_force_bias_slices")
+                if S == 1:
+                    (fb_a if si == 0 else fb_b).add_force_bias(gt, sc.rot_chol_t, W, G, Nn, alpha, acc, b["fb"])
+                else:
+                    part = torch.empty((S, W, G), dtype=torch.complex128, device=dev)
+                    b["part"].append(part)
+                    fb_a.add_force_bias(gt, sc.rot_chol_t, W, G, Nn, alpha, acc, part)
+                    (sum_a if si == 0 else sum_b).add_sum(part, b["fb"], S, 2 * W * G, alpha, acc)
+            shf.add_shift(views[fi], b["fb"], ctx.mf_shifts, ctx.sqrt_dt, W, G, b["xbar"], b["shift"], b["fbt"])
+            p = f.params
+            upd.add_update(b["det"][0], b["det"][1] if len(b["det"]) > 1 else None, st.overlaps, b["shift"], b["fbt"],
+                           st.weights, f._scal, st.node_encounters, W, ctx.dt, ctx.h0_prop,
+                           getattr(p, "weight_floor", 1.0e-3), getattr(p, "weight_cap", 100.0),
+                           getattr(p, "pop_control_damping", 0.1))
             self.per.append(b)
         terms = [g for pair in zip(tay, tayn) for g in pair if g.descs]
         self.ops = [g.finish() for g in (vhs, ob1, cp, *terms, ob2, ovl, inv, grn)]
+        # spin a's sums write fb before spin b's one-slice GEMMs and sums add to it
+        self.pre = [g.finish() for g in (fb_a, sum_a, fb_b, sum_b, shf) if g.descs]
+        self.post = [upd.finish()]
+
+    def launch_pre(self):
+        for g in self.pre:
+            g.launch()
 
     def launch(self):
         for g in self.ops:
+            g.launch()
+
+    def launch_post(self):
+        for g in self.post:
             g.launch()
 
 
@@ -176,10 +217,10 @@ class FragmentBatch:
     the graph joins the branches at the end, so fragments on different branches may run concurrently.
     Per fragment the kernels and their order are the same as in a one-by-one run.
 
-    ``grouped`` (default: when every fragment runs the FP64 step on sym-block V): the step's GEMMs,
-    inverses and copies run as one launch each over all fragments (trot_gemm_f64_grouped,
-    trot_inv_det_grouped, trot_copy_grouped) between two per-fragment phases on the branches; the
-    results stay bit-identical to the one-by-one run.
+    ``grouped`` (default: when every fragment runs the FP64 step on sym-block V): every operation of the
+    step runs as one launch over all fragments (force bias and its slice sums, shifted fields, VHS,
+    one-body, Taylor terms, overlaps, inverses, Green's functions, weight update; the trot_*_grouped
+    entry points); the results stay bit-identical to the one-by-one run.
     """
 
     def __init__(self, fragments: List[Fragment], n_streams: int = 4, grouped: bool | None = None):
@@ -265,21 +306,21 @@ class FragmentBatch:
         st.overlaps.copy_(ov_new)
 
     def _step_grouped(self, views):
-        self._phase(views, self._pre)
+        self.plan.launch_pre()  # force bias, slice sums, shifted fields
         self.plan.launch()
-        self._phase(views, self._post)
+        self.plan.launch_post()  # overlaps, weights, population shift
         for f in self.fragments:  # the fields came from the caller: one draw of the key per step
             s, c = f.state.rng_key
             f.state = f.state._replace(rng_key=(s, c + 1))
 
-    def _make_plan(self):
+    def _make_plan(self, views):
         """Static state with the scalars as views of one (2,) buffer the weight update writes in place."""
         for f in self.fragments:
             scal = torch.stack([f.state.pop_control_ene_shift.reshape(()), f.state.e_estimate.reshape(())]).to(
                 torch.float64).contiguous()
             f.state = f.state._replace(pop_control_ene_shift=scal[0], e_estimate=scal[1])
             f._scal = scal
-        self.plan = _GroupedPlan(self.fragments)
+        self.plan = _GroupedPlan(self.fragments, views)
         for f, b in zip(self.fragments, self.plan.per):
             b["scal"] = f._scal
 
@@ -303,9 +344,9 @@ class FragmentBatch:
                 _, g = f.meas_ops.engine.overlap_green(f.state.walkers, f.trial)
                 f.state = f.state._replace(green=g)
             f.state = _static_copy(f.state)
-        if self.grouped:
-            self._make_plan()
         views = self.field_views()
+        if self.grouped:
+            self._make_plan(views)
         self.fields.copy_(warmup_fields)
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())

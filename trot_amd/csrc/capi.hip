@@ -637,6 +637,90 @@ int trot_build_prop_tensors(const double* chol, const double* dm, int G, int N, 
 // ---------------------------------------------------------------------------
 // grouped launches: one launch per operation over many walker sets (the batching form)
 // ---------------------------------------------------------------------------
+int trot_force_bias_slices(int W, int G, long Nn) {
+  if (W < 1 || G < 1 || Nn < 1) return fail(TROT_ERR_INVALID, "force_bias_slices: bad sizes");
+  return force_bias_slices(W, G, Nn);
+}
+
+// The GEMM of trot_force_bias_splitk as one problem of a grouped launch (op TROT_GOP_FORCE_BIAS): with one
+// slice (trot_force_bias_slices) it writes fb = alpha acc (+ fb if accumulate) into out; with S > 1 it
+// writes the S FP64 partial products into out (S W G complex) for a trot_sum_problem to add in slice
+// order, as trot_force_bias_splitk does.  Returns the workgroup count.
+long trot_force_bias_problem(const double* green_t, const double* rot_chol_t, int W, int G, long Nn, double alpha,
+                             int accumulate, double* out, void* desc) {
+  if (!green_t || !rot_chol_t || !out || !desc) return fail(TROT_ERR_INVALID, "force_bias_problem: null pointer");
+  if (W < 1 || G < 1 || Nn < 1 || Nn >= (1L << 31)) return fail(TROT_ERR_INVALID, "force_bias_problem: bad sizes");
+  const int S = force_bias_slices(W, G, Nn);
+  GemmP p{};
+  p.A = green_t; p.lda = Nn;
+  p.B = rot_chol_t; p.ldb = Nn;
+  p.C = out; p.ldc = G;
+  p.M = W; p.N = G;
+  if (S == 1) {
+    p.K = (int)Nn; p.ar = alpha; p.beta = accumulate ? 1.0 : 0.0;
+  } else {
+    const long ks = (Nn + S - 1) / S;
+    p.sA = ks; p.sB = ks; p.sC = (long)W * G;
+    p.K = (int)ks; p.Klast = (int)(Nn - (S - 1) * ks); p.ar = 1.0;
+  }
+  *reinterpret_cast<GemmP*>(desc) = p;
+  return gemm_f64_workgroups<64, 64>(p, S);
+}
+
+size_t trot_sum_problem_bytes(void) { return sizeof(SumProb); }
+size_t trot_shift_problem_bytes(void) { return sizeof(ShiftProb); }
+size_t trot_update_problem_bytes(void) { return sizeof(UpdProb); }
+
+// dst (len doubles) = (accumulate ? dst : 0) + scale sum_{b < nb} src[b len ...]: the slice sum of
+// trot_force_bias_splitk.  Returns the workgroup count.
+long trot_sum_problem(const double* src, double* dst, int nb, long len, double scale, int accumulate, void* desc) {
+  if (!src || !dst || !desc || nb < 1 || len < 1) return fail(TROT_ERR_INVALID, "sum_problem: bad arguments");
+  *reinterpret_cast<SumProb*>(desc) = SumProb{src, dst, len, nb, accumulate, scale};
+  return (len + 255) / 256;
+}
+
+int trot_sum_grouped(const void* probs, const int* start, int nprob, int total_blocks, void* stream) {
+  if (!probs || !start || nprob < 1 || total_blocks < 1) return fail(TROT_ERR_INVALID, "sum_grouped: bad arguments");
+  HIPCHK(launch_sum_grouped((const SumProb*)probs, start, nprob, total_blocks, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+// trot_field_shift of one walker set; returns W (workgroups).
+long trot_shift_problem(const double* fields, const double* fb, const double* mf, double sqrt_dt, int W, int G,
+                        double* xbar, double* shift_term, double* fb_term, void* desc) {
+  if (!fields || !fb || !mf || !xbar || !shift_term || !fb_term || !desc)
+    return fail(TROT_ERR_INVALID, "shift_problem: null pointer");
+  if (W < 1 || G < 1) return fail(TROT_ERR_INVALID, "shift_problem: bad sizes");
+  *reinterpret_cast<ShiftProb*>(desc) = ShiftProb{fields, fb, mf, xbar, shift_term, fb_term, sqrt_dt, G};
+  return W;
+}
+
+int trot_shift_grouped(const void* probs, const int* start, int nprob, int total_w, void* stream) {
+  if (!probs || !start || nprob < 1 || total_w < 1) return fail(TROT_ERR_INVALID, "shift_grouped: bad arguments");
+  HIPCHK(launch_shift_grouped((const ShiftProb*)probs, start, nprob, total_w, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+// trot_overlap_combine + trot_phaseless_update + the copy of the new overlaps over the old ones, for one
+// walker set (det_b NULL for restricted walkers).
+int trot_update_problem(const double* det_a, const double* det_b, double* overlaps, const double* shift_term,
+                        const double* fb_term, double* weights, double* scalars, long long* nodes, int W, double dt,
+                        double h0p_re, double h0p_im, double w_floor, double w_cap, double damping, void* desc) {
+  if (!det_a || !overlaps || !shift_term || !fb_term || !weights || !scalars || !nodes || !desc)
+    return fail(TROT_ERR_INVALID, "update_problem: null pointer");
+  if (W < 1 || !(dt > 0)) return fail(TROT_ERR_INVALID, "update_problem: bad sizes");
+  *reinterpret_cast<UpdProb*>(desc) = UpdProb{det_a, det_b, overlaps, shift_term, fb_term, weights, scalars, nodes,
+                                               W, dt, sqrt(dt), h0p_re, h0p_im, w_floor, w_cap, damping};
+  return TROT_OK;
+}
+
+int trot_update_grouped(const void* probs, int nprob, void* stream) {
+  if (!probs || nprob < 1) return fail(TROT_ERR_INVALID, "update_grouped: bad arguments");
+  if (nprob > 65535 * 256) return fail(TROT_ERR_UNSUPPORTED, "update_grouped: too many sets");
+  HIPCHK(launch_update_grouped((const UpdProb*)probs, nprob, (hipStream_t)stream));
+  return TROT_OK;
+}
+
 size_t trot_gemm_problem_bytes(void) { return sizeof(GemmP); }
 size_t trot_inv_problem_bytes(void) { return sizeof(InvProb); }
 size_t trot_copy_problem_bytes(void) { return sizeof(CopyProb); }
@@ -701,6 +785,10 @@ int trot_gemm_f64_grouped(int op, const void* probs, const int* start, int nprob
       break;
     case TROT_GOP_VHS:
       HIPCHK((launch_gemm_f64_grouped<64, 64, true, false, true, true, false, EPI_STORE>(P, start, nprob, total_wg, s)));
+      break;
+    case TROT_GOP_FORCE_BIAS:  // problems from trot_force_bias_problem
+      HIPCHK((launch_gemm_f64_grouped<64, 64, true, false, true, false, false, EPI_STORE>(P, start, nprob, total_wg,
+                                                                                        s)));
       break;
     default: return fail(TROT_ERR_INVALID, "gemm_f64_grouped: unknown operation");
   }

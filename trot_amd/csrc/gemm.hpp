@@ -269,7 +269,8 @@ __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const GemmP* __re
   const int local = g - start[lo];
   const int ntn = (p.N + BN - 1) / BN, ntm = (p.M + BM - 1) / BM;
   const int tn = local % ntn, tm = (local / ntn) % ntm, b = local / (ntn * ntm);
-  gemm_f64_tile<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM>(p, b, 1 << 30, tm * BM, tn * BN, sm);
+  const int nbatch = (start[lo + 1] - start[lo]) / (ntn * ntm);  // the last batch may take Klast
+  gemm_f64_tile<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM>(p, b, nbatch, tm * BM, tn * BN, sm);
 }
 
 template <int BM, int BN>

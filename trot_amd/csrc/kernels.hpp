@@ -22,6 +22,40 @@ struct CopyProb {
 hipError_t launch_inv_det_grouped(const InvProb* probs, const int* start, int nprob, int total_w, hipStream_t s);
 hipError_t launch_copy_grouped(const CopyProb* probs, int nprob, long max_bytes, hipStream_t s);
 hipError_t launch_overlap_combine(const double* det_a, const double* det_b, double* ovlp, int W, hipStream_t s);
+// grouped launches over many walker sets (descriptor tables in device memory)
+struct SumProb {  // dst = (accumulate ? dst : 0) + scale * sum_b src[b]  (len doubles, nb slices)
+  const double* src;
+  double* dst;
+  long len;
+  int nb;
+  int accumulate;
+  double scale;
+};
+struct ShiftProb {
+  const double* fields;
+  const double* fb;
+  const double* mf;
+  double* xbar;
+  double* shift;
+  double* fbt;
+  double sqrt_dt;
+  int G;
+};
+struct UpdProb {
+  const double* det_a;
+  const double* det_b;  // null: restricted walkers (overlap det_a^2)
+  double* ovlp;         // old overlaps in, new overlaps out
+  const double* shift;
+  const double* fbt;
+  double* weights;
+  double* scal;
+  long long* nodes;
+  int W;
+  double dt, sqrt_dt, h0p_re, h0p_im, w_floor, w_cap, damping;
+};
+hipError_t launch_shift_grouped(const ShiftProb* probs, const int* start, int nprob, int total_w, hipStream_t s);
+hipError_t launch_sum_grouped(const SumProb* probs, const int* start, int nprob, int total_blocks, hipStream_t s);
+hipError_t launch_update_grouped(const UpdProb* probs, int nprob, hipStream_t s);
 hipError_t launch_shift(const double* fields, const double* fb, const double* mf, double sqrt_dt, int W, int G,
                         double* xbar, float* xbar32, double* shift, double* fbt, hipStream_t s);
 hipError_t launch_weight_update(double* weights, const double* ov_old, const double* ov_new, const double* shift,

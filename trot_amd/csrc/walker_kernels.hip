@@ -183,13 +183,11 @@ __global__ void overlap_combine_kernel(const double* __restrict__ det_a, const d
 // One workgroup per walker, deterministic tree reduction.
 // xbar32 (optional): complex64 copy for the FP32 / split-BF16 VHS.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void shift_kernel(const double* __restrict__ fields, const double* __restrict__ fb,
-                                                    const double* __restrict__ mf, double sqrt_dt, int G,
-                                                    double* __restrict__ xbar, float* __restrict__ xbar32,
-                                                    double* __restrict__ shift, double* __restrict__ fbt) {
-  __shared__ double red[256];
+__device__ __forceinline__ void shift_one(const double* __restrict__ fields, const double* __restrict__ fb,
+                                          const double* __restrict__ mf, double sqrt_dt, int G, long w,
+                                          double* __restrict__ xbar, float* __restrict__ xbar32,
+                                          double* __restrict__ shift, double* __restrict__ fbt, double* red) {
   const int t = threadIdx.x;
-  const long w = blockIdx.x;
   double s_re = 0, s_im = 0, f_re = 0, f_im = 0;
   for (int g = t; g < G; g += 256) {
     const double x = fields[w * G + g];
@@ -220,24 +218,75 @@ __global__ __launch_bounds__(256) void shift_kernel(const double* __restrict__ f
   }
 }
 
+__global__ __launch_bounds__(256) void shift_kernel(const double* __restrict__ fields, const double* __restrict__ fb,
+                                                    const double* __restrict__ mf, double sqrt_dt, int G,
+                                                    double* __restrict__ xbar, float* __restrict__ xbar32,
+                                                    double* __restrict__ shift, double* __restrict__ fbt) {
+  __shared__ double red[256];
+  shift_one(fields, fb, mf, sqrt_dt, G, blockIdx.x, xbar, xbar32, shift, fbt, red);
+}
+
+// Problem index of block b from the prefix sums start[0..nprob] (start[0] = 0).
+__device__ __forceinline__ int grouped_problem(const int* __restrict__ start, int nprob, int b) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One workgroup per (walker set, walker): shift_kernel's work for every set in one launch.
+__global__ __launch_bounds__(256) void shift_grouped_kernel(const ShiftProb* __restrict__ probs,
+                                                            const int* __restrict__ start, int nprob) {
+  __shared__ double red[256];
+  const int pi = grouped_problem(start, nprob, blockIdx.x);
+  const ShiftProb& q = probs[pi];
+  shift_one(q.fields, q.fb, q.mf, q.sqrt_dt, q.G, blockIdx.x - start[pi], q.xbar, nullptr, q.shift, q.fbt, red);
+}
+
+// sum_batches_kernel's work for every set: block b of set p covers elements (b - start[p]) * 256 ...
+__global__ __launch_bounds__(256) void sum_grouped_kernel(const SumProb* __restrict__ probs,
+                                                          const int* __restrict__ start, int nprob) {
+  const int pi = grouped_problem(start, nprob, blockIdx.x);
+  const SumProb& q = probs[pi];
+  const long e = (long)(blockIdx.x - start[pi]) * 256 + threadIdx.x;
+  if (e >= q.len) return;
+  double acc = 0.0;
+  for (int b = 0; b < q.nb; ++b) acc += q.src[(long)b * q.len + e];
+  q.dst[e] = (q.accumulate ? q.dst[e] : 0.0) + q.scale * acc;
+}
+
 // ---------------------------------------------------------------------------
 // phaseless weight update (single workgroup, deterministic).
 // scal[0] = pop_control_ene_shift (in/out), scal[1] = e_estimate (in).
 // nodes[0] += number of walkers with importance <= 0 after flooring.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void weight_update_kernel(
-    double* __restrict__ weights, const double* __restrict__ ov_old, const double* __restrict__ ov_new,
-    const double* __restrict__ shift, const double* __restrict__ fbt, double* __restrict__ scal,
-    long long* __restrict__ nodes, int W, double dt, double sqrt_dt, double h0p_re, double h0p_im, double w_floor,
-    double w_cap, double damping) {
-  __shared__ double red[1024];
+// det_a (optional): the new overlap of walker w is computed here as det_a[w] det_b[w] (det_a[w]^2 without
+// det_b), exactly as overlap_combine_kernel does, instead of read from ov_new; ov_out (optional) receives
+// the new overlaps after the old ones were read (in place over ov_old is allowed: one thread per walker).
+__device__ __forceinline__ void weight_update_wg(double* __restrict__ weights, const double* ov_old,
+                                                 const double* __restrict__ ov_new, const double* __restrict__ det_a,
+                                                 const double* __restrict__ det_b, double* ov_out,
+                                                 const double* __restrict__ shift, const double* __restrict__ fbt,
+                                                 double* __restrict__ scal, long long* __restrict__ nodes, int W,
+                                                 double dt, double sqrt_dt, double h0p_re, double h0p_im,
+                                                 double w_floor, double w_cap, double damping, double* red) {
   const int t = threadIdx.x;
   const double pop = scal[0];
   const double e_est = scal[1];
   double wsum = 0.0;
   double ncount = 0.0;
   for (int w = t; w < W; w += 1024) {
-    const cplx on = cload(ov_new, w), oo = cload(ov_old, w);
+    cplx on;
+    if (det_a) {
+      const cplx a = cload(det_a, w);
+      const cplx b = det_b ? cload(det_b, w) : a;
+      on = cmul(a, b);
+    } else {
+      on = cload(ov_new, w);
+    }
+    const cplx oo = cload(ov_old, w);
     const cplx ratio = cmul(on, cinv(oo));
     const cplx st = cload(shift, w), ft = cload(fbt, w);
     const cplx ex = {-sqrt_dt * st.re + ft.re + dt * (pop + h0p_re), -sqrt_dt * st.im + ft.im + dt * h0p_im};
@@ -257,6 +306,7 @@ __global__ __launch_bounds__(1024) void weight_update_kernel(
     if (wn > w_cap) wn = 0.0;
     weights[w] = wn;
     wsum += wn;
+    if (ov_out) cstore(ov_out, w, on);
   }
   wsum = block_sum<1024>(wsum, red);
   ncount = block_sum<1024>(ncount, red);
@@ -266,6 +316,25 @@ __global__ __launch_bounds__(1024) void weight_update_kernel(
     scal[0] = e_est - damping * (log(avg) / dt);
     nodes[0] += (long long)ncount;
   }
+}
+
+__global__ __launch_bounds__(1024) void weight_update_kernel(
+    double* __restrict__ weights, const double* __restrict__ ov_old, const double* __restrict__ ov_new,
+    const double* __restrict__ shift, const double* __restrict__ fbt, double* __restrict__ scal,
+    long long* __restrict__ nodes, int W, double dt, double sqrt_dt, double h0p_re, double h0p_im, double w_floor,
+    double w_cap, double damping) {
+  __shared__ double red[1024];
+  weight_update_wg(weights, ov_old, ov_new, nullptr, nullptr, nullptr, shift, fbt, scal, nodes, W, dt, sqrt_dt,
+                   h0p_re, h0p_im, w_floor, w_cap, damping, red);
+}
+
+// One workgroup per walker set: overlap combination, weight update and the new overlaps stored over the
+// old ones (overlap_combine_kernel + weight_update_kernel + a copy, per set).
+__global__ __launch_bounds__(1024) void update_grouped_kernel(const UpdProb* __restrict__ probs) {
+  __shared__ double red[1024];
+  const UpdProb& q = probs[blockIdx.x];
+  weight_update_wg(q.weights, q.ovlp, nullptr, q.det_a, q.det_b, q.ovlp, q.shift, q.fbt, q.scal, q.nodes, q.W, q.dt,
+                   q.sqrt_dt, q.h0p_re, q.h0p_im, q.w_floor, q.w_cap, q.damping, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -467,6 +536,24 @@ hipError_t launch_gather_rows(const double* src, double* dst, const int* idx, in
 hipError_t launch_mf_trace(const double* chol, const double* dm, int G, int N, double* mf, hipStream_t s) {
   if (G <= 0) return hipSuccess;
   hipLaunchKernelGGL(mf_trace_kernel, dim3(G), dim3(256), 0, s, chol, dm, N, mf);
+  return hipGetLastError();
+}
+
+hipError_t launch_shift_grouped(const ShiftProb* probs, const int* start, int nprob, int total_w, hipStream_t s) {
+  if (nprob <= 0 || total_w <= 0) return hipSuccess;
+  hipLaunchKernelGGL(shift_grouped_kernel, dim3(total_w), dim3(256), 0, s, probs, start, nprob);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_grouped(const SumProb* probs, const int* start, int nprob, int total_blocks, hipStream_t s) {
+  if (nprob <= 0 || total_blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sum_grouped_kernel, dim3(total_blocks), dim3(256), 0, s, probs, start, nprob);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_grouped(const UpdProb* probs, int nprob, hipStream_t s) {
+  if (nprob <= 0) return hipSuccess;
+  hipLaunchKernelGGL(update_grouped_kernel, dim3(nprob), dim3(1024), 0, s, probs);
   return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@ from __future__ import annotations
 import json
 import struct
 from dataclasses import dataclass
+import mmap
 from pathlib import Path
 from typing import Any, Dict
 
@@ -32,8 +33,10 @@ _SIG = b"\x89HDF\r\n\x1a\n"
 
 
 class _Reader:
-    def __init__(self, data: bytes):
-        self.b = data
+    def __init__(self, data):
+        """``data``: the file as bytes or a read-only mmap; sliced through one memoryview (no copies)."""
+        self.src = data  # for find()
+        self.b = memoryview(data)
         base = None
         for off in (0, 512, 1024, 2048, 4096, 8192):
             if data[off:off + 8] == _SIG:
@@ -70,8 +73,10 @@ class _Reader:
         return self.base_addr + a
 
     def cstr(self, off: int) -> str:
-        end = self.b.index(b"\0", off)
-        return self.b[off:end].decode("utf-8")
+        end = self.src.find(b"\0", off)
+        if end < 0:
+            raise ValueError("unterminated string")
+        return bytes(self.b[off:end]).decode("utf-8")
 
     # -- object headers ---------------------------------------------------------------------------
     def messages(self, a: int):
@@ -176,7 +181,7 @@ class _Reader:
             osize = self.u(q + 8, self.L)
             d = q + 8 + self.L
             if idx == index:
-                return b[d:d + osize]
+                return bytes(b[d:d + osize])
             if idx == 0:
                 break
             q = d + ((osize + 7) // 8) * 8
@@ -189,7 +194,7 @@ class _Reader:
             return arr.reshape(shape) if shape else arr[0]
         if dt["kind"] == "str":
             s = dt["size"]
-            vals = [raw[i * s:(i + 1) * s].split(b"\0", 1)[0].decode("utf-8") for i in range(n)]
+            vals = [bytes(raw[i * s:(i + 1) * s]).split(b"\0", 1)[0].decode("utf-8") for i in range(n)]
             return vals[0] if not shape else np.array(vals, dtype=object).reshape(shape)
         if dt["kind"] == "vlstr":
             vals = []
@@ -219,7 +224,7 @@ class _Reader:
             nlen, tlen, slen = self.u(d + 2, 2), self.u(d + 4, 2), self.u(d + 6, 2)
             pad = (lambda x: (x + 7) // 8 * 8) if ver == 1 else (lambda x: x)
             p = d + 8 + (1 if ver == 3 else 0)
-            name = b[p:p + nlen].split(b"\0", 1)[0].decode("utf-8")
+            name = bytes(b[p:p + nlen]).split(b"\0", 1)[0].decode("utf-8")
             p += pad(nlen)
             dt = self.datatype(p)
             p += pad(tlen)
@@ -255,7 +260,7 @@ class _Reader:
                 nsz = 1 << (flags & 3)
                 nlen = self.u(p, nsz)
                 p += nsz
-                name = b[p:p + nlen].decode("utf-8")
+                name = bytes(b[p:p + nlen]).decode("utf-8")
                 p += nlen
                 if ltype == 0:
                     out[name] = self.u(p, self.O)
@@ -332,7 +337,7 @@ class _Reader:
         else:
             raise NotImplementedError(f"data layout version {ver}")
         if len(raw) < nbytes:
-            raw = raw + b"\0" * (nbytes - len(raw))  # unallocated storage reads as the zero fill value
+            raw = bytes(raw) + b"\0" * (nbytes - len(raw))  # unallocated storage reads as the zero fill value
         return self.decode(raw, dt, shape)
 
     def is_dataset(self, a: int) -> bool:
@@ -343,7 +348,14 @@ class H5File:
     """Read-only view of an HDF5 file: ``f["ham/h1"]`` (numpy array), ``f.attrs(path)``, ``f.keys(path)``."""
 
     def __init__(self, path):
-        self._r = _Reader(Path(path).read_bytes())
+        # memory-mapped: datasets are sliced from the mapping without copies and copied once into the
+        # returned arrays (a 5 GB Cholesky array never sits in memory twice)
+        with open(Path(path), "rb") as fh:
+            try:
+                self._mm = mmap.mmap(fh.fileno(), 0, access=mmap.ACCESS_READ)
+            except ValueError as e:  # empty file
+                raise ValueError(f"{path}: not an HDF5 file") from e
+        self._r = _Reader(self._mm)
 
     def _resolve(self, path: str) -> int:
         a = self._r.root

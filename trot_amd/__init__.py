@@ -17,7 +17,8 @@ def __getattr__(name):
     import importlib
 
     mods = {"prop": ".prop", "meas": ".meas", "trial": ".trial", "walkers": ".walkers", "blocks": ".blocks",
-            "driver": ".driver", "synthetic": ".synthetic", "stats": ".stats"}
+            "driver": ".driver", "synthetic": ".synthetic", "stats": ".stats", "cisd": ".cisd", "fp": ".fp",
+            "graph": ".graph", "batch": ".batch", "diff": ".diff", "staged": ".staged"}
     if name in mods:
         return importlib.import_module(mods[name], __name__)
     raise AttributeError(name)

@@ -237,6 +237,35 @@ def main():
                 "traffic": traffic_from_profiles(f"vhs_{args.precision}_{args.workload}"),
                 "launch_ms": round(vhs_ms, 3), "flop_per_launch": vhs_flop * passes}
 
+    # the whole step against its ideal: T* = sum_k F_k / peak_k over the contractions as run here
+    # (dense complex products 8 flop / MAC, real x complex 4; split-BF16 passes counted against the BF16
+    # peak, FP64 ones against FP64), then T* / measured
+    ps = PEAK[args.precision] * 1e12
+    p64 = PEAK["f64"] * 1e12
+    tp = PEAK[args.taylor_precision] * 1e12
+    hi = args.taylor_hi_from if args.taylor_precision == "bf16x3" and args.taylor_hi_from > 0 else 0
+    k_terms = env["params"].n_exp_terms
+    t_passes = (3 * (min(hi, k_terms + 1) - 1) + (k_terms - hi + 1)) if hi else (3 if args.taylor_precision ==
+                                                                                 "bf16x3" else 1) * k_terms
+    ob_passes = 3 if args.taylor_precision == "bf16x3" else 1
+    terms = {
+        "vhs": vhs_flop * passes / ps,
+        "force_bias": 4.0 * W * G * N * n * passes / ps,
+        "taylor": 8.0 * W * N * N * n * t_passes / tp,
+        "one_body": 2 * 4.0 * W * N * N * n * ob_passes / tp,
+        "overlap_inverse_green_f64": (4.0 * W * N * n * n + 8.0 * W * N * n * n + 8.0 * W * n ** 3) / p64,
+    }
+    t_star = sum(terms.values())
+    step_roofline = {"t_star_ms": round(1e3 * t_star, 3), "ms_per_step": round(1e3 * t_mix / args.steps, 3),
+                     "frac": round(t_star / (t_mix / args.steps), 4),
+                     "terms_ms": {k: round(1e3 * v, 3) for k, v in terms.items()},
+                     "taylor_passes": t_passes}
+    if rate64 is not None:
+        f64_flop = (4.0 * W * G * cols + 4.0 * W * G * N * n + 8.0 * W * N * N * n * k_terms
+                    + 2 * 4.0 * W * N * N * n + 4.0 * W * N * n * n + 8.0 * W * N * n * n + 8.0 * W * n ** 3)
+        step_roofline["fp64_t_star_ms"] = round(1e3 * f64_flop / p64, 3)
+        step_roofline["fp64_frac"] = round(f64_flop / p64 / (W / rate64), 4)
+
     block_rate = None
     block_graph = None
     small = None
@@ -296,6 +325,7 @@ def main():
         "free_projection": fp_rates,
         "cisd": cisd,
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -46,6 +46,7 @@ struct PropB3P {
   int W, N, n, Np, np_, nb;
   float c;              // Taylor coefficient sqrt(dt) / k
   int first, last;      // Taylor term position
+  int tn_hi;            // the next kernel reads only the hi planes of tn (one-pass term): lo planes not stored
 };
 
 __device__ __forceinline__ bf16x8_t neg_bf16x8(bf16x8_t x) {
@@ -310,9 +311,11 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
         const long o = (long)gp * p.Np + gi0;
         typedef unsigned short us4 __attribute__((ext_vector_type(4)));
         *reinterpret_cast<us4*>(Tn + 0 * tplane + o) = (us4){hr[0], hr[1], hr[2], hr[3]};
-        *reinterpret_cast<us4*>(Tn + 1 * tplane + o) = (us4){lr[0], lr[1], lr[2], lr[3]};
         *reinterpret_cast<us4*>(Tn + 2 * tplane + o) = (us4){hi[0], hi[1], hi[2], hi[3]};
-        *reinterpret_cast<us4*>(Tn + 3 * tplane + o) = (us4){li[0], li[1], li[2], li[3]};
+        if (!p.tn_hi) {
+          *reinterpret_cast<us4*>(Tn + 1 * tplane + o) = (us4){lr[0], lr[1], lr[2], lr[3]};
+          *reinterpret_cast<us4*>(Tn + 3 * tplane + o) = (us4){li[0], li[1], li[2], li[3]};
+        }
       }
     }
 }
@@ -432,6 +435,8 @@ int run_b3_chunk(const VSrc& v, const bf16raw* ep, const double* w_in, double* w
     P.first = k == 1;
     P.last = k == n_terms;
     P.tn = (P.last && !ep) ? nullptr : bufs[cur ^ 1];
+    // the next term, when it is one bf16 pass on the hi planes, never reads the lo planes of this output
+    P.tn_hi = !v.c64 && k < n_terms && k + 1 >= v.hi_from;
     P.win = w_out;
     P.w = w_out;
     P.c = (float)(sqrt_dt / k);
@@ -443,6 +448,7 @@ int run_b3_chunk(const VSrc& v, const bf16raw* ep, const double* w_in, double* w
   if (ep) {  // second one-body factor on the planes of the propagated walker
     P.tp = bufs[cur];
     P.tn = nullptr;
+    P.tn_hi = 0;
     P.win = w_out;
     P.w = w_out;
     if (int rc = launch_prop<true>(P, s)) return rc;

@@ -238,3 +238,20 @@ def test_free_projection_block_cisd(gpu):
     for k in ("energy", "weight", "overlap", "abs_overlap"):
         g = complex(obs.scalars[k].item())
         assert abs(g - complex(robs[k])) <= TOL * max(abs(complex(robs[k])), 1e-300), k
+
+
+def test_non_symmetric_hamiltonian_is_refused(gpu):
+    """The kernels use L_g = L_g^T and h1 = h1^T; a Hamiltonian without that symmetry raises ValueError."""
+    from trot_amd import cisd
+    from trot_amd.core import HamChol
+
+    c = _case(12, 4, 0, 0, 6, 4, seed=3)
+    sys, ham, tr, mops, tops, mctx = _gpu(c)
+    bad = c["chol"].copy()
+    bad[2, 0, 5] += 1e-3
+    with pytest.raises(ValueError):
+        cisd.build_meas_ctx(HamChol(c["h0"], c["h1"], bad).to_device(), tr)
+    h1 = c["h1"].copy()
+    h1[1, 3] += 1e-3
+    with pytest.raises(ValueError):
+        cisd.build_meas_ctx(HamChol(c["h0"], h1, c["chol"]).to_device(), tr)

@@ -250,8 +250,22 @@ def build_meas_ctx(ham_data, trial_data: CisdTrial, cfg: CisdMeasCfg = CisdMeasC
     N = trial_data.norb
     if tuple(chol.shape[1:]) != (N, N) or tuple(h1.shape) != (N, N):
         raise ValueError(f"Hamiltonian with {tuple(h1.shape)} orbitals does not match the trial's {N}")
+    _require_symmetric(chol, h1)
     lx = torch.cat([chol, h1[None]], dim=0).contiguous()
     return CisdMeasCtx(chol=chol, h1=h1, trial=trial_data, cfg=cfg, lpack=_lib.pack_sym(chol), lx=lx)
+
+
+def _require_symmetric(chol: torch.Tensor, h1: torch.Tensor) -> None:
+    """The kernels use L_g = L_g^T and h1 = h1^T (real orbitals); anything else is refused, not misread.
+    Within 1e-12 of the largest element (a rotation to the MO basis leaves rounding-level asymmetry)."""
+    asym = float((h1 - h1.T).abs().max())
+    big = float(h1.abs().max())
+    for g0 in range(0, chol.shape[0], 256):  # in slices: no (G, N, N) temporary
+        c = chol[g0:g0 + 256]
+        asym = max(asym, float((c - c.transpose(1, 2)).abs().max()))
+        big = max(big, float(c.abs().max()))
+    if asym > 1e-12 * big:
+        raise ValueError("the CISD measurement needs symmetric h1 and Cholesky vectors (real orbitals)")
 
 
 def slice_meas_ctx_chol(ctx: CisdMeasCtx, nchol_keep: int | None) -> CisdMeasCtx:

@@ -158,8 +158,8 @@ def make_cisd_trial_ops(sys: System, memory_mode: Literal["high", "low"] = "high
 
 def make_cisd_trial_data(data: dict, sys: System) -> CisdTrial:
     """trot/trial/cisd.py:203-213."""
-    return CisdTrial(ci1=data["ci1"], ci2=data["ci2"], nocc_t_core=int(np.asarray(data.get("nocc_t_core", 0))),
-                     nvir_t_outer=int(np.asarray(data.get("nvir_t_outer", 0))))
+    return CisdTrial(ci1=data["ci1"], ci2=data["ci2"], nocc_t_core=int(data.get("nocc_t_core", 0)),
+                     nvir_t_outer=int(data.get("nvir_t_outer", 0)))
 
 
 def slice_trial_level(trial: CisdTrial, nvir_keep: int | None) -> CisdTrial:

@@ -134,6 +134,26 @@ int trot_local_energy(const double* rot_chol_a, const double* rot_chol_b, const 
 int trot_block_energy(const double* energy, double* weights, double* scalars, double* out, int W, double dt,
                       double shift_ema, void* stream);
 
+/* Block observables.  rdm1: out[w, spin] (out is (W, n_spin_out, N, N) complex) = GT[w] C^T, the mixed
+ * one-body density matrix w (C^T w)^-1 C^T of a real trial C (N x n) and the walker's transposed half
+ * Green's function GT (W, N, n) complex.
+ * Replaces: trot/meas/rhf.py:191-216 (rdm1_kernel_rw / _uw), trot/meas/uhf.py:97-110 (rdm1_kernel_uw). */
+int trot_rdm1_from_green(const double* green_t, const double* trial, int W, int N, int n, int n_spin_out, int spin,
+                         double* out, void* stream);
+
+/* Density correlations from dm (W, 2, N, N) complex: out (W, 3, N, N) = (uu, ud, dd) with
+ * uu_ij = na_i na_j - ga_ij ga_ji + delta_ij na_i, ud_ij = na_i nb_j, n = diag(g).
+ * Replaces: trot/meas/uhf.py:129-144 (_density_corr_from_greens), rhf.py:237-261. */
+int trot_density_corr(const double* dm, int W, int N, double* out, void* stream);
+
+/* Weighted block mean of per-walker samples x (W, L) doubles (complex samples as 2 doubles each):
+ * out[l] = sum_w weights[w] x[w, l] / *wsum, or 0 when *wsum == 0 (wsum is a device scalar, e.g. out[1] of
+ * trot_block_energy).  Fixed summation order: the same bits on every call.
+ * Replaces: trot/prop/blocks.py:259-268. */
+size_t trot_weighted_mean_workspace(int W, long L);
+int trot_weighted_mean(const double* x, const double* weights, const double* wsum, int W, long L, double* out,
+                       void* ws, size_t ws_bytes, void* stream);
+
 /* In place Q factor of w with real positive diag(R) (two Cholesky-QR passes).
  * Replaces: _qr / orthonormalize (trot/walkers.py:137-179). */
 size_t trot_orthonormalize_workspace(int W, int N, int n);

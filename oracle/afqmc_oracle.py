@@ -213,6 +213,39 @@ def half_green(w: np.ndarray, c: np.ndarray) -> np.ndarray:
     return np.linalg.solve(np.swapaxes(m, 1, 2), np.swapaxes(w, 1, 2))
 
 
+def rdm1_rw(w: np.ndarray, c: np.ndarray) -> np.ndarray:
+    """trot/meas/rhf.py:191-200: g = G_half^T C^H, the same for both spins; (W, 2, N, N)."""
+    g = np.swapaxes(half_green(w, c), 1, 2) @ c.conj().T
+    return np.stack([g, g], axis=1)
+
+
+def rdm1_uw(w: tuple, ca: np.ndarray, cb: np.ndarray | None = None) -> np.ndarray:
+    """trot/meas/rhf.py:203-216 (cb = ca), trot/meas/uhf.py:97-110; (W, 2, N, N)."""
+    cb = ca if cb is None else cb
+    ga = np.swapaxes(half_green(w[0], ca), 1, 2) @ ca.conj().T
+    gb = np.swapaxes(half_green(w[1], cb), 1, 2) @ cb.conj().T
+    return np.stack([ga, gb], axis=1)
+
+
+def density_corr_from_rdm1(dm: np.ndarray) -> np.ndarray:
+    """trot/meas/uhf.py:129-144 per walker: (W, 3, N, N) = (uu, ud, dd) from dm (W, 2, N, N)."""
+    ga, gb = dm[:, 0], dm[:, 1]
+    na = np.diagonal(ga, axis1=1, axis2=2)
+    nb = np.diagonal(gb, axis1=1, axis2=2)
+    eye = np.eye(ga.shape[1])
+    uu = na[:, :, None] * na[:, None, :] - ga * np.swapaxes(ga, 1, 2) + na[:, :, None] * eye
+    dd = nb[:, :, None] * nb[:, None, :] - gb * np.swapaxes(gb, 1, 2) + nb[:, :, None] * eye
+    ud = na[:, :, None] * nb[:, None, :]
+    return np.stack([uu, ud, dd], axis=1)
+
+
+def weighted_block_mean(samples: np.ndarray, weights: np.ndarray) -> np.ndarray:
+    """trot/prop/blocks.py:259-268: sum_w weights_w samples_w / sum_w weights_w, 0 when the sum is 0."""
+    w_sum = np.sum(weights)
+    num = np.tensordot(weights, samples, axes=(0, 0))
+    return np.zeros_like(num) if w_sum == 0 else num / w_sum
+
+
 def force_bias_rw(w: np.ndarray, ctx: RhfMeasCtx, c: np.ndarray) -> np.ndarray:
     """trot/meas/rhf.py:119-125: 2 * rot_chol_flat @ vec(G_half)."""
     g = half_green(w, c)
@@ -397,6 +430,7 @@ class RhfOps:
     force_bias: Callable[[Any], np.ndarray]
     energy: Callable[[Any], np.ndarray]
     walker_kind: str = "restricted"
+    observables: dict = field(default_factory=dict)
 
 
 def make_rhf_ops(h0: float, h1: np.ndarray, chol: np.ndarray, c: np.ndarray, walker_kind: str = "restricted",
@@ -409,6 +443,8 @@ def make_rhf_ops(h0: float, h1: np.ndarray, chol: np.ndarray, c: np.ndarray, wal
             force_bias=lambda w: force_bias_rw(w, ctx, c),
             energy=lambda w: energy_rw(w, h0, ctx, c),
             walker_kind="restricted",
+            observables={"rdm1": lambda w: rdm1_rw(w, c),
+                         "density_corr": lambda w: density_corr_from_rdm1(rdm1_rw(w, c))},
         )
     if walker_kind == "unrestricted":
         return RhfOps(
@@ -416,6 +452,8 @@ def make_rhf_ops(h0: float, h1: np.ndarray, chol: np.ndarray, c: np.ndarray, wal
             force_bias=lambda w: force_bias_uw(w, ctx, c),
             energy=lambda w: energy_uw(w, h0, ctx, c),
             walker_kind="unrestricted",
+            observables={"rdm1": lambda w: rdm1_uw(w, c),
+                         "density_corr": lambda w: density_corr_from_rdm1(rdm1_uw(w, c))},
         )
     raise ValueError(f"unknown walker_kind: {walker_kind}")
 
@@ -428,6 +466,8 @@ def make_uhf_ops(h0: float, h1: np.ndarray, chol: np.ndarray, ca: np.ndarray, cb
         force_bias=lambda w: force_bias_uhf(w, ctx, ca, cb),
         energy=lambda w: energy_uhf(w, h0, ctx, ca, cb),
         walker_kind="unrestricted",
+        observables={"rdm1": lambda w: rdm1_uw(w, ca, cb),
+                     "density_corr": lambda w: density_corr_from_rdm1(rdm1_uw(w, ca, cb))},
     )
 
 
@@ -521,10 +561,12 @@ def stochastic_reconfiguration(w, weights: np.ndarray, zeta: float, walker_kind:
 # ---------------------------------------------------------------------------
 
 
-def block(state: PropState, fields_seq, zeta: float, ctx: PropCtx, ops: RhfOps, params: QmcParams):
+def block(state: PropState, fields_seq, zeta: float, ctx: PropCtx, ops: RhfOps, params: QmcParams,
+          observable_names: tuple = ()):
     """trot/prop/blocks.py:195-287 with pre-drawn fields (n_prop_steps, W, G) and SR zeta.
 
-    Returns (state, {"energy": e_block, "weight": w_sum, "e_samples": ..., "sr_idx": ...}).
+    Returns (state, {"energy": e_block, "weight": w_sum, "e_samples": ..., "sr_idx": ...,
+    "observables": {name: weighted block mean}}).
     """
     for s in range(params.n_prop_steps):
         state = afqmc_step(state, fields_seq[s], ctx, ops, params)
@@ -546,11 +588,14 @@ def block(state: PropState, fields_seq, zeta: float, ctx: PropCtx, ops: RhfOps, 
     alpha = params.shift_ema
     state.weights = weights
     state.e_estimate = float((1.0 - alpha) * state.e_estimate + alpha * e_block)
+    obs = {name: weighted_block_mean(ops.observables[name](state.walkers), weights)  # blocks.py:259-268
+           for name in observable_names}
     w_sr, weights_sr, idx = stochastic_reconfiguration(state.walkers, state.weights, zeta, ops.walker_kind)
     state.walkers = w_sr
     state.weights = weights_sr
     state.overlaps = ops.overlap(w_sr)  # blocks.py:273-276
-    return state, {"energy": float(e_block), "weight": float(w_sum), "e_samples": e_samples, "sr_idx": idx}
+    return state, {"energy": float(e_block), "weight": float(w_sum), "e_samples": e_samples, "sr_idx": idx,
+                   "observables": obs}
 
 
 # ---------------------------------------------------------------------------

@@ -269,4 +269,5 @@ def make_ops(h0: float, h1: np.ndarray, chol: np.ndarray, trial: CisdTrial):
 
     ctx = build_meas_ctx(chol, trial)
     return RhfOps(overlap=lambda w: overlap(w, trial), force_bias=lambda w: force_bias(w, chol, ctx, trial),
-                  energy=lambda w: energy(w, h0, h1, chol, ctx, trial), walker_kind="restricted")
+                  energy=lambda w: energy(w, h0, h1, chol, ctx, trial), walker_kind="restricted",
+                  observables={"rdm1": lambda w: rdm1(w, trial)})

@@ -155,9 +155,11 @@ def test_block_matches_oracle(gpu):
     seed, ctr = st.rng_key
     zeta = float(pctx.fields.uniform((seed, ctr + params.n_prop_steps))[0])
     st, obs = blocks.block(st, sys=sys, params=params, ham_data=ham, trial_data=tr, trial_ops=tops, meas_ops=mops,
-                           meas_ctx=mctx, prop_ops=pops, prop_ctx=pctx,
+                           meas_ctx=mctx, prop_ops=pops, prop_ctx=pctx, observable_names=("rdm1",),
                            fields_seq=[torch.from_numpy(f).cuda() for f in fseq])
-    ref, robs = O.block(ref, fseq, zeta, ref_ctx, ref_ops, O.QmcParams(dt=0.01, n_walkers=W, n_prop_steps=3))
+    ref, robs = O.block(ref, fseq, zeta, ref_ctx, ref_ops, O.QmcParams(dt=0.01, n_walkers=W, n_prop_steps=3),
+                        observable_names=("rdm1",))
+    assert rel(obs.observables["rdm1"].cpu().numpy(), robs["observables"]["rdm1"]) < TOL
     assert abs(obs.scalars["energy"].item() - robs["energy"]) < TOL * abs(robs["energy"])
     assert abs(obs.scalars["weight"].item() - robs["weight"]) < TOL * abs(robs["weight"])
     assert rel(st.walkers.cpu().numpy(), ref.walkers) < TOL

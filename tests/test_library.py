@@ -154,6 +154,15 @@ def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
     assert h.trot_update_problem(d, None, d, d, d, d, d, d, 4, 0.01, 0.0, 0.0, 1e-3, 100.0, 0.1, udesc) == 0
     assert h.trot_update_grouped(d, 0, None) == -1
 
+    # block observables (error paths only)
+    assert h.trot_rdm1_from_green(None, d, 4, 8, 2, 2, 0, d, None) == -1
+    assert h.trot_rdm1_from_green(d, d, 4, 8, 2, 2, 2, d, None) == -1  # spin slot out of range
+    assert h.trot_rdm1_from_green(d, d, 4, 8, 9, 2, 0, d, None) == -1  # nocc > norb
+    assert h.trot_rdm1_from_green(d, d, 4, 80, 65, 2, 0, d, None) == -2  # nocc > 64
+    assert h.trot_density_corr(d, 0, 8, d, None) == -1
+    assert h.trot_weighted_mean(d, d, d, 4, 10, d, d, 0, None) == -1  # workspace too small
+    assert h.trot_weighted_mean(d, d, None, 4, 10, d, d, 1 << 20, None) == -1
+
 
 def test_workspace_queries_are_sizes():
     from trot_amd import _lib

@@ -56,6 +56,10 @@ _SIGS = {
     "trot_local_energy_workspace": (_sz, [_i, _i]),
     "trot_local_energy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _d, _vp, _vp, _sz, _vp]),
     "trot_block_energy": (_i, [_vp, _vp, _vp, _vp, _i, _d, _d, _vp]),
+    "trot_rdm1_from_green": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
+    "trot_density_corr": (_i, [_vp, _i, _i, _vp, _vp]),
+    "trot_weighted_mean_workspace": (_sz, [_i, _l]),
+    "trot_weighted_mean": (_i, [_vp, _vp, _vp, _i, _l, _vp, _vp, _sz, _vp]),
     "trot_orthonormalize_workspace": (_sz, [_i, _i, _i]),
     "trot_orthonormalize": (_i, [_vp, _i, _i, _i, _vp, _sz, _vp]),
     "trot_orthogonalize": (_i, [_vp, _i, _i, _i, _vp, _vp, _sz, _vp]),
@@ -476,6 +480,45 @@ def block_energy(energy: torch.Tensor, weights: torch.Tensor, scalars: torch.Ten
     out = torch.empty((2,), dtype=F64, device=energy.device)
     _check(lib().trot_block_energy(_p(energy), _p(weights), _p(scalars), _p(out), W, float(dt), float(shift_ema),
                                    _stream()), "trot_block_energy")
+    return out
+
+
+def rdm1_from_green(green_t: torch.Tensor, trial: torch.Tensor, out: torch.Tensor, spin: int) -> torch.Tensor:
+    """out[:, spin] <- green_t @ trial^T for each walker (out: (W, S, N, N) complex128)."""
+    W, N, n = green_t.shape
+    _req(green_t, "green_t", C128, (W, N, n))
+    _req(trial, "trial", F64, (N, n))
+    if out.dim() != 4:
+        raise ValueError(f"out must be (W, S, N, N), got {tuple(out.shape)}")
+    _req(out, "out", C128, (W, out.shape[1], N, N))
+    _check(lib().trot_rdm1_from_green(_p(green_t), _p(trial), W, N, n, out.shape[1], int(spin), _p(out), _stream()),
+           "trot_rdm1_from_green")
+    return out
+
+
+def density_corr(dm: torch.Tensor) -> torch.Tensor:
+    """(W, 3, N, N) complex128 density correlations (uu, ud, dd) from dm (W, 2, N, N)."""
+    W, S, N, _ = dm.shape
+    _req(dm, "dm", C128, (W, 2, N, N))
+    out = torch.empty((W, 3, N, N), dtype=C128, device=dm.device)
+    _check(lib().trot_density_corr(_p(dm), W, N, _p(out), _stream()), "trot_density_corr")
+    return out
+
+
+def weighted_mean(x: torch.Tensor, weights: torch.Tensor, wsum: torch.Tensor) -> torch.Tensor:
+    """sum_w weights[w] x[w] / wsum (0 when wsum == 0), of shape x.shape[1:]; x float64 or complex128."""
+    W = x.shape[0]
+    if x.dtype not in (F64, C128) or not x.is_contiguous():
+        raise ValueError("x must be a contiguous float64 or complex128 tensor")
+    _req(weights, "weights", F64, (W,))
+    if wsum.dtype != F64 or wsum.numel() != 1:
+        raise ValueError("wsum must be one float64 element")
+    L = x.numel() // W * (2 if x.dtype == C128 else 1)
+    out = torch.empty(x.shape[1:], dtype=x.dtype, device=x.device)
+    lb = lib()
+    ws = _ws(lb.trot_weighted_mean_workspace(W, L), x.device)
+    _check(lb.trot_weighted_mean(_p(x), _p(weights), _p(wsum), W, L, _p(out), _p(ws), ws.numel(), _stream()),
+           "trot_weighted_mean")
     return out
 
 

@@ -16,7 +16,8 @@ from .prop import PropState
 
 @dataclass
 class BlockObs:
-    """trot/prop/blocks.py (BlockObs): scalars {"energy", "weight"} as 0-dim GPU tensors."""
+    """trot/prop/blocks.py (BlockObs): scalars {"energy", "weight"} as 0-dim GPU tensors; observables
+    {name: weighted block mean} as GPU tensors."""
 
     scalars: dict
     observables: dict = field(default_factory=dict)
@@ -25,13 +26,14 @@ class BlockObs:
 def block(state: PropState, *, sys: System, params, ham_data, trial_data, trial_ops: TrialOps, meas_ops: MeasOps,
           meas_ctx, prop_ops, prop_ctx, sr_fn: Callable = wk.stochastic_reconfiguration,
           observable_names: tuple = (), fields_seq=None) -> tuple[PropState, BlockObs]:
-    """Steps, QR, overlap recompute, energy, clamp / weighted mean / EMA, SR, overlap recompute.
+    """Steps, QR, overlap recompute, energy, clamp / weighted mean / EMA, block observables, SR, overlap
+    recompute.  Each name of ``observable_names`` is a MeasOps observable; its block value is the mean of the
+    per-walker samples weighted by the block's weights (0 when they sum to 0).
 
     ``fields_seq`` (optional) is a sequence of n_prop_steps (W, G) field tensors
     that replaces the provider, for step-by-step comparisons.
     """
-    for name in observable_names:
-        raise KeyError(f"missing requested observable '{name}'. available: []")
+    obs_kernels = {name: meas_ops.require_observable(name) for name in observable_names}
     for s in range(params.n_prop_steps):  # blocks.py:213-228
         f = None if fields_seq is None else fields_seq[s]
         state = prop_ops.step(state, params=params, ham_data=ham_data, trial_data=trial_data, trial_ops=trial_ops,
@@ -43,6 +45,16 @@ def block(state: PropState, *, sys: System, params, ham_data, trial_data, trial_
     weights = state.weights.clone()
     scal = torch.stack([state.pop_control_ene_shift, state.e_estimate]).to(torch.float64).contiguous()
     out = _lib.block_energy(e_samples, weights, scal, params.dt, params.shift_ema)  # blocks.py:240-257
+    obs_means = {}
+    for name, kernel in obs_kernels.items():  # blocks.py:259-268, weights with non-finite energies zeroed
+        if name in getattr(engine, "observables", ()):  # from the Green's functions already at hand
+            samples = engine.observable_from_green(name, greens, meas_ctx, trial_data)
+        else:
+            samples = kernel(walkers_new, ham_data, meas_ctx, trial_data)
+        samples = torch.as_tensor(samples, device=weights.device)
+        if samples.dtype not in (torch.float64, torch.complex128):
+            samples = samples.to(torch.complex128 if samples.is_complex() else torch.float64)
+        obs_means[name] = _lib.weighted_mean(samples.contiguous(), weights, out[1:2])
     zeta, key = prop_ctx.fields.uniform(state.rng_key)  # blocks.py:270-271
     if sr_fn is wk.stochastic_reconfiguration:
         idx, weights_sr = wk.sr_indices(weights, zeta)
@@ -64,4 +76,4 @@ def block(state: PropState, *, sys: System, params, ham_data, trial_data, trial_
         node_encounters=state.node_encounters,
         green=greens_sr,
     )
-    return state, BlockObs(scalars={"energy": out[0], "weight": out[1]}, observables={})
+    return state, BlockObs(scalars={"energy": out[0], "weight": out[1]}, observables=obs_means)

@@ -325,6 +325,14 @@ class CisdEngine:
         gam = _lib.cisd_gamma(greens[0], self._ps(greens, meas_ctx), meas_ctx.trial.nocc_t_core, packed=False)
         return torch.stack([gam, gam], dim=1)
 
+    observables = (o_rdm1,)  # names observable_from_green evaluates
+
+    def observable_from_green(self, name: str, greens, meas_ctx: CisdMeasCtx, trial_data=None) -> torch.Tensor:
+        """The block observables of the CISD trial (trot/meas/cisd.py:850): "rdm1" only."""
+        if name != o_rdm1:
+            raise KeyError(f"missing requested observable '{name}'. available: [{o_rdm1}]")
+        return self.rdm1_from_green(greens, meas_ctx)
+
 
 def _h0(ham_data) -> float:
     h0 = ham_data.h0

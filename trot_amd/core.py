@@ -21,6 +21,7 @@ WalkerKind = Literal["restricted", "unrestricted", "generalized"]
 k_energy = "energy"
 k_force_bias = "force_bias"
 o_rdm1 = "rdm1"
+o_density_corr = "density_corr"
 
 
 @dataclass(frozen=True)

@@ -13,13 +13,15 @@ from typing import Any, NamedTuple
 import numpy as np
 
 from .blocks import block as default_block
-from .stats import blocking_analysis_ratio, reject_outliers
+from .stats import blocking_analysis_ratio, jackknife_ratios, rebin_observable, reject_outliers
 
 print = partial(print, flush=True)
 
 
 class QmcResult(NamedTuple):
-    """trot/driver.py:33-40 (observables are not part of this path: always empty)."""
+    """trot/driver.py:33-40.  block_observables: name -> (equilibration + kept sampling blocks, *shape) host
+    arrays; observable_means / observable_stderrs: name -> weighted mean over the kept sampling blocks and
+    its jackknife error at the energy's B* (NaN when that leaves fewer than 2 super-blocks)."""
 
     mean_energy: float
     stderr_energy: float | None
@@ -47,30 +49,40 @@ def run_qmc(*, sys, params, ham_data, trial_data, meas_ops, trial_ops, prop_ops,
                                          meas_ops=meas_ops, params=params)
     log = print if verbose else (lambda *a, **k: None)
 
+    names = tuple(observable_names)
+    extra = {"observable_names": names} if names else {}
+
     def run_blocks(st, n):
         es, ws = [], []
+        os_ = {name: [] for name in names}
         for _ in range(n):
             st, obs = block_fn(st, sys=sys, params=params, ham_data=ham_data, trial_data=trial_data,
                                trial_ops=trial_ops, meas_ops=meas_ops, meas_ctx=meas_ctx, prop_ops=prop_ops,
-                               prop_ctx=prop_ctx)
+                               prop_ctx=prop_ctx, **extra)
             es.append(obs.scalars["energy"])
             ws.append(obs.scalars["weight"])
+            for name in names:
+                os_[name].append(obs.observables[name])
         import torch
 
         e = torch.stack(es).cpu().numpy() if es else np.zeros(0)
         w = torch.stack(ws).cpu().numpy() if ws else np.zeros(0)
-        return st, e, w
+        o = {name: torch.stack(v).cpu().numpy() for name, v in os_.items() if v}
+        return st, e, w, o
 
     t0 = time.perf_counter()
     block_e_eq = [float(state.e_estimate.item())]
     block_w_eq = [float(state.weights.sum().item())]
+    obs_eq = {name: [] for name in names}
     log("\nEquilibration:\n")
     chunk = max(1, params.n_eql_blocks // 5)
     for start in range(0, params.n_eql_blocks, chunk):
         n = min(chunk, params.n_eql_blocks - start)
-        state, e, w = run_blocks(state, n)
+        state, e, w, o = run_blocks(state, n)
         block_e_eq.extend(e.tolist())
         block_w_eq.extend(w.tolist())
+        for name in names:
+            obs_eq[name].append(o[name])
         log(f"[eql {start + n:4d}/{params.n_eql_blocks}]  {float(np.mean(e * w) / np.mean(w)):14.10f}  "
             f"{float(np.mean(w)):12.6e}  {int(state.node_encounters.item()):10d}  {time.perf_counter() - t0:8.1f}")
 
@@ -78,13 +90,16 @@ def run_qmc(*, sys, params, ham_data, trial_data, meas_ops, trial_ops, prop_ops,
     target_error = 0.0 if target_error is None else target_error
     block_e_s: list[float] = []
     block_w_s: list[float] = []
+    obs_s = {name: [] for name in names}
     chunk = max(1, params.n_blocks // 10)
     t_mark = time.perf_counter()
     for start in range(0, params.n_blocks, chunk):
         n = min(chunk, params.n_blocks - start)
-        state, e, w = run_blocks(state, n)
+        state, e, w, o = run_blocks(state, n)
         block_e_s.extend(e.tolist())
         block_w_s.extend(w.tolist())
+        for name in names:
+            obs_s[name].append(o[name])
         stats = blocking_analysis_ratio(np.asarray(block_e_s), np.asarray(block_w_s), print_q=False)
         se = stats["se_star"]
         dt_blk = (time.perf_counter() - t_mark) / n
@@ -99,13 +114,45 @@ def run_qmc(*, sys, params, ham_data, trial_data, meas_ops, trial_ops, prop_ops,
     data = np.column_stack((block_e_s, block_w_s))
     clean, keep = reject_outliers(data, obs=0)
     log(f"\nRejected {data.shape[0] - clean.shape[0]} outlier blocks.")
+    if verbose:
+        log("\nFinal blocking analysis:")
     stats = blocking_analysis_ratio(clean[:, 0], clean[:, 1], print_q=verbose)
+    w_s = clean[:, 1]
+    b_star = stats["B_star"]
+    block_obs, means, errs = {}, {}, {}
+    for name in names:  # trot/driver.py:352-400
+        eq = np.concatenate(obs_eq[name], axis=0) if obs_eq[name] else None
+        smp = np.concatenate(obs_s[name], axis=0)[keep] if obs_s[name] else None
+        if eq is None and smp is None:
+            block_obs[name] = np.zeros((0,))
+            means[name] = np.zeros((0,))
+            errs[name] = np.zeros((0,))
+            continue
+        block_obs[name] = np.concatenate([eq if eq is not None else smp[:0], smp if smp is not None else eq[:0]])
+        if smp is None:
+            means[name] = np.zeros((0,))
+            errs[name] = np.zeros((0,))
+            continue
+        means[name] = weighted_block_mean(smp, w_s)
+        errs[name] = np.full(smp.shape[1:], np.nan)
+        if b_star is not None and b_star >= 1:
+            num, den = rebin_observable(smp, w_s, b_star)
+            if num.shape[0] >= 2:
+                errs[name] = jackknife_ratios(num, den)[1]
     return QmcResult(
         mean_energy=stats["mu"],
         stderr_energy=stats["se_star"],
         block_energies=np.concatenate([block_e_eq, clean[:, 0]]),
         block_weights=np.concatenate([block_w_eq, clean[:, 1]]),
-        block_observables={},
-        observable_means={},
-        observable_stderrs={},
+        block_observables=block_obs,
+        observable_means=means,
+        observable_stderrs=errs,
     )
+
+
+def weighted_block_mean(values: np.ndarray, weights: np.ndarray) -> np.ndarray:
+    """trot/driver.py:56-61: sum_b w_b values_b / sum_b w_b over blocks, 0 when the weights sum to 0."""
+    values = np.asarray(values)
+    weights = np.asarray(weights).ravel()
+    num = np.tensordot(weights, values, axes=(0, 0))
+    return np.zeros_like(num) if weights.sum() == 0 else num / weights.sum()

@@ -50,7 +50,7 @@ def _copy_state(dst: PropState, src: PropState) -> None:
 
 
 class _Recorded:
-    """Static state buffers, a device counter and one recorded HIP graph of ``run(static) -> (state, scalars)``.
+    """Static state buffers, a device counter and one recorded HIP graph of ``run(static) -> (state, scalars, arrays)``.
 
     The first call runs ``run`` eagerly (on a side stream, as graph capture requires) on a copy of the
     given state; its result fixes the structure of the static buffers and is the first call's result.
@@ -62,12 +62,15 @@ class _Recorded:
         self.last = None
         self.rng = None
         self.obs = None
+        self.arrays = ()
         self.replays = 0
 
-    def _store(self, state_out, scalars):
+    def _store(self, state_out, scalars, arrays=()):
         _copy_state(self.static, state_out)
         if scalars:
             self.obs.copy_(torch.stack([x.reshape(()) for x in scalars]))
+        for dst, src in zip(self.arrays, arrays):
+            dst.copy_(src)
 
     def _load(self, state):
         _copy_state(self.static, state)
@@ -89,14 +92,15 @@ class _Recorded:
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
-                out, scalars = run(_static_copy(state), args)
+                out, scalars, arrays = run(_static_copy(state), args)
                 self.static = _static_copy(out)
                 if scalars:
                     self.obs = torch.stack([x.reshape(()) for x in scalars]).to(torch.complex128 if any(
                         x.is_complex() for x in scalars) else torch.float64).contiguous()
+                self.arrays = tuple(a.detach().clone().contiguous() for a in arrays)
             torch.cuda.current_stream().wait_stream(side)
             torch.cuda.synchronize()
-            del out, scalars
+            del out, scalars, arrays
             self.graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(self.graph):
                 self._store(*run(self.static, args))
@@ -127,7 +131,7 @@ class GraphStep(_Recorded):
             raise ValueError("GraphStep draws its fields on the device; fields is not supported")
 
         def run(st, args):
-            return self.prop_ops.step(st, **args), None
+            return self.prop_ops.step(st, **args), None, ()
 
         return self._run(state, kw, run, self.draws_per_step)
 
@@ -148,6 +152,7 @@ class GraphBlock(_Recorded):
         self.block_fn = block_fn
         self.draws_per_block = draws_per_block or (lambda params: params.n_prop_steps + 1)
         self.names = None
+        self.obs_names = ()
 
     @property
     def blocks_done(self) -> int:
@@ -157,12 +162,18 @@ class GraphBlock(_Recorded):
         if kw.pop("fields_seq", None) is not None:
             raise ValueError("GraphBlock draws its fields on the device; fields_seq is not supported")
 
+        names = tuple(kw.get("observable_names", ()))
+        if self.names is not None and names != self.obs_names:
+            raise ValueError(f"recorded with observables {self.obs_names}, called with {names}")
+
         def run(st, args):
             out, obs = self.block_fn(st, **args)
             if self.names is None:
                 self.names = list(obs.scalars)
-            return out, tuple(obs.scalars[k] for k in self.names)
+                self.obs_names = tuple(obs.observables)
+            return out, tuple(obs.scalars[k] for k in self.names), tuple(obs.observables[k] for k in self.obs_names)
 
         st = self._run(state, kw, run, self.draws_per_block(kw["params"]))
-        obs = BlockObs(scalars={k: self.obs[i].clone() for i, k in enumerate(self.names)}, observables={})
+        obs = BlockObs(scalars={k: self.obs[i].clone() for i, k in enumerate(self.names)},
+                       observables={k: self.arrays[i].clone() for i, k in enumerate(self.obs_names)})
         return st, obs

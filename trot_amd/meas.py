@@ -17,8 +17,8 @@ from dataclasses import dataclass, field
 import torch
 
 from . import _b3, _lib
-from .core import MeasOps, System, k_energy, k_force_bias
-from .trial import RhfTrial, UhfTrial, _batched, overlap, overlap_green
+from .core import MeasOps, System, k_energy, k_force_bias, o_density_corr, o_rdm1
+from .trial import RhfTrial, UhfTrial, _batched, _coeffs, overlap, overlap_green
 
 
 @dataclass(frozen=True)
@@ -131,6 +131,8 @@ class Engine:
     walkers) or "uu" (UHF trial, unrestricted walkers).
     """
 
+    observables = (o_rdm1, o_density_corr)  # names observable_from_green evaluates
+
     def __init__(self, kind: str, precision: str = "f64"):
         if precision not in ("f64", "f32", "bf16x3"):
             raise ValueError(f"unknown precision: {precision}")
@@ -158,6 +160,24 @@ class Engine:
 
     def overlap_green(self, walkers, trial_data):
         return overlap_green(walkers, trial_data, want_green=True)
+
+    def observable_from_green(self, name: str, greens, meas_ctx, trial_data) -> torch.Tensor:
+        """Per-walker block observables from the transposed half Green's functions.
+
+        "rdm1": (W, 2, N, N) complex, spin s = GT_s C_s^T = w_s (C_s^T w_s)^-1 C_s^T (restricted walkers
+        give the same matrix for both spins); trot/meas/rhf.py:191-216, trot/meas/uhf.py:97-110.
+        "density_corr": (W, 3, N, N) complex (uu, ud, dd) from the two spins' rdm1;
+        trot/meas/rhf.py:237-261, trot/meas/uhf.py:129-160.
+        """
+        if name not in (o_rdm1, o_density_corr):
+            raise KeyError(f"missing requested observable '{name}'. available: [{o_density_corr}, {o_rdm1}]")
+        ca, cb = _coeffs(trial_data)
+        g_a, g_b = greens
+        W, N, _ = g_a.shape
+        dm = torch.empty((W, 2, N, N), dtype=torch.complex128, device=g_a.device)
+        _lib.rdm1_from_green(g_a, ca, dm, 0)
+        _lib.rdm1_from_green(g_a if g_b is None else g_b, cb, dm, 1)
+        return dm if name == o_rdm1 else _lib.density_corr(dm)
 
     def force_bias_from_green(self, greens, meas_ctx) -> torch.Tensor:
         sa, sb = self.spins(meas_ctx)
@@ -229,6 +249,20 @@ def _make_kernels(engine: Engine):
     return force_bias_kernel, energy_kernel
 
 
+def _make_observables(engine: Engine) -> dict:
+    def kernel(name):
+        def obs_kernel(walker, ham_data, meas_ctx, trial_data):
+            w, squeeze = _batched(walker)
+            _, greens = engine.overlap_green(w, trial_data)
+            out = engine.observable_from_green(name, greens, meas_ctx, trial_data)
+            return out[0] if squeeze else out
+
+        obs_kernel.__name__ = f"{name}_kernel"
+        return obs_kernel
+
+    return {o_rdm1: kernel(o_rdm1), o_density_corr: kernel(o_density_corr)}
+
+
 def make_rhf_meas_ops(sys: System, memory_mode: str = "high", *, precision: str = "f64") -> MeasOps:
     """trot/meas/rhf.py:322-367 for restricted and unrestricted walkers.
 
@@ -251,7 +285,7 @@ def make_rhf_meas_ops(sys: System, memory_mode: str = "high", *, precision: str 
         overlap=overlap,
         build_meas_ctx=lambda ham_data, trial_data: build_meas_ctx(ham_data, trial_data, cfg),
         kernels={k_force_bias: fb, k_energy: en},
-        observables={},
+        observables=_make_observables(engine),
         engine=engine,
     )
 
@@ -269,6 +303,6 @@ def make_uhf_meas_ops(sys: System, *, precision: str = "f64") -> MeasOps:
         overlap=overlap,
         build_meas_ctx=build_uhf_meas_ctx,
         kernels={k_force_bias: fb, k_energy: en},
-        observables={},
+        observables=_make_observables(engine),
         engine=engine,
     )

@@ -84,4 +84,4 @@ def test_block_energy_derivative(gpu):
     lam = torch.tensor(0.0, dtype=torch.float64, requires_grad=True)
     e = diff.BlockEnergy.apply(lam, lambda x: res if x == 0.0 else run_at(x)[1])
     e.backward()
-    assert float(e) == res.energy and float(lam.grad) == res.denergy
+    assert e.item() == res.energy and lam.grad.item() == res.denergy

@@ -6,8 +6,10 @@ each through trot/prop/afqmc.py:82-144).  ``FragmentBatch.capture`` records one 
 fragment into a single HIP graph (torch.cuda.CUDAGraph on the ROCm stream) over static state buffers,
 with the step's auxiliary fields read from one flat device buffer; ``step`` copies the next fields into
 that buffer and replays the graph.  A step of all fragments is then one field copy and one graph launch
-instead of about thirty kernel launches per fragment, and the result is bit-identical to stepping the
-fragments one at a time: the replay runs the same kernels on the same data in the same order.
+instead of about thirty kernel launches per fragment.  For the FP64 step (the default when every fragment
+runs it) each operation of the step is moreover one grouped launch over all fragments, whose
+workgroups run every fragment's own problem.  Either way the result is bit-identical to stepping the
+fragments one at a time: every fragment sees the same kernels on the same data in the same order.
 
 Captured code must not synchronise with the host, allocate outside torch's graph pool or record memset
 nodes; the FP64 step path (trot_amd.prop.afqmc_step with given fields) satisfies all three.
@@ -91,7 +93,7 @@ def _groupable(f: Fragment) -> bool:
     ctx = f.prop_ctx
     eng = f.meas_ops.engine
     return (ctx.precision == "f64" and ctx.taylor_precision == "f64" and bool(ctx.low.get("symblk"))
-            and 0 >= ctx.walker_chunk and eng.precision == "f64" and getattr(eng, "kind", "") in ("rr", "ru", "uu")
+            and ctx.walker_chunk <= 0 and eng.precision == "f64" and getattr(eng, "kind", "") in ("rr", "ru", "uu")
             and f.sys.walker_kind in ("restricted", "unrestricted") and max(f.sys.nelec) <= _lib.MAX_NOCC)
 
 
@@ -278,32 +280,6 @@ class FragmentBatch:
         for dst, src in zip(_tensors(f.state), _tensors(new)):
             dst.copy_(src)
         f.state = f.state._replace(rng_key=new.rng_key)
-
-    def _phase(self, views, fn):
-        main = torch.cuda.current_stream()
-        for st, idx in zip(self.streams, self.branch):
-            st.wait_stream(main)
-            with torch.cuda.stream(st):
-                for i in idx:
-                    fn(i, views)
-        for st in self.streams:
-            main.wait_stream(st)
-
-    def _pre(self, i, views):
-        f, b = self.fragments[i], self.plan.per[i]
-        fb = f.meas_ops.engine.force_bias_from_green(f.state.green, f.meas_ctx)  # afqmc.py:95-98
-        _lib.field_shift(views[i], fb, f.prop_ctx.mf_shifts, f.prop_ctx.sqrt_dt,
-                         out=(b["xbar"], b["shift"], b["fbt"]))  # afqmc.py:99-103
-
-    def _post(self, i, views):
-        f, b = self.fragments[i], self.plan.per[i]
-        st, ctx, params = f.state, f.prop_ctx, f.params
-        det = b["det"]
-        ov_new = _lib.overlap_combine(det[0], det[1] if len(det) > 1 else None)  # afqmc.py:108-110
-        _lib.phaseless_update(st.weights, st.overlaps, ov_new, b["shift"], b["fbt"], b["scal"], st.node_encounters,
-                              ctx.dt, ctx.h0_prop, getattr(params, "weight_floor", 1.0e-3),
-                              getattr(params, "weight_cap", 100.0), getattr(params, "pop_control_damping", 0.1))
-        st.overlaps.copy_(ov_new)
 
     def _step_grouped(self, views):
         self.plan.launch_pre()  # force bias, slice sums, shifted fields

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import time
 from functools import partial
-from typing import Any, NamedTuple
+from typing import NamedTuple
 
 import numpy as np
 

@@ -27,7 +27,7 @@ import scipy.linalg
 import torch
 
 from . import _b3, _lib
-from .core import MeasOps, System, TrialOps, k_energy
+from .core import MeasOps, System, TrialOps
 from . import walkers as wk
 
 

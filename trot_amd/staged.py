@@ -21,7 +21,6 @@ NotImplementedError rather than guessing.  ``load_staged`` returns the reference
 from __future__ import annotations
 
 import json
-import struct
 from dataclasses import dataclass
 import mmap
 from pathlib import Path

@@ -169,7 +169,7 @@ class Runner:
 def main():
     args = parse()
     if args.gpus != 1:
-        sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (--gpus must be 1)")
+        sys.exit("bench.py measures one GPU only: trot_amd has no multi-GPU path (--gpus must be 1)")
     if args.calibrate_copy:
         calibrate_copy()
         return

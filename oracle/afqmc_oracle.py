@@ -1,6 +1,6 @@
 """CPU FP64 oracle for the phaseless AFQMC walker step and local energy.
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker, never the product:
+TEST INFRASTRUCTURE ONLY.  This module is the test oracle, never the product:
 only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import it.  The product path (``trot_amd``) never imports,
 calls or falls back to anything here.

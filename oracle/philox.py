@@ -1,6 +1,6 @@
 """CPU twin of the counter-based field stream (Philox4x64-10 + Box-Muller).
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker, never the product: only ``tests/``,
+TEST INFRASTRUCTURE ONLY.  This module is the test oracle, never the product: only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may import it.
 
 The reference draws its auxiliary fields and the SR comb offset from JAX's threefry stream

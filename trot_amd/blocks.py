@@ -1,10 +1,13 @@
-"""One propagation + measurement block (trot/prop/blocks.py:195-287) on the GPU."""
+"""One propagation + measurement block (trot/prop/blocks.py:195-287) on the GPU, and the per-block state
+files of trot/prop/blocks.py:66-192 (``make_block_state_logger`` and its loaders)."""
 
 from __future__ import annotations
 
 from dataclasses import dataclass, field
+from pathlib import Path
 from typing import Callable
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -77,3 +80,77 @@ def block(state: PropState, *, sys: System, params, ham_data, trial_data, trial_
         green=greens_sr,
     )
     return state, BlockObs(scalars={"energy": out[0], "weight": out[1]}, observables=obs_means)
+
+
+# ---------------------------------------------------------------------------
+# per-block state files (trot/prop/blocks.py:66-192)
+# ---------------------------------------------------------------------------
+
+
+def dump_prop_state_npz(state: PropState, path, *, compressed: bool = True) -> None:
+    """Write a PropState to one ``.npz`` file (trot/prop/blocks.py:66-98): walkers (``walkers`` or
+    ``walkers_0`` / ``walkers_1`` per spin), weights, overlaps, rng_key (seed, counter), the population
+    shift, e_estimate and the node count.  The Green's-function cache is not written: it is recomputed
+    from the walkers when needed."""
+    out = Path(path).expanduser().resolve()
+    out.parent.mkdir(parents=True, exist_ok=True)
+
+    def host(x):
+        return x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
+
+    arrays = {
+        "weights": host(state.weights),
+        "overlaps": host(state.overlaps),
+        "rng_key": np.asarray([int(k) for k in state.rng_key], dtype=np.int64),
+        "pop_control_ene_shift": host(state.pop_control_ene_shift),
+        "e_estimate": host(state.e_estimate),
+        "node_encounters": host(state.node_encounters),
+    }
+    if isinstance(state.walkers, tuple):
+        for i, w in enumerate(state.walkers):
+            arrays[f"walkers_{i}"] = host(w)
+    else:
+        arrays["walkers"] = host(state.walkers)
+    (np.savez_compressed if compressed else np.savez)(out, **arrays)
+
+
+def load_prop_state_npz(path, device="cuda") -> PropState:
+    """A PropState written by ``dump_prop_state_npz`` (trot/prop/blocks.py:101-121), its tensors on
+    ``device``; ``green`` is None."""
+    with np.load(Path(path).expanduser().resolve()) as data:
+        def dev(name):
+            return torch.from_numpy(np.array(data[name], order="C")).to(device)  # keeps 0-d arrays 0-d
+
+        if "walkers" in data.files:
+            walkers = dev("walkers")
+        else:
+            walkers = tuple(dev(k) for k in sorted(k for k in data.files if k.startswith("walkers_")))
+        return PropState(walkers=walkers, weights=dev("weights"), overlaps=dev("overlaps"),
+                         rng_key=tuple(int(k) for k in data["rng_key"]),
+                         pop_control_ene_shift=dev("pop_control_ene_shift"), e_estimate=dev("e_estimate"),
+                         node_encounters=dev("node_encounters"))
+
+
+def load_all_prop_states_npz(directory, *, prefix: str = "block_state", device="cuda") -> list:
+    """Every ``{prefix}_*.npz`` of ``directory``, in file-name order (trot/prop/blocks.py:124-131)."""
+    files = sorted(Path(directory).expanduser().resolve().glob(f"{prefix}_*.npz"))
+    return [load_prop_state_npz(f, device) for f in files]
+
+
+def make_block_state_logger(directory, *, base_block_fn: Callable | None = None, prefix: str = "block_state",
+                            start_index: int = 0, compressed: bool = True) -> Callable:
+    """A block function that runs ``base_block_fn`` (default ``block``) and writes the state at the end
+    of every block to ``{directory}/{prefix}_{index:05d}.npz`` (trot/prop/blocks.py:134-192).  Each file
+    is written after the block's kernels finish (a device-to-host copy per block)."""
+    base = block if base_block_fn is None else base_block_fn
+    out_dir = Path(directory).expanduser().resolve()
+    out_dir.mkdir(parents=True, exist_ok=True)
+    counter = [int(start_index)]
+
+    def logging_block_fn(state: PropState, **kw):
+        state, obs = base(state, **kw)
+        dump_prop_state_npz(state, out_dir / f"{prefix}_{counter[0]:05d}.npz", compressed=compressed)
+        counter[0] += 1
+        return state, obs
+
+    return logging_block_fn

@@ -395,68 +395,14 @@ struct VSrc {
   int hi_from;  // terms k >= hi_from: one bf16 pass on the hi planes of V and of the term
 };
 
-// w_out <- [E .] T_K(i sqrt(dt) V) [. E] w_in for W walkers (one chunk); buffers sized for them.
-int run_b3_chunk(const VSrc& v, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
-                 double sqrt_dt, int n_terms, bf16raw* const bufs[2], float* S, hipStream_t s) {
-  const int Np = pad_up(N, PB_BM), np_ = pad_up(n, PB_BN), nb = (N + 7) / 8;
-  if (!ep && n_terms == 0) {
-    if (w_out != w_in) HIPCHK(hipMemcpyAsync(w_out, w_in, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
-    return TROT_OK;
-  }
-  hipLaunchKernelGGL(split_transpose_w_kernel, dim3(Np / 64, np_ / 64, W), dim3(256), 0, s, w_in, bufs[0], N, n, Np,
-                     np_);
-  HIPCHK(hipGetLastError());
-  int cur = 0;
-  PropB3P P{};
-  P.vsym = v.c64;
-  P.vh = v.vh;
-  P.vl = v.vl;
-  P.vcols = symblk_count(nb) * 64;
-  P.ep = ep;
-  P.S = S;
-  P.W = W;
-  P.N = N;
-  P.n = n;
-  P.Np = Np;
-  P.np_ = np_;
-  P.nb = nb;
-  if (ep) {  // first one-body factor: w_out = w_in + (E - I) w_in, with its planes
-    P.tp = bufs[cur];
-    P.tn = bufs[cur ^ 1];
-    P.win = w_in;
-    P.w = w_out;
-    if (int rc = launch_prop<true>(P, s)) return rc;
-    cur ^= 1;
-  } else if (w_out != w_in) {
-    HIPCHK(hipMemcpyAsync(w_out, w_in, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
-  }
-  for (int k = 1; k <= n_terms; ++k) {
-    P.tp = bufs[cur];
-    P.first = k == 1;
-    P.last = k == n_terms;
-    P.tn = (P.last && !ep) ? nullptr : bufs[cur ^ 1];
-    // the next term, when it is one bf16 pass on the hi planes, never reads the lo planes of this output
-    P.tn_hi = !v.c64 && k < n_terms && k + 1 >= v.hi_from;
-    P.win = w_out;
-    P.w = w_out;
-    P.c = (float)(sqrt_dt / k);
-    const int rc = v.c64 ? launch_prop<false, 0>(P, s)
-                         : (k >= v.hi_from ? launch_prop<false, 2>(P, s) : launch_prop<false, 1>(P, s));
-    if (rc) return rc;
-    cur ^= 1;
-  }
-  if (ep) {  // second one-body factor on the planes of the propagated walker
-    P.tp = bufs[cur];
-    P.tn = nullptr;
-    P.tn_hi = 0;
-    P.win = w_out;
-    P.w = w_out;
-    if (int rc = launch_prop<true>(P, s)) return rc;
-  }
-  return TROT_OK;
-}
-
 // w_out <- [E .] T_K(i sqrt(dt) V) [. E] w_in; one-body factors when ep is non-null.
+//
+// The split of w_in into planes and both one-body factors run once over all walkers (E is shared, so
+// these launches are large and short-lived per walker); the Taylor series runs over walker chunks whose
+// V, term planes and partial sums (about 1.8 MB per walker at N = 512, n = 64) stay within the 256 MB
+// Infinity Cache across the whole series, while each launch still has at least 512 workgroups (two per
+// CU).  Measured at N = 512, n = 64, W = 2048 with the one-body factors inside the chunks: 128 walkers per
+// chunk 10.5 ms, all at once 11.9 ms, 64 per chunk 14.3 ms (fewer workgroups than CUs can hold).
 int run_b3(const VSrc& v, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
            double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, hipStream_t s) {
   const int Np = pad_up(N, PB_BM), np_ = pad_up(n, PB_BN), nb = (N + 7) / 8;
@@ -464,28 +410,77 @@ int run_b3(const VSrc& v, const bf16raw* ep, const double* w_in, double* w_out, 
   bf16raw* bufs[2] = {cv.take<bf16raw>((size_t)W * 4 * np_ * Np), cv.take<bf16raw>((size_t)W * 4 * np_ * Np)};
   float* S = cv.take<float>((size_t)W * Np * np_ * 2);
   if (!bufs[0] || !bufs[1] || !S) return fail(TROT_ERR_WORKSPACE, "propagation b3: workspace");
-  // Walker chunks whose V, term planes and partial sums (about 1.8 MB per walker at N = 512, n = 64)
-  // stay within the 256 MB Infinity Cache across the whole series, while each launch still has at least
-  // 512 workgroups (two per CU).  Measured at N = 512, n = 64, W = 2048: 128 walkers per chunk 10.5 ms,
-  // all at once 11.9 ms, 64 per chunk 14.3 ms (fewer workgroups than CUs can hold).
+  if (!ep && n_terms == 0) {
+    if (w_out != w_in) HIPCHK(hipMemcpyAsync(w_out, w_in, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
+    return TROT_OK;
+  }
+  hipLaunchKernelGGL(split_transpose_w_kernel, dim3(Np / 64, np_ / 64, W), dim3(256), 0, s, w_in, bufs[0], N, n, Np,
+                     np_);
+  HIPCHK(hipGetLastError());
   const long vcols = symblk_count(nb) * 64;
+  PropB3P base{};
+  base.vcols = vcols;
+  base.ep = ep;
+  base.N = N;
+  base.n = n;
+  base.Np = Np;
+  base.np_ = np_;
+  base.nb = nb;
+  int cur = 0;
+  if (ep) {  // first one-body factor, all walkers: w_out = w_in + (E - I) w_in, with its planes
+    PropB3P P = base;
+    P.W = W;
+    P.tp = bufs[0];
+    P.tn = bufs[1];
+    P.win = w_in;
+    P.w = w_out;
+    if (int rc = launch_prop<true>(P, s)) return rc;
+    cur = 1;
+  } else if (w_out != w_in) {
+    HIPCHK(hipMemcpyAsync(w_out, w_in, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
+  }
   const double per_walker = 8.0 * vcols + 2.0 * 4 * np_ * Np * 2 + 8.0 * Np * np_;
   const int tpw = (Np / PB_BM) * (np_ / PB_BN);
   int chunk = (int)(200e6 / per_walker);
   chunk = chunk < (512 + tpw - 1) / tpw ? (512 + tpw - 1) / tpw : chunk;
   chunk = (chunk + 7) / 8 * 8;
   if (chunk > W) chunk = W;
+  const size_t plane_w = (size_t)4 * np_ * Np;  // bf16 words of one walker's planes
+  int cur_end = cur;
   for (int c0 = 0; c0 < W; c0 += chunk) {
     const int wc = W - c0 < chunk ? W - c0 : chunk;
-    bf16raw* cb[2] = {bufs[0] + (size_t)c0 * 4 * np_ * Np, bufs[1] + (size_t)c0 * 4 * np_ * Np};
-    const size_t wo = (size_t)c0 * N * n * 2;
-    VSrc vc = v;
-    if (v.c64) vc.c64 = v.c64 + (size_t)c0 * vcols * 2;
-    if (v.vh) vc.vh = v.vh + (size_t)c0 * vcols;
-    if (v.vl) vc.vl = v.vl + (size_t)c0 * vcols;
-    if (int rc = run_b3_chunk(vc, ep, w_in + wo, w_out + wo, wc, N, n, sqrt_dt, n_terms, cb,
-                              S + (size_t)c0 * Np * np_ * 2, s))
-      return rc;
+    PropB3P P = base;
+    P.W = wc;
+    if (v.c64) P.vsym = v.c64 + (size_t)c0 * vcols * 2;
+    if (v.vh) P.vh = v.vh + (size_t)c0 * vcols;
+    if (v.vl) P.vl = v.vl + (size_t)c0 * vcols;
+    P.S = S + (size_t)c0 * Np * np_ * 2;
+    P.w = w_out + (size_t)c0 * N * n * 2;
+    P.win = P.w;
+    int c = cur;
+    for (int k = 1; k <= n_terms; ++k) {
+      P.tp = bufs[c] + c0 * plane_w;
+      P.first = k == 1;
+      P.last = k == n_terms;
+      P.tn = (P.last && !ep) ? nullptr : bufs[c ^ 1] + c0 * plane_w;
+      // the next term, when it is one bf16 pass on the hi planes, never reads the lo planes of this output
+      P.tn_hi = !v.c64 && k < n_terms && k + 1 >= v.hi_from;
+      P.c = (float)(sqrt_dt / k);
+      const int rc = v.c64 ? launch_prop<false, 0>(P, s)
+                           : (k >= v.hi_from ? launch_prop<false, 2>(P, s) : launch_prop<false, 1>(P, s));
+      if (rc) return rc;
+      c ^= 1;
+    }
+    cur_end = c;
+  }
+  if (ep) {  // second one-body factor, all walkers, on the planes of the propagated walkers
+    PropB3P P = base;
+    P.W = W;
+    P.tp = bufs[cur_end];
+    P.tn = nullptr;
+    P.win = w_out;
+    P.w = w_out;
+    if (int rc = launch_prop<true>(P, s)) return rc;
   }
   return TROT_OK;
 }

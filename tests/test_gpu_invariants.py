@@ -7,6 +7,8 @@
 * Walker order: permuting the walkers (with their fields and weights) permutes the walkers, overlaps and
   weights after a step in the same way, bit for bit; the population shift, a sum over walkers, agrees to
   rounding; so does the block energy.
+* Walker kind (tests/test_uhf.py:131-170): restricted walkers w and unrestricted walkers (w, w) agree to
+  1e-12 in every measurement and after two steps.
 """
 
 from __future__ import annotations
@@ -96,3 +98,48 @@ def test_walker_order_is_immaterial(gpu):
     blk_a = complex((wa * e_a).sum().item() / wa.sum().item())
     blk_b = complex((wb * e_b).sum().item() / wb.sum().item())
     assert abs(blk_a - blk_b) < 1e-12 * abs(blk_a)
+
+
+def test_restricted_walkers_equal_unrestricted_with_equal_spins(gpu):
+    """The reference's walker-kind equivalence (tests/test_uhf.py:131-170, at 1e-12): restricted walkers w and
+    unrestricted walkers (w, w) under the RHF trial give the same overlaps, force bias, local energy and
+    observables, and the same walkers, weights and population shift after two steps."""
+    from trot_amd import meas, prop, trial
+    from trot_amd.core import HamChol, System
+
+    print("[invariants] restricted == unrestricted (w, w)", flush=True)
+    p = problem(24, 5, 30, 12, seed=63)
+    ham = HamChol(p["h0"], p["h1"], p["chol"]).to_device()
+    tr = trial.RhfTrial(p["ca"])
+    out = {}
+    for kind in ("restricted", "unrestricted"):
+        sys_ = System(24, (5, 5), kind)
+        mops = meas.make_rhf_meas_ops(sys_)
+        tops = trial.make_rhf_trial_ops(sys_)
+        pops = prop.make_prop_ops("restricted", kind)
+        params = prop.QmcParams(dt=0.01, n_walkers=12, seed=3)
+        pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
+        mctx = mops.build_meas_ctx(ham, tr)
+        w = torch.from_numpy(p["wa"]).cuda()
+        wk = w if kind == "restricted" else (w, w.clone())
+        res = {"ov": tops.overlap(wk, tr), "fb": mops.require_kernel("force_bias")(wk, ham, mctx, tr),
+               "e": mops.require_kernel("energy")(wk, ham, mctx, tr),
+               "dm": mops.require_observable("rdm1")(wk, ham, mctx, tr),
+               "dc": mops.require_observable("density_corr")(wk, ham, mctx, tr)}
+        st = pops.init_prop_state(sys=sys_, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops,
+                                  params=params, initial_walkers=wk)
+        for s in range(2):
+            st = pops.step(st, fields=torch.from_numpy(p["fields"][s]).cuda(), params=params, ham_data=ham,
+                           trial_data=tr, trial_ops=tops, meas_ops=mops, meas_ctx=mctx, prop_ctx=pctx)
+        res["state"] = st
+        out[kind] = res
+    r, u = out["restricted"], out["unrestricted"]
+    for name in ("ov", "fb", "e", "dm", "dc"):
+        assert rel(u[name].cpu().numpy(), r[name].cpu().numpy()) < 1e-12, name
+    sr, su = r["state"], u["state"]
+    for x in su.walkers:
+        assert rel(x.cpu().numpy(), sr.walkers.cpu().numpy()) < 1e-12
+    assert rel(su.weights.cpu().numpy(), sr.weights.cpu().numpy()) < 1e-12
+    assert rel(su.overlaps.cpu().numpy(), sr.overlaps.cpu().numpy()) < 1e-12
+    assert abs(su.pop_control_ene_shift.item() - sr.pop_control_ene_shift.item()) < 1e-12 * abs(
+        sr.pop_control_ene_shift.item())

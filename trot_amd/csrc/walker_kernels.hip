@@ -21,13 +21,13 @@
 // identity block and held in registers: element (i, j) in lane i of wave j / 16,
 // register j % 16.  Rows are never moved: each lane keeps the position its row
 // would have after LAPACK's swaps (lp), the pivot search runs over lp >= k with
-// ties to the smallest lp, and the pivot row and column reach every lane through
-// LDS (double-buffered by step parity).  Two barriers per step.  The arithmetic
-// per row is the row-swapping algorithm's; only the storage differs.
+// ties to the smallest lp; the pivot column reaches every lane through LDS
+// (double-buffered by step parity, so one barrier per step suffices) and each
+// wave takes its columns of the pivot row from lane pr by shuffles.  The
+// arithmetic per row is the row-swapping algorithm's; only the storage differs.
 // ---------------------------------------------------------------------------
 struct InvShared {
   cplx colk[2][64];  // column k of every (physical) row, before the step
-  cplx rowp[2][64];  // the pivot row, every column
   int s_piv[2];
   int perm[64];      // position swapped with k at step k (LAPACK ipiv)
   int outcol[64];
@@ -37,7 +37,6 @@ struct InvShared {
 __device__ __forceinline__ void inv_det_one(const double* __restrict__ mw, double* __restrict__ ow,
                                             double* __restrict__ detw, int n, InvShared& sh) {
   cplx(&colk)[2][64] = sh.colk;
-  cplx(&rowp)[2][64] = sh.rowp;
   int(&s_piv)[2] = sh.s_piv;
   int(&perm)[64] = sh.perm;
   int(&outcol)[64] = sh.outcol;
@@ -81,10 +80,6 @@ __device__ __forceinline__ void inv_det_one(const double* __restrict__ mw, doubl
     __syncthreads();
     const int pr = s_piv[pb];
     const int lpp = __shfl(lp, pr);  // position of the pivot row (>= k)
-    if (lane == pr) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) rowp[pb][wq * 16 + r] = a[r];
-    }
     if (t == 0) {
       perm[k] = lpp;
       dacc = cmul(dacc, colk[pb][pr]);
@@ -92,7 +87,6 @@ __device__ __forceinline__ void inv_det_one(const double* __restrict__ mw, doubl
     }
     if (lp == k) lp = lpp;  // the swap of positions k and lpp
     if (lane == pr) lp = k;
-    __syncthreads();
     const cplx piv = colk[pb][pr];
     const cplx ipiv = cinv(piv);
     // with u_j = a[pr][j] (1 at j = k): the pivot row becomes u ipiv, every other row i becomes
@@ -104,7 +98,7 @@ __device__ __forceinline__ void inv_det_one(const double* __restrict__ mw, doubl
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int j = wq * 16 + r;
-      cplx u = rowp[pb][j];
+      cplx u = {__shfl(a[r].re, pr), __shfl(a[r].im, pr)};  // the pivot row, from lane pr of this wave
       if (j == k) u = {1.0, 0.0};
       const bool zero = is_p || j == k;
       const cplx x = {zero ? 0.0 : a[r].re, zero ? 0.0 : a[r].im};

@@ -169,7 +169,8 @@ def test_block_matches_oracle(gpu):
 
 @pytest.mark.parametrize("mixed", [False, True])
 def test_graph_block_cisd_matches_eager(gpu, mixed):
-    """One HIP graph per block (GraphBlock) with the CISD trial, bit-identical to the eager block."""
+    """One HIP graph per block (GraphBlock) with the CISD trial and its rdm1 observable, bit-identical to
+    the eager block."""
     from trot_amd import prop
     from trot_amd.batch import _tensors
     from trot_amd.blocks import block
@@ -193,20 +194,22 @@ def test_graph_block_cisd_matches_eager(gpu, mixed):
         return pops.init_prop_state(sys=sys, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops,
                                     params=params, initial_walkers=torch.from_numpy(c["w"]).cuda())
 
+    names = ("rdm1",)  # the block's rdm1 observable rides in the same graph
     s_e = init()
     eager = []
     for _ in range(3):
-        s_e, o = block(s_e, **kw)
+        s_e, o = block(s_e, observable_names=names, **kw)
         eager.append((s_e, o))
     gb = GraphBlock()
     s_g = init()
     for b in range(3):
-        s_g, o = gb(s_g, **kw)
+        s_g, o = gb(s_g, observable_names=names, **kw)
         torch.cuda.synchronize()
         se, oe = eager[b]
         for x, y in zip(_tensors(s_g), _tensors(se)):
             assert torch.equal(x, y), b
         assert torch.equal(o.scalars["energy"], oe.scalars["energy"]), b
+        assert torch.equal(o.observables["rdm1"], oe.observables["rdm1"]), b
         torch.randn(4096, device="cuda").sum().item()  # unrelated eager work between replays
 
 

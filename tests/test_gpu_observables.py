@@ -207,3 +207,29 @@ def test_run_qmc_collects_observables(gpu):
         assert np.array_equal(err, stats.jackknife_ratios(num, den)[1]) and np.isfinite(err).all()
     else:
         assert np.isnan(err).all()
+
+
+def test_run_blocks_and_run_qmc_energy(gpu):
+    """make_run_blocks (trot/driver.py:64-110) and run_qmc_energy (trot/driver.py:771-803)."""
+    from trot_amd import blocks, driver
+
+    print("[observables] make_run_blocks / run_qmc_energy", flush=True)
+    kw, init = _graph_setup()
+    run_blocks = driver.make_run_blocks(block_fn=blocks.block, sys=kw["sys"], params=kw["params"],
+                                        trial_ops=kw["trial_ops"], meas_ops=kw["meas_ops"], prop_ops=kw["prop_ops"],
+                                        observable_names=("rdm1",))
+    st, scal, obs = run_blocks(init(), ham_data=kw["ham_data"], trial_data=kw["trial_data"],
+                               meas_ctx=kw["meas_ctx"], prop_ctx=kw["prop_ctx"], n_blocks=3)
+    assert scal["energy"].shape == (3,) and scal["weight"].shape == (3,) and obs[0].shape == (3, 2, 24, 24)
+    s_e = init()
+    for b in range(3):
+        s_e, o = blocks.block(s_e, observable_names=("rdm1",), **kw)
+        assert torch.equal(o.scalars["energy"], scal["energy"][b]) and torch.equal(o.observables["rdm1"], obs[0][b])
+    assert torch.equal(s_e.walkers, st.walkers)
+    args = dict(sys=kw["sys"], params=kw["params"], ham_data=kw["ham_data"], trial_data=kw["trial_data"],
+                meas_ops=kw["meas_ops"], trial_ops=kw["trial_ops"], prop_ops=kw["prop_ops"], block_fn=blocks.block,
+                meas_ctx=kw["meas_ctx"], prop_ctx=kw["prop_ctx"], verbose=False)
+    mean, err, be, bw = driver.run_qmc_energy(state=init(), **args)
+    res = driver.run_qmc(state=init(), **args)
+    assert mean == res.mean_energy and err == res.stderr_energy
+    assert np.array_equal(be, res.block_energies) and np.array_equal(bw, res.block_weights)

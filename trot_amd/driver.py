@@ -32,6 +32,45 @@ class QmcResult(NamedTuple):
     observable_stderrs: dict
 
 
+def make_run_blocks(*, block_fn, sys, params, trial_ops, meas_ops, prop_ops, observable_names: tuple = ()):
+    """trot/driver.py:64-110: ``run_blocks(state, *, ham_data, trial_data, meas_ctx, prop_ctx, n_blocks)`` runs
+    n_blocks blocks back to back on the GPU and returns (state, scalars, observables): scalars maps each
+    block scalar ("energy", "weight") to an (n_blocks,) device tensor, observables is a tuple with one
+    (n_blocks, ...) device tensor per name of ``observable_names`` (empty tuples / tensors for n_blocks = 0)."""
+    import torch
+
+    names = tuple(observable_names)
+    extra = {"observable_names": names} if names else {}
+
+    def run_blocks(state, *, ham_data, trial_data, meas_ctx, prop_ctx, n_blocks: int):
+        scal, obs = {}, {name: [] for name in names}
+        for _ in range(int(n_blocks)):
+            state, o = block_fn(state, sys=sys, params=params, ham_data=ham_data, trial_data=trial_data,
+                                trial_ops=trial_ops, meas_ops=meas_ops, meas_ctx=meas_ctx, prop_ops=prop_ops,
+                                prop_ctx=prop_ctx, **extra)
+            for k, v in o.scalars.items():
+                scal.setdefault(k, []).append(v.reshape(()))
+            for name in names:
+                obs[name].append(o.observables[name])
+        scalars = {k: torch.stack(v) for k, v in scal.items()}
+        if not scalars:
+            scalars = {"energy": torch.zeros(0, dtype=torch.float64), "weight": torch.zeros(0, dtype=torch.float64)}
+        return state, scalars, tuple(torch.stack(obs[name]) if obs[name] else None for name in names)
+
+    return run_blocks
+
+
+def run_qmc_energy(*, sys, params, ham_data, trial_data, meas_ops, trial_ops, prop_ops, block_fn=default_block,
+                   state=None, meas_ctx=None, prop_ctx=None, target_error: float | None = None, mesh=None,
+                   verbose: bool = True):
+    """trot/driver.py:771-803: run_qmc without observables; (mean energy, its error, block energies,
+    block weights)."""
+    out = run_qmc(sys=sys, params=params, ham_data=ham_data, trial_data=trial_data, meas_ops=meas_ops,
+                  trial_ops=trial_ops, prop_ops=prop_ops, block_fn=block_fn, state=state, meas_ctx=meas_ctx,
+                  prop_ctx=prop_ctx, target_error=target_error, mesh=mesh, observable_names=(), verbose=verbose)
+    return out.mean_energy, out.stderr_energy, out.block_energies, out.block_weights
+
+
 def run_qmc(*, sys, params, ham_data, trial_data, meas_ops, trial_ops, prop_ops, block_fn=default_block,
             state=None, meas_ctx=None, prop_ctx=None, target_error: float | None = None, mesh=None,
             observable_names: tuple = (), verbose: bool = True) -> QmcResult:
@@ -50,24 +89,15 @@ def run_qmc(*, sys, params, ham_data, trial_data, meas_ops, trial_ops, prop_ops,
     log = print if verbose else (lambda *a, **k: None)
 
     names = tuple(observable_names)
-    extra = {"observable_names": names} if names else {}
+    blocks_on_gpu = make_run_blocks(block_fn=block_fn, sys=sys, params=params, trial_ops=trial_ops,
+                                    meas_ops=meas_ops, prop_ops=prop_ops, observable_names=names)
 
-    def run_blocks(st, n):
-        es, ws = [], []
-        os_ = {name: [] for name in names}
-        for _ in range(n):
-            st, obs = block_fn(st, sys=sys, params=params, ham_data=ham_data, trial_data=trial_data,
-                               trial_ops=trial_ops, meas_ops=meas_ops, meas_ctx=meas_ctx, prop_ops=prop_ops,
-                               prop_ctx=prop_ctx, **extra)
-            es.append(obs.scalars["energy"])
-            ws.append(obs.scalars["weight"])
-            for name in names:
-                os_[name].append(obs.observables[name])
-        import torch
-
-        e = torch.stack(es).cpu().numpy() if es else np.zeros(0)
-        w = torch.stack(ws).cpu().numpy() if ws else np.zeros(0)
-        o = {name: torch.stack(v).cpu().numpy() for name, v in os_.items() if v}
+    def run_blocks(st, n):  # one host transfer per chunk of blocks, as the reference's jitted chunks
+        st, scal, obs = blocks_on_gpu(st, ham_data=ham_data, trial_data=trial_data, meas_ctx=meas_ctx,
+                                      prop_ctx=prop_ctx, n_blocks=n)
+        e = scal["energy"].cpu().numpy()
+        w = scal["weight"].cpu().numpy()
+        o = {name: v.cpu().numpy() for name, v in zip(names, obs) if v is not None}
         return st, e, w, o
 
     t0 = time.perf_counter()

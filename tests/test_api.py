@@ -73,3 +73,26 @@ def test_complex_trial_rejected():
         return
     with pytest.raises(ValueError):
         trial.RhfTrial(np.eye(3)[:, :1] * (1 + 1j))
+
+
+def test_driver_and_block_names_of_the_reference():
+    """The callables a trot driver script imports from trot.driver, trot.prop.blocks and trot.stat_utils
+    exist under the same names (trot/driver.py:64-110, 168-406, 771-803; trot/prop/blocks.py:66-195;
+    trot/stat_utils.py:45-313)."""
+    import inspect
+
+    from trot_amd import blocks, driver, stats
+
+    for mod, names in ((driver, ["make_run_blocks", "run_qmc", "run_qmc_energy", "QmcResult"]),
+                       (blocks, ["block", "BlockObs", "dump_prop_state_npz", "load_prop_state_npz",
+                                 "load_all_prop_states_npz", "make_block_state_logger"]),
+                       (stats, ["blocking_analysis_ratio", "reject_outliers", "jackknife_ratios",
+                                "rebin_observable"])):
+        for n in names:
+            assert hasattr(mod, n), f"{mod.__name__}.{n}"
+    assert list(driver.QmcResult._fields) == ["mean_energy", "stderr_energy", "block_energies", "block_weights",
+                                              "block_observables", "observable_means", "observable_stderrs"]
+    sig = inspect.signature(driver.run_qmc).parameters
+    for p in ("sys", "params", "ham_data", "trial_data", "meas_ops", "trial_ops", "prop_ops", "block_fn", "state",
+              "meas_ctx", "prop_ctx", "target_error", "mesh", "observable_names"):
+        assert p in sig, p

@@ -50,6 +50,11 @@ def block(state: PropState, *, sys: System, params, ham_data, trial_data, trial_
     out = _lib.block_energy(e_samples, weights, scal, params.dt, params.shift_ema)  # blocks.py:240-257
     obs_means = {}
     for name, kernel in obs_kernels.items():  # blocks.py:259-268, weights with non-finite energies zeroed
+        if name in getattr(engine, "observables", ()) and hasattr(engine, "observable_mean_from_green"):
+            mean = engine.observable_mean_from_green(name, greens, weights, out[1:2], meas_ctx, trial_data)
+            if mean is not None:  # a linear observable: reduced before the product, no per-walker samples
+                obs_means[name] = mean
+                continue
         if name in getattr(engine, "observables", ()):  # from the Green's functions already at hand
             samples = engine.observable_from_green(name, greens, meas_ctx, trial_data)
         else:

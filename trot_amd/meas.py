@@ -179,6 +179,24 @@ class Engine:
         _lib.rdm1_from_green(g_a if g_b is None else g_b, cb, dm, 1)
         return dm if name == o_rdm1 else _lib.density_corr(dm)
 
+    def observable_mean_from_green(self, name: str, greens, weights, wsum, meas_ctx, trial_data):
+        """The block value of a linear observable without per-walker samples, or None for others.
+
+        rdm1 is linear in GT: sum_k w_k GT_k C^T / sum_k w_k = (sum_k w_k GT_k / sum_k w_k) C^T, so the block
+        mean needs one weighted mean of GT (W N n values) and one product, not the W x 2 x N x N samples.
+        """
+        if name != o_rdm1:
+            return None
+        ca, cb = _coeffs(trial_data)
+        g_a, g_b = greens
+        N = g_a.shape[1]
+        ma = _lib.weighted_mean(g_a, weights, wsum).unsqueeze(0).contiguous()
+        mb = ma if g_b is None else _lib.weighted_mean(g_b, weights, wsum).unsqueeze(0).contiguous()
+        dm = torch.empty((1, 2, N, N), dtype=torch.complex128, device=g_a.device)
+        _lib.rdm1_from_green(ma, ca, dm, 0)
+        _lib.rdm1_from_green(mb, cb, dm, 1)
+        return dm[0]
+
     def force_bias_from_green(self, greens, meas_ctx) -> torch.Tensor:
         sa, sb = self.spins(meas_ctx)
         alpha = 2.0 if self.kind == "rr" else 1.0  # trot/meas/rhf.py:124-125 (both spins); 150-153; uhf.py:55-57

@@ -59,22 +59,20 @@ __device__ __forceinline__ void inv_det_one(const double* __restrict__ mw, doubl
       colk[pb][lane] = x;
       const double c = cabs1(x);
       double v = (lp >= k && lane < n && c == c) ? c : -1.0;  // NaN never wins
-      int key = lp, bi = lane;
+      int key = lp;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
         const double v2 = __shfl_xor(v, off);
         const int k2 = __shfl_xor(key, off);
-        const int b2 = __shfl_xor(bi, off);
         if (v2 > v || (v2 == v && k2 < key)) {
           v = v2;
           key = k2;
-          bi = b2;
         }
       }
-      if (v < 0.0) {  // column of NaNs: keep the row at position k
-        const unsigned long long at_k = __ballot(lp == k);
-        bi = __ffsll((long long)at_k) - 1;
-      }
+      // the winning row is the lane whose position is key (positions are distinct); a column of NaNs
+      // keeps the row at position k
+      const unsigned long long at = __ballot(lp == (v < 0.0 ? k : key));
+      const int bi = __ffsll((long long)at) - 1;
       if (lane == 0) s_piv[pb] = bi;
     }
     __syncthreads();

@@ -201,6 +201,17 @@ int trot_sr_comb(const double* weights, double zeta, int W, int* idx, double* we
 int trot_sr_comb_dev(const double* weights, const double* zeta, int W, int* idx, double* weights_new, void* ws,
                      size_t ws_bytes, void* stream);
 
+/* trot_sr_comb_dev without a workspace argument: the cumulative weights go to a static device array of
+ * TROT_SR_NOALLOC_MAX_W doubles (2 MiB) that each GPU holds until the process ends.  Allocates nothing and
+ * calls no allocating routine; W above the limit returns TROT_ERR_UNSUPPORTED.  Calls on one GPU share the
+ * array, so calls that overlap on one GPU (two streams, two threads) race; calls ordered on one stream do not.
+ * Results are bit-identical to trot_sr_comb_dev.
+ * No counterpart in the reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). */
+#define TROT_SR_NOALLOC_MAX_W 262144
+int trot_sr_comb_dev_noalloc(const double* weights, const double* zeta, int W, int* idx, double* weights_new,
+                             void* stream);
+
 /* Counter-based random numbers: Philox4x64-10 (Salmon et al., SC'11; numpy.random.Philox), block j of
  * draw `counter` for purpose p is Philox(counter words (j, counter, p, 0), key words (seed, 0)).
  * `state`, when non-null, is a device array {seed, counter} read in place of the by-value pair, so

@@ -56,6 +56,8 @@ def test_error_codes_without_gpu():
     assert h.trot_local_energy_workspace(8, 100) >= 8 * 4 * 16
     assert h.trot_force_bias(None, None, 1, 1, 1, 2.0, 0, None, None) == -1
     assert h.trot_sr_comb(dummy, 0.5, 0, dummy, dummy, dummy, 4096, None) == -1
+    assert h.trot_sr_comb_dev_noalloc(dummy, None, 8, dummy, dummy, None) == -1
+    assert h.trot_sr_comb_dev_noalloc(dummy, dummy, 262145, dummy, dummy, None) == -2  # above the static scratch
     assert h.trot_one_body_apply(dummy, dummy, 1, 4, 2, dummy, None) == -1  # in place is rejected
     assert h.trot_orthonormalize(dummy, 2, 8, 70, dummy, 1 << 30, None) == -1  # n > N
     assert h.trot_orthonormalize(dummy, 2, 100, 70, dummy, 1 << 30, None) == -2  # n > 64

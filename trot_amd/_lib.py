@@ -74,6 +74,7 @@ _SIGS = {
     "trot_sr_comb": (_i, [_vp, _d, _i, _vp, _vp, _vp, _sz, _vp]),
     "trot_gather_rows": (_i, [_vp, _vp, _vp, _i, _l, _vp]),
     "trot_sr_comb_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _sz, _vp]),
+    "trot_sr_comb_dev_noalloc": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "trot_gemm_problem_bytes": (_sz, []),
     "trot_gemm_problem": (_l, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _vp]),
     "trot_gemm_f64_grouped": (_i, [_i, _vp, _vp, _i, _i, _vp]),

@@ -24,6 +24,31 @@
 
 thread_local char g_trot_err[512];
 
+// ---------------------------------------------------------------------------
+// Static scratch of the allocation-free comb (trot_sr_comb_dev_noalloc): the cumulative weights of up to
+// TROT_SR_NOALLOC_MAX_W walkers.  Each GPU holds its own copy of the array until the process ends.  In host
+// code `scratch` is only a handle: its device address on the current GPU comes from hipGetSymbolAddress,
+// looked up once per device and kept in g_scratch_dev.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[TROT_SR_NOALLOC_MAX_W];
+static std::mutex g_scratch_mu;
+static std::map<int, double*> g_scratch_dev;  // device ordinal -> address of `scratch` on that device
+
+static int scratch_on_current_device(double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_dev.find(dev);
+  if (it == g_scratch_dev.end()) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(TROT_ERR_HIP, "sr_comb_dev_noalloc: no device address for the static scratch");
+    it = g_scratch_dev.emplace(dev, static_cast<double*>(p)).first;
+  }
+  *out = it->second;
+  return TROT_OK;
+}
+
 extern "C" {
 
 const char* trot_version(void) { return "trot_amd 0.1 (gfx950, FP64 MFMA)"; }
@@ -559,6 +584,19 @@ int trot_sr_comb_dev(const double* weights, const double* zeta, int W, int* idx,
   if (W < 1) return fail(TROT_ERR_INVALID, "sr_comb_dev: bad sizes");
   if (ws_bytes < trot_sr_workspace(W)) return fail(TROT_ERR_WORKSPACE, "sr_comb_dev: workspace too small");
   HIPCHK(launch_sr_index(weights, (double*)ws, idx, weights_new, W, 0.0, zeta, (hipStream_t)stream));
+  return TROT_OK;
+}
+
+int trot_sr_comb_dev_noalloc(const double* weights, const double* zeta, int W, int* idx, double* weights_new,
+                             void* stream) {
+  if (!weights || !zeta || !idx || !weights_new) return fail(TROT_ERR_INVALID, "sr_comb_dev_noalloc: null pointer");
+  if (W < 1) return fail(TROT_ERR_INVALID, "sr_comb_dev_noalloc: bad sizes");
+  if (W > TROT_SR_NOALLOC_MAX_W)
+    return fail(TROT_ERR_UNSUPPORTED, "sr_comb_dev_noalloc: W above TROT_SR_NOALLOC_MAX_W (the static scratch)");
+  double* cw = nullptr;  // the kernel writes all W cumulative weights before reading any: nothing to clear
+  const int rc = scratch_on_current_device(&cw);
+  if (rc != TROT_OK) return rc;
+  HIPCHK(launch_sr_index(weights, cw, idx, weights_new, W, 0.0, zeta, (hipStream_t)stream));
   return TROT_OK;
 }
 

@@ -10,8 +10,10 @@ reference package ``trot`` (JAX).  Every function cites the reference
 file:line it follows (paths relative to the reference root).  The reference
 itself cannot be imported here (JAX, jaxlib and PySCF are absent), so this
 restatement is pinned by closed forms, known-answer tests taken from the
-reference's own tests, and brute-force Fock-space evaluation on tiny systems
-(see ``oracle/fock.py`` and ``tests/test_oracle_*.py``).
+reference's own tests, brute-force Fock-space evaluation on tiny systems, and,
+for the field shifts and importance weights, the unbiasedness of importance
+sampling (Gauss-Hermite quadrature) and the Fock-space propagator
+exp(-dt (H - E_shift)) (see ``oracle/fock.py`` and ``tests/test_oracle_*.py``).
 
 Walker layouts follow the reference: restricted walkers are complex arrays of
 shape (W, N, n); unrestricted walkers are pairs ``(wu (W,N,na), wd (W,N,nb))``.
@@ -388,6 +390,16 @@ def shift_terms(fields: np.ndarray, force_bias: np.ndarray, ctx: PropCtx):
     return xbar, shift_term, fb_term
 
 
+def importance_exponent(shift_term: np.ndarray, fb_term: np.ndarray, pop_shift: float, ctx: PropCtx) -> np.ndarray:
+    """trot/prop/afqmc.py:114-116: exponent = -sqrt(dt) shift_term + fb_term + dt (E_shift + h0_prop)."""
+    return -ctx.sqrt_dt * shift_term + fb_term + ctx.dt * (pop_shift + ctx.h0_prop)
+
+
+def phaseless_theta(shift_term: np.ndarray, ratio: np.ndarray, ctx: PropCtx) -> np.ndarray:
+    """trot/prop/afqmc.py:119: theta = angle(exp(-sqrt(dt) shift_term) ratio), in (-pi, pi]."""
+    return np.angle(np.exp(-ctx.sqrt_dt * shift_term) * ratio)
+
+
 def phaseless_update(
     weights: np.ndarray,
     overlaps_old: np.ndarray,
@@ -408,9 +420,9 @@ def phaseless_update(
     """
     with np.errstate(all="ignore"):
         ratio = overlaps_new / overlaps_old
-        exponent = -ctx.sqrt_dt * shift_term + fb_term + ctx.dt * (pop_shift + ctx.h0_prop)
+        exponent = importance_exponent(shift_term, fb_term, pop_shift, ctx)
         imp_fun = np.exp(exponent) * ratio
-        theta = np.angle(np.exp(-ctx.sqrt_dt * shift_term) * ratio)
+        theta = phaseless_theta(shift_term, ratio, ctx)
         imp_ph = np.abs(imp_fun) * np.cos(theta)
         imp_ph = np.where(~np.isfinite(imp_ph) | (imp_ph < params.weight_floor), 0.0, imp_ph)
         nodes = int(node_encounters) + int(np.sum(imp_ph <= 0.0))

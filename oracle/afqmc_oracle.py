@@ -332,7 +332,17 @@ def energy_uhf(w: tuple, h0: float, ctx: UhfMeasCtx, ca: np.ndarray, cb: np.ndar
 
 
 def make_vhs(xbar: np.ndarray, chol_flat: np.ndarray, norb: int) -> np.ndarray:
-    """trot/prop/chol_afqmc_ops.py:83-87: V = (Re x @ Lflat + i Im x @ Lflat).reshape(N, N)."""
+    """trot/prop/chol_afqmc_ops.py:83-87: V = (Re x @ Lflat + i Im x @ Lflat).reshape(N, N).
+
+    A float32 chol_flat is the reference's mixed mode (afqmc.py:177 stores chol_flat as float32;
+    chol_afqmc_ops.py:247-257 casts x to complex64): the two products run in float32 and V is cast back
+    to complex128 before the Taylor series (chol_afqmc_ops.py:160).
+    """
+    if chol_flat.dtype == np.float32:
+        x = xbar.astype(np.complex64)
+        v_re = np.ascontiguousarray(np.real(x)) @ chol_flat
+        v_im = np.ascontiguousarray(np.imag(x)) @ chol_flat
+        return (v_re.astype(np.float64) + 1.0j * v_im.astype(np.float64)).reshape(xbar.shape[0], norb, norb)
     # contiguous copies keep numpy on its BLAS path (a strided view of a complex array is not)
     v_re = np.ascontiguousarray(np.real(xbar)) @ chol_flat
     v_im = np.ascontiguousarray(np.imag(xbar)) @ chol_flat

@@ -24,7 +24,10 @@ from dataclasses import dataclass
 import numpy as np
 from scipy.special import erf
 
-# STO-6G hydrogen 1s (zeta = 1.24): (exponent, contraction coefficient of the normalized primitive)
+# STO-6G hydrogen 1s (zeta = 1.24): (exponent, contraction coefficient of the normalized primitive).
+# Source: W. J. Hehre, R. F. Stewart and J. A. Pople, J. Chem. Phys. 51, 2657 (1969), the least-squares
+# six-Gaussian fit of a 1s Slater function, exponents scaled by zeta^2; the values as distributed by the
+# Basis Set Exchange (basis "STO-6G", element H).
 STO6G_H = (
     (35.52322122, 0.00916359628),
     (6.513143725, 0.04936149294),

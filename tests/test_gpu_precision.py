@@ -298,3 +298,73 @@ def test_local_energy_b3(gpu, shape, kind):
     e_ref = ref.energy(wn)
     assert rel(e64 - p["h0"], e_ref - p["h0"]) < 1e-10
     assert rel(e - p["h0"], e_ref - p["h0"]) < 4 * TOL["bf16x3"]
+
+
+def test_reference_mixed_precision_flag(gpu):
+    """make_prop_ops(mixed_precision=True) is the reference's mixed mode: xbar as complex64 and chol_flat as
+    float32 for the VHS only, V cast back to complex128, one-body factors and Taylor series in FP64
+    (trot/prop/chol_afqmc_ops.py:160, 247-257; trot/prop/afqmc.py:177).
+
+    Part 1: with xbar and every L_g multiples of 1/64 below 2 in magnitude and G = 16, every product and
+    partial sum of the VHS is exact in FP32, so the FP32 VHS equals the FP64 one and the whole Trotter factor
+    must meet the FP64 oracle at 1e-12; an FP32 one-body or Taylor product would miss that by ~1e5.
+    Part 2: with general inputs, four steps stay within the FP32 bound of the oracle run with the reference's
+    casts (float32 chol_flat).
+    """
+    from dataclasses import replace
+
+    from trot_amd import meas, prop, trial
+    from trot_amd.core import HamChol, System
+
+    N, n, G, W = 40, 6, 16, 9
+    rng = np.random.default_rng(77)
+    a = rng.standard_normal((N, N)) / np.sqrt(N)
+    h1 = np.triu(a) + np.triu(a, 1).T
+    b = np.round(rng.uniform(-1.9, 1.9, (G, N, N)) * 64) / 64
+    chol = np.triu(b) + np.swapaxes(np.triu(b, 1), 1, 2)  # symmetric, still multiples of 1/64
+    chol *= 1.0 / 64  # |L| < 2/64: the products stay multiples of 2^-18, sums below 2^6
+    ca = np.linalg.eigh(h1)[1][:, :n]
+    sys_ = System(N, (n, n), "restricted")
+    ham = HamChol(0.0, h1, chol).to_device()
+    tr = trial.RhfTrial(ca)
+    tops = trial.make_rhf_trial_ops(sys_)
+    pops = prop.make_prop_ops("restricted", "restricted", mixed_precision=True)
+    params = prop.QmcParams(dt=0.01, n_walkers=W, seed=1)
+    pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
+    assert pctx.precision == "f32" and pctx.taylor_precision == "f64"
+    ref_ctx = O.build_prop_ctx(0.0, h1, chol, O.rhf_rdm1(ca), 0.01)
+    w = ca[None] + 0.05 * (rng.standard_normal((W, N, n)) + 1j * rng.standard_normal((W, N, n)))
+    xbar = (np.round(rng.uniform(-1.9, 1.9, (W, G)) * 64) + 1j * np.round(rng.uniform(-1.9, 1.9, (W, G)) * 64)) / 64
+    got = prop.apply_trotter(_t(w), _t(xbar), pctx, 6).cpu().numpy()
+    exact = O.apply_trotter_r(w, xbar, ref_ctx, 6)
+    assert rel(got - w, exact - w) < 1e-12
+
+    # part 2: general inputs against the oracle with the reference's float32 chol_flat
+    p = problem(48, 8, 60, 32, seed=33)
+    ham = HamChol(p["h0"], p["h1"], p["chol"]).to_device()
+    tr = trial.RhfTrial(p["ca"])
+    sys_ = System(48, (8, 8), "restricted")
+    mops = meas.make_rhf_meas_ops(sys_)
+    tops = trial.make_rhf_trial_ops(sys_)
+    params = prop.QmcParams(dt=0.01, n_walkers=32, seed=1)
+    pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
+    mctx = mops.build_meas_ctx(ham, tr)
+    st = pops.init_prop_state(sys=sys_, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops, params=params,
+                              initial_walkers=_t(p["wa"]))
+    ref_ops = O.make_rhf_ops(p["h0"], p["h1"], p["chol"], p["ca"])
+    ref64_ctx = O.build_prop_ctx(p["h0"], p["h1"], p["chol"], O.rhf_rdm1(p["ca"]), 0.01)
+    ref_ctx = replace(ref64_ctx, chol_flat=ref64_ctx.chol_flat.astype(np.float32))
+    ref = O.init_prop_state(p["wa"], ref_ops, 32)
+    ref64 = O.init_prop_state(p["wa"], ref_ops, 32)
+    for s in range(4):
+        f = p["fields"][s]
+        st = pops.step(st, params=params, ham_data=ham, trial_data=tr, trial_ops=tops, meas_ops=mops, meas_ctx=mctx,
+                       prop_ctx=pctx, fields=_t(f))
+        ref = O.afqmc_step(ref, f, ref_ctx, ref_ops, O.QmcParams(dt=0.01, n_walkers=32))
+        ref64 = O.afqmc_step(ref64, f, ref64_ctx, ref_ops, O.QmcParams(dt=0.01, n_walkers=32))
+    wg = st.walkers.cpu().numpy()
+    # the FP32 VHS error, relative to the walkers' change over four steps
+    assert rel(wg - p["wa"], ref.walkers - p["wa"]) < 50 * TOL["f32"]
+    assert rel(st.weights.cpu().numpy(), ref.weights) < 50 * TOL["f32"]
+    # and the reference's mixed mode itself differs from FP64 by FP32 rounding only
+    assert rel(ref.walkers - p["wa"], ref64.walkers - p["wa"]) < 50 * TOL["f32"]

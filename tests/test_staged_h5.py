@@ -118,3 +118,21 @@ def test_staged_rhf_cache_drives_the_gpu_step(gpu):
         ref = O.afqmc_step(ref, f, ref_ctx, ref_ops, ref_params)
     assert rel(st.walkers.cpu().numpy(), ref.walkers) < 1e-10
     assert rel(st.weights.cpu().numpy(), ref.weights) < 1e-10
+
+
+def test_truncated_file_is_refused(tmp_path):
+    """Contiguous storage that runs past the end of the file is an error (as in h5py), not a zero tail."""
+    from trot_amd import staged
+
+    data = (GOLD / "staged_rhf_earliest.h5").read_bytes()
+    exp = _exp()
+    start = data.find(exp["rhf_chol"].tobytes())
+    assert start > 0
+    cut = tmp_path / "truncated.h5"
+    cut.write_bytes(data[:start + exp["rhf_chol"].nbytes // 2])  # ends inside the Cholesky vectors
+    f = staged.H5File(cut)
+    assert np.array_equal(f["ham/h1"], exp["rhf_h1"])  # datasets before the cut still read
+    with pytest.raises(ValueError, match="truncated"):
+        f["ham/chol"]
+    with pytest.raises(ValueError, match="truncated"):
+        staged.load_staged(cut)

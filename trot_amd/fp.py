@@ -92,7 +92,8 @@ def make_prop_ops_fp(ham_basis: str, walker_kind: str, sys: System, mixed_precis
                      precision: str | None = None, taylor_precision: str | None = None,
                      taylor_hi_from: int = 0) -> PropOps:
     """trot/prop/afqmc_fp.py:75-120.  ``mixed_precision`` runs the VHS on FP32 MFMA (the reference casts
-    chol_flat to float32); ``precision`` / ``taylor_precision`` name the forms of make_prop_ops."""
+    chol_flat to float32 and xbar to complex64) and the Taylor series in FP64 on V cast back to complex128;
+    ``precision`` / ``taylor_precision`` name the forms of make_prop_ops."""
     if ham_basis != "restricted":
         raise NotImplementedError(f"Not implemented for ham_basis={ham_basis}")
     if walker_kind.lower() not in ("restricted", "unrestricted"):

@@ -193,7 +193,7 @@ def build_prop_ctx(ham_data, rdm1, dt: float, precision: str = "f64", walker_chu
         chol_flat=chol.reshape(G, N * N),
         norb=N,
         precision=precision,
-        taylor_precision=taylor_precision or ("f64" if precision == "f64" else "f32"),
+        taylor_precision=taylor_precision or "f64",
         walker_chunk=walker_chunk,
         taylor_hi_from=int(taylor_hi_from),
     )
@@ -455,10 +455,14 @@ def make_prop_ops(ham_basis: str, walker_kind: str, mixed_precision: bool = Fals
                   walker_chunk: int = 0, vhs_layout: str = "auto", taylor_hi_from: int = 0) -> PropOps:
     """trot/prop/afqmc.py:147-180.
 
-    ``mixed_precision=True`` selects the FP32 VHS GEMM (the reference's mixed
-    mode, trot/prop/chol_afqmc_ops.py:247-257); ``precision`` may name "f64",
-    "f32" or "bf16x3" explicitly.  ``taylor_precision`` (default: "f64" for an
-    FP64 VHS, else "f32") sets the MFMA precision of the Taylor terms; the
+    ``mixed_precision=True`` is the reference's mixed mode
+    (trot/prop/chol_afqmc_ops.py:247-257, afqmc.py:177): xbar rounded to
+    complex64 and chol_flat to float32 for the VHS GEMM only (FP32 MFMA, FP32
+    accumulation), V returned as complex128, and the one-body factors and the
+    Taylor series in FP64 as for complex128 walkers (chol_afqmc_ops.py:160).
+    ``precision`` may name the VHS form "f64", "f32" or "bf16x3" explicitly.
+    ``taylor_precision`` (default "f64" in every form) puts the one-body and
+    Taylor products on FP32 ("f32") or split-BF16 ("bf16x3") MFMA instead; the
     walkers and the running Taylor sum stay complex128.
     """
     if taylor_precision not in (None, "f64", "f32", "bf16x3"):

@@ -318,16 +318,14 @@ class _Reader:
             if cls == 0:
                 raw = b[p + 4:p + 4 + self.u(p + 2, 2)]
             elif cls == 1:
-                start = self.u(p + 2, self.O)
-                raw = b"" if start == self.undef else b[self.addr(start):self.addr(start) + nbytes]
+                raw = self.contiguous(self.u(p + 2, self.O), nbytes)
             else:
                 raise NotImplementedError("chunked or virtual dataset layout")
         elif ver in (1, 2):
             rank, cls = b[p + 1], b[p + 2]
             q = p + 8
             if cls == 1:
-                start = self.u(q, self.O)
-                raw = b[self.addr(start):self.addr(start) + nbytes]
+                raw = self.contiguous(self.u(q, self.O), nbytes)
             elif cls == 0:
                 q += 4 * rank
                 raw = b[q + 4:q + 4 + self.u(q, 4)]
@@ -338,6 +336,17 @@ class _Reader:
         if len(raw) < nbytes:
             raw = bytes(raw) + b"\0" * (nbytes - len(raw))  # unallocated storage reads as the zero fill value
         return self.decode(raw, dt, shape)
+
+    def contiguous(self, start: int, nbytes: int):
+        """Bytes of contiguous storage at file address `start`; the undefined address is unallocated storage
+        (read as the zero fill value), and storage that runs past the end of the file is a truncated file."""
+        if start == self.undef:
+            return b""
+        a = self.addr(start)
+        if a + nbytes > len(self.b):
+            raise ValueError(f"dataset storage [{a}, {a + nbytes}) runs past the end of the file ({len(self.b)} "
+                             "bytes): truncated file")
+        return self.b[a:a + nbytes]
 
     def is_dataset(self, a: int) -> bool:
         return any(m[0] == 0x08 for m in self.messages(a))

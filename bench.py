@@ -5,19 +5,22 @@
 
 Workload (default): the nbasis=512 synthetic system, N=512 orbitals, n=64
 occupied, G=2560 Cholesky vectors, 2048 complex128 walkers, dt=0.005, RHF
-trial, restricted walkers (seeded synthetic Hamiltonian, trot_amd/synthetic.py).
-A step is one phaseless step for all walkers: field draw, force bias, shifted
-fields, VHS build, one-body / Taylor / one-body propagation, overlaps and
-Green's functions, weight update.  All inputs are resident in HBM before the
-timed regions.
+trial, restricted walkers (seeded synthetic Hamiltonian, trot_amd/synthetic.py,
+every L_gamma scaled by 1/sqrt(N) so that the walkers keep O(1) weights: with
+the recipe's unscaled L every importance leaves [floor, cap] within ~10 steps
+and the overlaps overflow, in the CPU oracle as on the GPU; the rate on the
+unscaled recipe is reported beside the headline).  A step is one phaseless step
+for all walkers: field draw, force bias, shifted fields, VHS build, one-body /
+Taylor / one-body propagation, overlaps and Green's functions, weight update.
+All inputs are resident in HBM before the timed regions.
 
-The same process times K steps of the FP64 path and K steps of the mixed
-path (reduced-precision Cholesky GEMMs and Taylor terms on MFMA; FP64
-determinants, inverse, weights, energies).  `value` is the mixed rate; the
-FP64 rate and the speedup sit beside it.  The mixed path's accuracy is
-measured against the FP64 path (which the test suite pins to the CPU oracle)
-by running both from one state with identical fields and comparing the
-per-walker local energies and the weighted block energy, in mHa.
+The same process times K steps of the FP64 path and K steps of each reduced-
+precision form (split-BF16 with Taylor terms 3-6 as one bf16 pass: `value`;
+split-BF16 with every product three-pass; FP32; the reference's own mixed mode,
+FP32 VHS with FP64 Taylor).  Each form's accuracy is measured against the FP64
+path (which the test suite pins to the CPU oracle) by running all of them in
+lockstep from one state with identical fields and comparing the per-walker
+local energies and the weighted block energy, in mHa.
 
 `roofline` describes the dominant kernel of the mixed step (the VHS GEMM);
 its duration comes from HIP events recorded around its launches inside the
@@ -58,6 +61,9 @@ def parse():
     ap.add_argument("--no-block", action="store_true", help="skip the block-tail timing")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (profiling runs)")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replayed step (profiling runs)")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the other precision forms (all-pass split BF16, FP32, the reference's mixed mode)")
+    ap.add_argument("--no-unscaled", action="store_true", help="skip the rate on the unscaled recipe")
     ap.add_argument("--cpu-walkers", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--cisd-workload", default="cisd_n128_o16_g512_w1024")
@@ -98,17 +104,21 @@ def traffic_from_profiles(tag: str):
 class Runner:
     """One precision configuration of the step on a shared synthetic system."""
 
-    def __init__(self, env, precision: str, taylor_precision: str, hi_from: int = 0):
+    def __init__(self, env, precision: str | None, taylor_precision: str | None, hi_from: int = 0,
+                 mixed_precision: bool = False):
         from trot_amd import meas, prop
 
         self.env = env
-        self.mops = meas.make_rhf_meas_ops(env["sys"], precision=precision)
-        self.pops = prop.make_prop_ops("restricted", "restricted", precision=precision,
+        # the reference's mixed mode keeps the RHF force bias and energy in FP64 (trot/meas/rhf.py has no
+        # precision switch); the explicit forms run them in the VHS's precision
+        self.meas_precision = "f64" if mixed_precision else precision
+        self.mops = meas.make_rhf_meas_ops(env["sys"], precision=self.meas_precision)
+        self.pops = prop.make_prop_ops("restricted", "restricted", mixed_precision, precision=precision,
                                        taylor_precision=taylor_precision, taylor_hi_from=hi_from)
         self.pctx = self.pops.build_prop_ctx(env["ham"], env["tops"].get_rdm1(env["tr"]), env["params"])
         self.mctx = self.mops.build_meas_ctx(env["ham"], env["tr"])
-        self.precision = precision
-        self.taylor_precision = taylor_precision
+        self.precision = self.pctx.precision
+        self.taylor_precision = self.pctx.taylor_precision
 
     def init(self):
         e = self.env
@@ -166,6 +176,13 @@ class Runner:
         return st, dt, timer.mean_ms("vhs")
 
 
+FORMS = (  # (name, precision, taylor_precision, taylor_hi_from, mixed_precision flag)
+    ("bf16x3_all_passes", "bf16x3", "bf16x3", 0, False),
+    ("f32", "f32", "f32", 0, False),
+    ("reference_mixed", None, None, 0, True),
+)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -186,8 +203,8 @@ def main():
     N, n, G, W, dt = cfg["norb"], cfg["nocc"], cfg["nchol"], cfg["n_walkers"], cfg["dt"]
     dev = torch.device("cuda")
     t0 = time.perf_counter()
-    syn = synthetic.make_synthetic(N, n, G, W, cfg["seed"], cfg["tau"], device=dev)
-    log(f"synthetic system N={N} n={n} G={G} W={W} in {time.perf_counter() - t0:.1f}s")
+    syn = synthetic.make_synthetic(N, n, G, W, cfg["seed"], cfg["tau"], device=dev, chol_scale=1.0 / np.sqrt(N))
+    log(f"synthetic system N={N} n={n} G={G} W={W} (L / sqrt(N)) in {time.perf_counter() - t0:.1f}s")
     sys_ = System(N, (n, n), "restricted")
     env = dict(sys=sys_, ham=HamChol(syn.h0, torch.from_numpy(syn.h1).to(dev), syn.chol),
                tr=trial.RhfTrial(syn.trial), tops=trial.make_rhf_trial_ops(sys_),
@@ -199,11 +216,14 @@ def main():
     log(f"mixed contexts ready at {time.perf_counter() - t0:.1f}s (sym-block VHS: {bool(mixed.pctx.low.get('symblk'))})")
     st_mix, t_mix, vhs_ms = mixed.timed(st_mix, args.warmup, args.steps, f"mixed {args.precision}")
     value = W * args.steps / t_mix
+    w_min, w_max = float(st_mix.weights.min()), float(st_mix.weights.max())
+    log(f"weights after the timed steps in [{w_min:.4f}, {w_max:.4f}]")
     # the same step replayed from one HIP graph per step (fields drawn on the device)
     rate_graph = None
     if not args.no_graph:
         st_mix, t_mix_graph = mixed.timed_graph(st_mix, args.warmup, args.steps, f"mixed {args.precision} graph")
         rate_graph = W * args.steps / t_mix_graph
+    del st_mix
 
     fp64 = None
     rate64 = None
@@ -212,18 +232,48 @@ def main():
         st64 = fp64.init()
         st64, t64, vhs64_ms = fp64.timed(st64, args.warmup, args.steps, "fp64")
         rate64 = W * args.steps / t64
+        del st64
+
+    # the other precision forms: rates, then all of them in lockstep with FP64 for the energy error
+    forms = {}
+    runners = {"headline": mixed}
+    if not args.no_forms:
+        for name, prec, tprec, hi, mp in FORMS:
+            r = Runner(env, prec, tprec, hi, mixed_precision=mp)
+            st = r.init()
+            st, t_f, _ = r.timed(st, args.warmup, args.steps, name)
+            del st
+            rate = W * args.steps / t_f
+            forms[name] = {"precision": r.pctx.precision, "taylor_precision": r.pctx.taylor_precision,
+                           "meas_precision": r.meas_precision, "walker_steps_per_s": round(rate, 2),
+                           "ms_per_step": round(1e3 * t_f / args.steps, 3),
+                           "speedup_vs_fp64": None if rate64 is None else round(rate / rate64, 3)}
+            runners[name] = r
 
     accuracy = None
     if fp64 is not None and args.accuracy_steps > 0:
-        # same sizes and draws, L scaled by 1/sqrt(N) so that walkers survive (see synthetic.make_synthetic)
-        del st64, st_mix
-        env_acc = dict(env, ham=HamChol(syn.h0, env["ham"].h1, syn.chol * (1.0 / np.sqrt(N))))
-        mixed_acc = Runner(env_acc, args.precision, args.taylor_precision, args.taylor_hi_from)
-        fp64_acc = Runner(env_acc, "f64", "f64")
-        accuracy = paired_accuracy(env_acc, mixed_acc, fp64_acc, args.accuracy_steps)
-        accuracy["system"] = "same sizes and seed, every L_gamma scaled by 1/sqrt(N)"
-        del mixed_acc, fp64_acc, env_acc
-        st_mix = mixed.init()
+        acc = paired_accuracy(env, runners, fp64, args.accuracy_steps)
+        accuracy = acc.pop("headline")
+        for name in forms:
+            forms[name]["accuracy_vs_fp64"] = acc[name]
+    for name in forms:
+        del runners[name]
+    del runners
+
+    unscaled = None
+    if not args.no_unscaled:
+        # the recipe as written (no 1/sqrt(N)): same flops, walkers overflow within a few steps
+        env_u = dict(env, ham=HamChol(syn.h0, env["ham"].h1, syn.chol * float(np.sqrt(N))))
+        r_u = Runner(env_u, args.precision, args.taylor_precision, args.taylor_hi_from)
+        st_u = r_u.init()
+        st_u, t_u, _ = r_u.timed(st_u, args.warmup, args.steps, "mixed, unscaled recipe")
+        unscaled = {"system": "the recipe as written (L_gamma without 1/sqrt(N))",
+                    "mixed_ms_per_step": round(1e3 * t_u / args.steps, 3),
+                    "mixed_walker_steps_per_s": round(W * args.steps / t_u, 2),
+                    "finite_weights_after": int(torch.isfinite(st_u.weights).sum()),
+                    "weights_in_floor_cap_after": int(((st_u.weights > 0) & (st_u.weights < 100)).sum())}
+        del r_u, st_u, env_u
+        torch.cuda.empty_cache()
 
     # roofline of the dominant kernel (VHS GEMM of the mixed step): flops the algorithm run here needs
     sym = bool(mixed.pctx.low.get("symblk"))
@@ -232,9 +282,13 @@ def main():
     vhs_flop = 4.0 * W * G * cols  # real x complex MAC = 4 flop over the stored V entries
     achieved = vhs_flop * passes / (vhs_ms * 1e-3) / 1e12
     peak = PEAK[args.precision]
+    tag = f"vhs_{args.precision}_{args.workload}"
     roofline = {"bound": "mfma", "kernel": f"VHS GEMM ({args.precision}, {'sym-block' if sym else 'dense'} V)",
                 "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                "traffic": traffic_from_profiles(f"vhs_{args.precision}_{args.workload}"),
+                "traffic": traffic_from_profiles(tag),
+                "traffic_source": f"profiles/{tag}_traffic.md: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over "
+                                  "this bench command, each scaled by the counter's reading for a 1 GiB device copy "
+                                  "(not measured inside this run)",
                 "launch_ms": round(vhs_ms, 3), "flop_per_launch": vhs_flop * passes}
 
     # the whole step against its ideal: T* = sum_k F_k / peak_k over the contractions as run here
@@ -269,22 +323,14 @@ def main():
     block_rate = None
     block_graph = None
     small = None
-    if not args.no_block:
-        # The tail is timed on walkers that stay finite: with the recipe's unscaled L the walkers overflow
-        # within a few steps, and the exchange GEMM then runs on degenerate operands about twice as fast
-        # (measured) as on real ones.  The step itself times the same on both (21.7 vs 22.3 ms).
-        env_blk = dict(env, ham=HamChol(syn.h0, env["ham"].h1, syn.chol * (1.0 / np.sqrt(N))))
-        del st_mix
-        mixed_blk = Runner(env_blk, args.precision, args.taylor_precision, args.taylor_hi_from)
-        st_blk = mixed_blk.init()
-        for _ in range(2):
-            st_blk = mixed_blk.step(st_blk)
-        block_rate = block_inclusive(env_blk, mixed_blk, st_blk, t_mix / args.steps)
-        block_graph = graph_block_rate(env_blk, mixed_blk, st_blk)
-        del mixed_blk, st_blk, env_blk
-
     fp_rates = None
     if not args.no_block:
+        st_blk = mixed.init()
+        for _ in range(2):
+            st_blk = mixed.step(st_blk)
+        block_rate = block_inclusive(env, mixed, st_blk, t_mix / args.steps)
+        block_graph = graph_block_rate(env, mixed, st_blk)
+        del st_blk
         small = launch_bound_case()
         fp_rates = free_projection_rates(syn, env, args)
 
@@ -308,16 +354,20 @@ def main():
                  + (f" 1-{args.taylor_hi_from - 1}, one-pass bf16 Taylor terms {args.taylor_hi_from}-6"
                     if args.taylor_hi_from > 0 else "")
                  + "; f64 walkers, det, inverse, QR, weights and energy sums",
-        "data": "synthetic (seeded numpy PCG64 Hamiltonian, trial and walkers; on-device N(0,1) fields)",
+        "data": "synthetic (seeded numpy PCG64 Hamiltonian, trial and walkers, every L_gamma / sqrt(N); "
+                "on-device N(0,1) fields)",
         "config": {"workload": args.workload, "nbasis": N, "nocc": n, "nchol": G, "n_walkers": W, "dt": dt,
                    "n_exp_terms": env["params"].n_exp_terms, "trial": "RHF", "walker_kind": "restricted",
                    "precision": args.precision, "taylor_precision": args.taylor_precision,
-                   "taylor_hi_from": args.taylor_hi_from,
+                   "taylor_hi_from": args.taylor_hi_from, "chol_scale": "1/sqrt(N)",
                    "vhs_layout": "sym-block" if sym else "dense"},
+        "weights_after_timed_steps": [round(w_min, 6), round(w_max, 6)],
         "fp64_walker_steps_per_s": None if rate64 is None else round(rate64, 2),
         "fp64_ms_per_step": None if rate64 is None else round(1e3 * W / rate64, 3),
         "speedup_vs_fp64": None if rate64 is None else round(value / rate64, 3),
         "accuracy_vs_fp64": accuracy,
+        "precision_forms": forms,
+        "unscaled_recipe": unscaled,
         "graph_step_walker_steps_per_s": None if rate_graph is None else round(rate_graph, 2),
         "block_inclusive_walker_steps_per_s": None if block_rate is None else round(block_rate, 2),
         "block_graph": block_graph,
@@ -331,53 +381,62 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def paired_accuracy(env, mixed: Runner, fp64: Runner, steps: int, every: int = 20) -> dict:
-    """Both paths from one initial state with identical fields, compared every ``every`` steps.
+def paired_accuracy(env, forms: dict, fp64: Runner, steps: int, every: int = 50) -> dict:
+    """Every form and the FP64 path from one initial state with identical fields, compared every ``every``
+    steps.
 
-    The mixed path's local energies come from its own (reduced-precision) exchange GEMMs on its own
-    walkers; the reference values are the FP64 energies of the FP64 walkers.  The block energy is the
-    weighted mean of Re E_L with each path's own weights.  Reported: the largest per-walker and
-    block-energy differences over all checkpoints, and the values at the last one.
+    Each form's local energies come from its own (reduced-precision) exchange kernels on its own walkers;
+    the reference values are the FP64 energies of the FP64 walkers.  The block energy is the weighted mean
+    of Re E_L with each path's own weights.  Reported per form: the largest per-walker and block-energy
+    differences over all checkpoints, and the values at the last one.
     """
     import torch
 
     from trot_amd.meas import _h0
 
     W = env["params"].n_walkers
-    G = mixed.pctx.chol_flat.shape[0]
+    G = fp64.pctx.chol_flat.shape[0]
     h0 = _h0(env["ham"])
-    a, b = mixed.init(), fp64.init()
-    worst = {"walker": 0.0, "block": 0.0}
-    last = {}
+    states = {k: r.init() for k, r in forms.items()}
+    b = fp64.init()
+    worst = {k: {"walker": 0.0, "block": 0.0} for k in forms}
+    last = {k: {} for k in forms}
+
+    def energies(r, st):
+        _, g = r.mops.engine.overlap_green(st.walkers, env["tr"])
+        return r.mops.engine.energy_from_green(g, h0, r.mctx).real
+
     for s in range(steps):
         g = torch.Generator(device="cuda")
         g.manual_seed(9000 + s)
         f = torch.randn((W, G), dtype=torch.float64, device="cuda", generator=g)
-        a = mixed.step(a, f)
         b = fp64.step(b, f)
+        for k, r in forms.items():
+            states[k] = r.step(states[k], f)
         if (s + 1) % every and s + 1 != steps:
             continue
-        _, ga = mixed.mops.engine.overlap_green(a.walkers, env["tr"])
-        _, gb = fp64.mops.engine.overlap_green(b.walkers, env["tr"])
-        ea = mixed.mops.engine.energy_from_green(ga, h0, mixed.mctx).real
-        eb = fp64.mops.engine.energy_from_green(gb, h0, fp64.mctx).real
-        blk_a = float((a.weights * ea).sum() / a.weights.sum())
+        eb = energies(fp64, b)
         blk_b = float((b.weights * eb).sum() / b.weights.sum())
-        dw = 1e3 * float((ea - eb).abs().max())
-        worst["walker"] = max(worst["walker"], dw)
-        worst["block"] = max(worst["block"], 1e3 * abs(blk_a - blk_b))
-        last = {"per_walker_max_abs_dE_mHa": round(dw, 6),
-                "per_walker_rms_dE_mHa": round(1e3 * float(((ea - eb) ** 2).mean().sqrt()), 6),
-                "block_energy_dE_mHa": round(1e3 * abs(blk_a - blk_b), 6), "block_energy_fp64_Ha": round(blk_b, 8),
-                "max_rel_weight_diff": float(((a.weights - b.weights).abs()
-                                              / b.weights.abs().clamp_min(1e-300)).max())}
-        log(f"accuracy at paired step {s + 1}: {last}")
-    out = {"steps": steps, "checkpoint_every": every,
-           "max_over_checkpoints": {"per_walker_max_abs_dE_mHa": round(worst["walker"], 6),
-                                    "block_energy_dE_mHa": round(worst["block"], 6)},
-           "final": last,
-           "energies": "mixed E_L from the mixed path's own exchange GEMMs on its walkers vs FP64 E_L of FP64 walkers"}
-    return out
+        for k, r in forms.items():
+            a = states[k]
+            ea = energies(r, a)
+            blk_a = float((a.weights * ea).sum() / a.weights.sum())
+            dw = 1e3 * float((ea - eb).abs().max())
+            worst[k]["walker"] = max(worst[k]["walker"], dw)
+            worst[k]["block"] = max(worst[k]["block"], 1e3 * abs(blk_a - blk_b))
+            last[k] = {"per_walker_max_abs_dE_mHa": round(dw, 6),
+                       "per_walker_rms_dE_mHa": round(1e3 * float(((ea - eb) ** 2).mean().sqrt()), 6),
+                       "block_energy_dE_mHa": round(1e3 * abs(blk_a - blk_b), 6),
+                       "block_energy_fp64_Ha": round(blk_b, 8),
+                       "max_rel_weight_diff": float(((a.weights - b.weights).abs()
+                                                     / b.weights.abs().clamp_min(1e-300)).max())}
+            log(f"accuracy of {k} at paired step {s + 1}: {last[k]}")
+    return {k: {"steps": steps, "checkpoint_every": every,
+                "max_over_checkpoints": {"per_walker_max_abs_dE_mHa": round(worst[k]["walker"], 6),
+                                         "block_energy_dE_mHa": round(worst[k]["block"], 6)},
+                "final": last[k],
+                "energies": "E_L from the form's own exchange kernels on its own walkers vs FP64 E_L of FP64 walkers"}
+            for k in forms}
 
 
 def block_inclusive(env, r: Runner, st, t_step: float) -> float:
@@ -496,14 +555,14 @@ def launch_bound_case() -> dict:
 
 def free_projection_rates(syn, env, args) -> dict:
     """Free-projection step (trot_amd.fp, ten Taylor terms, QR norm and comb every step) at the workload's
-    sizes, mixed and FP64, on the scaled system (every L_gamma / sqrt(N)) so that the walkers stay finite."""
+    sizes, mixed and FP64, on the workload's (scaled) system."""
     import torch
 
     from trot_amd import fp, meas
     from trot_amd.core import HamChol
 
     N = env["sys"].norb
-    ham = HamChol(syn.h0, env["ham"].h1, syn.chol * (1.0 / np.sqrt(N)))
+    ham = env["ham"]  # syn.chol already carries the 1/sqrt(N)
     W = env["params"].n_walkers
     params = fp.QmcParamsFp(dt=env["params"].dt, n_walkers=W, seed=11, ene0=0.0)
     out = {"workload": args.workload, "n_exp_terms": fp.FP_TAYLOR_TERMS}
@@ -622,26 +681,30 @@ def cpu_baseline(syn, r: Runner, args, dt):
     ops = O.RhfOps(overlap=lambda w: O.overlap_r(w, c), force_bias=lambda w: O.force_bias_rw(w, mctx, c),
                    energy=None)
     w0 = syn.walkers[:W].cpu().numpy()
-    st = O.PropState(walkers=w0, weights=np.ones(W), overlaps=ops.overlap(w0), pop_control_ene_shift=0.0,
-                     e_estimate=0.0)
+    e_est = float(r.init().e_estimate.item())  # walker 0's energy, as the GPU state starts
+    st = O.PropState(walkers=w0, weights=np.ones(W), overlaps=ops.overlap(w0), pop_control_ene_shift=e_est,
+                     e_estimate=e_est)
     params = O.QmcParams(dt=dt, n_walkers=W)
     rng = np.random.default_rng(0)
     G = chol_flat.shape[0]
     log(f"cpu baseline: {W} walkers, warm-up step ({threads} BLAS threads)")
-    # the workload's unscaled L (0.1 N(0,1), no 1/sqrt(N)) drives determinants past the double range within a
-    # few steps, on the GPU paths as on this one; the timing is of the same arithmetic either way
-    with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+    st = O.afqmc_step(st, rng.standard_normal((W, G)), ctx, ops, params)
+    t0 = time.perf_counter()
+    for s in range(args.cpu_steps):
         st = O.afqmc_step(st, rng.standard_normal((W, G)), ctx, ops, params)
-        t0 = time.perf_counter()
-        for s in range(args.cpu_steps):
-            st = O.afqmc_step(st, rng.standard_normal((W, G)), ctx, ops, params)
-            log(f"cpu baseline: step {s + 1}/{args.cpu_steps} at {time.perf_counter() - t0:.1f}s")
-        dt_cpu = time.perf_counter() - t0
+        log(f"cpu baseline: step {s + 1}/{args.cpu_steps} at {time.perf_counter() - t0:.1f}s")
+    dt_cpu = time.perf_counter() - t0
     rate = W * args.cpu_steps / dt_cpu
-    log(f"cpu baseline: {rate:.2f} walker-steps/s")
+    log(f"cpu baseline: {rate:.2f} walker-steps/s, weights in [{st.weights.min():.4f}, {st.weights.max():.4f}]")
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        affinity = None
     return {"value": round(rate, 3), "unit": "walker-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{W} of the workload's 2048 walkers x {args.cpu_steps} FP64 steps after 1 warm-up step, "
-                      f"numpy oracle (oracle/afqmc_oracle.py), same Hamiltonian and trial, {dt_cpu:.1f} s"}
+            "host_cpus": os.cpu_count(), "affinity_cpus": affinity,
+            "sample": f"{W} of the workload's {syn.walkers.shape[0]} walkers x {args.cpu_steps} FP64 steps after 1 "
+                      f"warm-up step, numpy oracle (oracle/afqmc_oracle.py) with {threads} BLAS threads, same "
+                      f"(scaled) Hamiltonian and trial, {dt_cpu:.1f} s"}
 
 
 if __name__ == "__main__":

@@ -30,3 +30,21 @@ def rel(a, b) -> float:
     b = np.asarray(b)
     den = max(float(np.max(np.abs(b))), 1e-300)
     return float(np.max(np.abs(a - b))) / den
+
+
+def rel_each(a, b, floor: float = 1e-6) -> float:
+    """Largest elementwise relative error |a_i - b_i| / max(|b_i|, floor max|b|).
+
+    Every entry must be right to the tolerance, not only the large ones: a small weight or overlap that is
+    wrong by orders of magnitude fails here and would pass rel().  Entries below floor x the largest are held
+    to that floor (masked weights are exact zeros on both sides).
+    """
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.shape != b.shape:
+        raise ValueError(f"shapes differ: {a.shape} vs {b.shape}")
+    if a.size == 0:
+        return 0.0
+    big = float(np.max(np.abs(b)))
+    den = np.maximum(np.abs(b), max(floor * big, 1e-300))
+    return float(np.max(np.abs(a - b) / den))

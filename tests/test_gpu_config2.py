@@ -7,18 +7,28 @@ population-control shift couples them): walkers, weights, overlaps and the shift
 relative after every step.  The recipe as written drives determinants past the double range from the
 fourth step on (oracle and GPU alike), after which two elimination orders no longer agree to 1e-10 on
 the overflowing values.  Over the full 100 steps the GPU run is repeated and must be bit-identical.
+
+On the same draws with every L_g scaled by 1/sqrt(N) the walkers keep O(1) weights, and all 100 steps of all
+1024 walkers meet the oracle (tests/golden/config2_scaled_100steps.npz, written by the oracle with
+tests/golden/make_config2_golden.py): the population shift and node count after every step, and weights,
+overlaps and four random linear functionals of every walker matrix after steps 1, 10, 50 and 100, elementwise
+at 1e-10.
 """
 
 from __future__ import annotations
+
+from pathlib import Path
 
 import numpy as np
 import pytest
 import torch
 
 from oracle import afqmc_oracle as O
-from tests.cases import rel
+from tests.cases import rel, rel_each
 
 pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden" / "config2_scaled_100steps.npz"
 
 
 def _rel_finite(a, b) -> float:
@@ -79,3 +89,44 @@ def test_config2_full_size_steps(gpu):
     assert np.array_equal(runs[0][0], runs[1][0], equal_nan=True)
     assert np.array_equal(runs[0][1], runs[1][1], equal_nan=True)
     assert runs[0][2] == runs[1][2] or (np.isnan(runs[0][2]) and np.isnan(runs[1][2]))
+
+
+def test_config2_scaled_100_steps_against_the_oracle(gpu):
+    from trot_amd import meas, prop, synthetic, trial
+    from trot_amd.core import HamChol, System
+
+    gold = np.load(GOLD)
+    cfg = synthetic.WORKLOADS["n256_o32_g1024_w1024"]
+    N, n, G, W, dt = cfg["norb"], cfg["nocc"], cfg["nchol"], cfg["n_walkers"], cfg["dt"]
+    syn = synthetic.make_synthetic(N, n, G, W, cfg["seed"], cfg["tau"], chol_scale=1.0 / np.sqrt(N))
+    sys_ = System(N, (n, n), "restricted")
+    ham = HamChol(syn.h0, syn.h1, syn.chol).to_device()
+    tr = trial.RhfTrial(syn.trial)
+    mops = meas.make_rhf_meas_ops(sys_)
+    tops = trial.make_rhf_trial_ops(sys_)
+    pops = prop.make_prop_ops("restricted", "restricted")
+    params = prop.QmcParams(dt=dt, n_walkers=W, seed=cfg["seed"])
+    pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
+    mctx = mops.build_meas_ctx(ham, tr)
+    st = pops.init_prop_state(sys=sys_, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops, params=params,
+                              initial_walkers=torch.from_numpy(syn.walkers).cuda())
+    assert abs(st.e_estimate.item() - float(gold["e_estimate"])) <= 1e-10 * abs(float(gold["e_estimate"]))
+    r = torch.from_numpy(gold["probe_r"]).cuda()
+    c = torch.from_numpy(gold["probe_c"]).cuda()
+    for s in range(100):
+        f = torch.from_numpy(synthetic.fields(cfg["seed"], s, W, G)).cuda()
+        st = pops.step(st, params=params, ham_data=ham, trial_data=tr, trial_ops=tops, meas_ops=mops, meas_ctx=mctx,
+                       prop_ctx=pctx, fields=f)
+        k = s + 1
+        pop = st.pop_control_ene_shift.item()
+        assert abs(pop - gold["pop_shift"][s]) <= 1e-10 * abs(gold["pop_shift"][s]), k
+        assert int(st.node_encounters.item()) == int(gold["nodes"][s]), k
+        if f"weights_{k}" in gold:
+            assert rel_each(st.weights.cpu().numpy(), gold[f"weights_{k}"]) < 1e-10, k
+            assert rel_each(st.overlaps.cpu().numpy(), gold[f"overlaps_{k}"]) < 1e-10, k
+            probes = ((r @ st.walkers) * c).sum(-1).cpu().numpy()  # r_j^T w c_j, (W, 4)
+            assert rel_each(probes, gold[f"probes_{k}"]) < 1e-10, k
+            fro = torch.linalg.vector_norm(st.walkers, dim=(1, 2)).cpu().numpy()
+            assert rel_each(fro, gold[f"fro_{k}"]) < 1e-10, k
+            print(f"[config2] step {k}: weights in [{st.weights.min().item():.4f}, {st.weights.max().item():.4f}]",
+                  flush=True)

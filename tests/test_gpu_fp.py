@@ -17,7 +17,7 @@ import pytest
 import torch
 
 from oracle import afqmc_oracle as O
-from tests.cases import problem, rel
+from tests.cases import problem, rel, rel_each
 
 pytestmark = pytest.mark.gpu
 
@@ -93,7 +93,7 @@ def test_fp_step_matches_oracle(gpu, kind, shape):
                             zeta=float(c["zetas"][s]))
         ref = O.afqmc_step_fp(ref, c["fields"][s], float(c["zetas"][s]), c["ref_ctx"], kind)
         assert _cmp(st.walkers, ref.walkers) < TOL, s
-        assert rel(st.weights.cpu().numpy(), ref.weights) < TOL, s
+        assert rel_each(st.weights.cpu().numpy(), ref.weights) < TOL, s
     assert st.rng_key[1] == 4  # one field draw and one comb offset per step
 
 
@@ -111,8 +111,8 @@ def test_fp_block_matches_oracle(gpu, kind):
     ref_params = O.QmcParamsFp(dt=0.01, n_prop_steps=4, n_qr_blocks=2, ene0=-1.0, n_walkers=12)
     ref, robs = O.block_fp(ref, c["fields"], [float(z) for z in c["zetas"]], c["ref_ctx"], c["ref_ops"], ref_params)
     assert _cmp(st.walkers, ref.walkers) < TOL
-    assert rel(st.weights.cpu().numpy(), ref.weights) < TOL
-    assert rel(st.overlaps.cpu().numpy(), ref.overlaps) < TOL
+    assert rel_each(st.weights.cpu().numpy(), ref.weights) < TOL
+    assert rel_each(st.overlaps.cpu().numpy(), ref.overlaps) < TOL
     for k in ("energy", "weight", "overlap", "abs_overlap"):
         g = complex(obs.scalars[k].item())
         assert abs(g - complex(robs[k])) <= TOL * max(abs(complex(robs[k])), 1e-300), k

@@ -7,7 +7,7 @@ import pytest
 import torch
 
 from oracle import afqmc_oracle as O
-from tests.cases import problem, rel
+from tests.cases import problem, rel, rel_each
 
 pytestmark = pytest.mark.gpu
 
@@ -31,7 +31,7 @@ def test_overlap_green(gpu, shape):
     ref_ov = O.overlap_r(p["wa"], p["ca"])
     ref_g = O.half_green(p["wa"], p["ca"])  # (W, n, N)
     torch.cuda.synchronize()
-    assert rel(ov.cpu().numpy(), ref_ov) < TOL
+    assert rel_each(ov.cpu().numpy(), ref_ov) < TOL
     assert rel(gt.cpu().numpy(), np.swapaxes(ref_g, 1, 2)) < TOL
 
 
@@ -124,7 +124,7 @@ def test_uhf_force_bias_energy_overlap(gpu, nab):
     w = (_t(p["wa"]), _t(p["wb"]))
     ref = O.make_uhf_ops(p["h0"], p["h1"], p["chol"], p["ca"], p["cb"])
     wn = (p["wa"], p["wb"])
-    assert rel(ops.overlap(w, tr).cpu().numpy(), ref.overlap(wn)) < TOL
+    assert rel_each(ops.overlap(w, tr).cpu().numpy(), ref.overlap(wn)) < TOL
     assert rel(ops.require_kernel("force_bias")(w, ham, ctx, tr).cpu().numpy(), ref.force_bias(wn)) < TOL
     assert rel(ops.require_kernel("energy")(w, ham, ctx, tr).cpu().numpy(), ref.energy(wn)) < TOL
 

@@ -12,7 +12,7 @@ import pytest
 import torch
 
 from oracle import afqmc_oracle as O
-from tests.cases import problem, rel
+from tests.cases import problem, rel, rel_each
 
 pytestmark = pytest.mark.gpu
 
@@ -95,8 +95,8 @@ def test_steps_match_oracle(gpu, case, layout):
                           meas_ctx=mctx, prop_ctx=pctx, fields=torch.from_numpy(f).cuda())
         ref = O.afqmc_step(ref, f, ref_ctx, ref_ops, ref_params)
         assert _cmp_walkers(state.walkers, ref.walkers) < TOL, s
-        assert rel(state.overlaps.cpu().numpy(), ref.overlaps) < TOL, s
-        assert rel(state.weights.cpu().numpy(), ref.weights) < TOL, s
+        assert rel_each(state.overlaps.cpu().numpy(), ref.overlaps) < TOL, s
+        assert rel_each(state.weights.cpu().numpy(), ref.weights) < TOL, s
         assert abs(state.pop_control_ene_shift.item() - ref.pop_control_ene_shift) < 1e-10 * max(
             1.0, abs(ref.pop_control_ene_shift)), s
         assert int(state.node_encounters.item()) == ref.node_encounters
@@ -169,8 +169,8 @@ def test_block_matches_oracle(gpu, kind):
     assert abs(obs.scalars["weight"].item() - robs["weight"]) < 1e-10 * abs(robs["weight"])
     assert abs(state.e_estimate.item() - ref.e_estimate) < 1e-10 * abs(ref.e_estimate)
     assert _cmp_walkers(state.walkers, ref.walkers) < TOL
-    assert rel(state.overlaps.cpu().numpy(), ref.overlaps) < TOL
-    assert rel(state.weights.cpu().numpy(), ref.weights) < 1e-13
+    assert rel_each(state.overlaps.cpu().numpy(), ref.overlaps) < TOL
+    assert rel_each(state.weights.cpu().numpy(), ref.weights) < 1e-13
 
 
 def test_run_qmc_short(gpu):

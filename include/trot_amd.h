@@ -154,7 +154,9 @@ size_t trot_weighted_mean_workspace(int W, long L);
 int trot_weighted_mean(const double* x, const double* weights, const double* wsum, int W, long L, double* out,
                        void* ws, size_t ws_bytes, void* stream);
 
-/* In place Q factor of w with real positive diag(R) (two Cholesky-QR passes).
+/* In place Q factor of w with real positive diag(R) (two Cholesky-QR passes).  A walker whose Cholesky breaks
+ * down (a pivot not finite or below 1e-8 of its Gram diagonal: ill-conditioned, kappa above ~1e4..1e8) is redone
+ * from its original columns by Gram-Schmidt with reorthogonalisation.
  * Replaces: _qr / orthonormalize (trot/walkers.py:137-179). */
 size_t trot_orthonormalize_workspace(int W, int N, int n);
 int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t ws_bytes, void* stream);

@@ -360,3 +360,45 @@ def test_taylor_tile_shapes_bit_identical(gpu, shape, symblk):
     best, ms = _lib.autotune_taylor(W, N, n, symblk)
     assert 0 <= best < 4 and all(m > 0 for m in ms) and _lib.get_taylor_tile(W, N, n, symblk) == best
     assert np.array_equal(run(), outs[0])
+
+
+def _conditioned_walkers(rng, N, n, kappas):
+    """w = U diag(s) V^H with singular values from 1 down to 1/kappa (kappa = 1: orthonormal columns)."""
+    out = []
+    for kappa in kappas:
+        u = np.linalg.qr(rng.standard_normal((N, n)) + 1j * rng.standard_normal((N, n)))[0]
+        v = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+        s = np.logspace(0.0, -np.log10(kappa), n)
+        out.append((u * s) @ v.conj().T)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("N,n", [(40, 8), (130, 64)])
+def test_orthonormalize_ill_conditioned(gpu, N, n):
+    """Walkers with kappa 1e6 and 1e10 break Cholesky-QR (kappa^2 eps > 1); they are redone by Gram-Schmidt
+    with reorthogonalisation, next to well-conditioned walkers that stay on Cholesky-QR.  The result is the
+    reference's Q (trot/walkers.py:137-148, the oracle's qr_phase) to kappa eps, orthonormal to rounding,
+    spans the walker's columns, and R = Q^H w is upper triangular with a real positive diagonal."""
+    from trot_amd import _lib
+
+    rng = np.random.default_rng(44)
+    kappas = [1.0, 1e10, 3.0, 1e6, 1e10, 10.0]
+    w = _conditioned_walkers(rng, N, n, kappas)
+    q = _lib.orthonormalize(_t(w)).cpu().numpy()
+    q_ref, _ = O.qr_phase(w)
+    _, det_r = _lib.orthogonalize(_t(w))
+    det_norm = _lib.qr_norm(_t(w)).cpu().numpy()
+    for b, kappa in enumerate(kappas):
+        assert np.all(np.isfinite(q[b])), b
+        assert np.max(np.abs(q[b].conj().T @ q[b] - np.eye(n))) < 1e-12, b
+        r = q[b].conj().T @ w[b]
+        assert np.max(np.abs(w[b] - q[b] @ r)) < 1e-12 * np.max(np.abs(w[b])), b
+        assert np.max(np.abs(np.tril(r, -1))) < 1e-12 * np.max(np.abs(w[b])), b
+        d = np.diagonal(r)
+        assert np.all(d.real > 0) and np.max(np.abs(d.imag)) < 1e-12 * np.max(np.abs(w[b])), b
+        assert rel(q[b], q_ref[b]) < max(1e-10, 1e3 * kappa * 2.2e-16), b
+        # |det R| = product of the singular values (where that product is a normal double)
+        prod_s = np.prod(np.logspace(0.0, -np.log10(kappa), n))
+        if prod_s > 1e-280:
+            assert abs(det_r.cpu().numpy()[b] - prod_s) < max(1e-10, 1e3 * kappa * 2.2e-16) * prod_s, b
+            assert abs(det_norm[b] - prod_s) < max(1e-10, 1e3 * kappa * 2.2e-16) * prod_s, b

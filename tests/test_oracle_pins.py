@@ -304,3 +304,22 @@ def test_init_walkers_span_trial():
     w = O.init_walkers(O.rhf_rdm1(c), (n, n), 4, "restricted")
     assert w.shape == (4, N, n)
     assert np.allclose(np.abs(np.linalg.det(c.T @ w[0])), 1.0)
+
+
+def test_qr_phase_ill_conditioned():
+    """The reference's Householder QR with the phase fix (trot/walkers.py:137-148) stays orthonormal for
+    kappa = 1e10 walkers (where Cholesky-QR breaks down): Q^H Q = I, Q R = w, R upper triangular with a real
+    positive diagonal whose product is the product of the singular values."""
+    rng = np.random.default_rng(45)
+    N, n = 30, 6
+    u = np.linalg.qr(rng.standard_normal((N, n)) + 1j * rng.standard_normal((N, n)))[0]
+    v = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+    s = np.logspace(0.0, -10.0, n)
+    w = ((u * s) @ v.conj().T)[None]
+    q, det_r = O.qr_phase(w)
+    assert np.max(np.abs(q[0].conj().T @ q[0] - np.eye(n))) < 1e-13
+    r = q[0].conj().T @ w[0]
+    assert np.max(np.abs(q[0] @ r - w[0])) < 1e-14
+    assert np.max(np.abs(np.tril(r, -1))) < 1e-14
+    assert np.all(np.diagonal(r).real > 0) and np.max(np.abs(np.diagonal(r).imag)) < 1e-14
+    assert abs(det_r[0] - np.prod(s)) < 1e-5 * np.prod(s)

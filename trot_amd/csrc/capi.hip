@@ -502,10 +502,13 @@ int trot_block_energy(const double* energy, double* weights, double* scalars, do
 // ---------------------------------------------------------------------------
 // orthonormalisation by two passes of Cholesky-QR: Q = w R^-1 with R the
 // Cholesky factor of w^H w (real positive diagonal, the phase convention of
-// trot/walkers.py:137-148).
+// trot/walkers.py:137-148).  A walker whose Cholesky breaks down in either pass
+// (ill-conditioned: a pivot not finite or below 1e-8 of its Gram diagonal) is
+// redone from its original columns by Gram-Schmidt with reorthogonalisation
+// before the second pass's product, which then multiplies it by the identity.
 // ---------------------------------------------------------------------------
 size_t trot_orthonormalize_workspace(int W, int N, int n) {
-  return 2 * align256((size_t)W * n * n * 16) + align256((size_t)W * N * n * 16);
+  return 2 * align256((size_t)W * n * n * 16) + align256((size_t)W * N * n * 16) + align256((size_t)W * 4);
 }
 
 int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t ws_bytes, void* stream) {
@@ -520,7 +523,8 @@ int trot_orthogonalize(double* walkers, int W, int N, int n, double* det_r, void
   double* gram = c.take<double>((size_t)W * n * n * 2);
   double* rinv = c.take<double>((size_t)W * n * n * 2);
   double* tmp = c.take<double>((size_t)W * N * n * 2);
-  if (!gram || !rinv || !tmp) return fail(TROT_ERR_WORKSPACE, "orthonormalize: workspace too small");
+  int* flags = c.take<int>((size_t)W);
+  if (!gram || !rinv || !tmp || !flags) return fail(TROT_ERR_WORKSPACE, "orthonormalize: workspace too small");
   hipStream_t s = (hipStream_t)stream;
   double* src = walkers;
   double* dst = tmp;
@@ -531,7 +535,9 @@ int trot_orthogonalize(double* walkers, int W, int N, int n, double* det_r, void
     p.C = gram; p.ldc = n; p.sC = (long)n * n;
     p.M = n; p.N = n; p.K = N; p.ar = 1.0;
     HIPCHK((launch_gemm_f64<64, 64, true, true, false, true, true, EPI_STORE>(p, W, s)));
-    HIPCHK(launch_chol_rinv(gram, rinv, n, W, s, det_r, pass));
+    HIPCHK(launch_chol_rinv(gram, rinv, n, W, s, det_r, pass, flags));
+    // the walkers are still intact here: broken ones get Q in tmp and R^-1 = I for the second product
+    if (pass == 1) HIPCHK(launch_qr_fallback(walkers, tmp, flags, N, n, W, rinv, det_r, s));
     GemmP q{};
     q.A = src; q.lda = n; q.sA = (long)N * n;
     q.B = rinv; q.ldb = n; q.sB = (long)n * n;
@@ -544,7 +550,9 @@ int trot_orthogonalize(double* walkers, int W, int N, int n, double* det_r, void
   return TROT_OK;
 }
 
-size_t trot_qr_norm_workspace(int W, int N, int n) { return align256((size_t)W * n * n * 16); }
+size_t trot_qr_norm_workspace(int W, int N, int n) {
+  return align256((size_t)W * n * n * 16) + align256((size_t)W * N * n * 16) + align256((size_t)W * 4);
+}
 
 int trot_qr_norm(const double* walkers, int W, int N, int n, double* det_r, void* ws, size_t ws_bytes, void* stream) {
   if (!walkers || !det_r) return fail(TROT_ERR_INVALID, "qr_norm: null pointer");
@@ -552,7 +560,9 @@ int trot_qr_norm(const double* walkers, int W, int N, int n, double* det_r, void
   if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "qr_norm: nocc > 64");
   Carve c{(char*)ws, ws_bytes, 0};
   double* gram = c.take<double>((size_t)W * n * n * 2);
-  if (!gram) return fail(TROT_ERR_WORKSPACE, "qr_norm: workspace too small");
+  double* q = c.take<double>((size_t)W * N * n * 2);
+  int* flags = c.take<int>((size_t)W);
+  if (!gram || !q || !flags) return fail(TROT_ERR_WORKSPACE, "qr_norm: workspace too small");
   hipStream_t s = (hipStream_t)stream;
   GemmP p{};
   p.A = walkers; p.lda = n; p.sA = (long)N * n;
@@ -560,7 +570,8 @@ int trot_qr_norm(const double* walkers, int W, int N, int n, double* det_r, void
   p.C = gram; p.ldc = n; p.sC = (long)n * n;
   p.M = n; p.N = n; p.K = N; p.ar = 1.0;
   HIPCHK((launch_gemm_f64<64, 64, true, true, false, true, true, EPI_STORE>(p, W, s)));
-  HIPCHK(launch_chol_rinv(gram, nullptr, n, W, s, det_r, 0));
+  HIPCHK(launch_chol_rinv(gram, nullptr, n, W, s, det_r, 0, flags));
+  HIPCHK(launch_qr_fallback(walkers, q, flags, N, n, W, nullptr, det_r, s));  // |det R| of broken walkers
   return TROT_OK;
 }
 

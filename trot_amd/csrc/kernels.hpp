@@ -63,7 +63,9 @@ hipError_t launch_weight_update(double* weights, const double* ov_old, const dou
                                 double h0p_re, double h0p_im, double w_floor, double w_cap, double damping,
                                 hipStream_t s);
 hipError_t launch_chol_rinv(const double* gram, double* rinv, int n, int W, hipStream_t s, double* dprod = nullptr,
-                            int accumulate = 0);
+                            int accumulate = 0, int* flags = nullptr);
+hipError_t launch_qr_fallback(const double* src, double* q, const int* flags, int N, int n, int W, double* rinv,
+                              double* dprod, hipStream_t s);
 // zeta_dev: comb offset read on the device when non-null (else zeta)
 hipError_t launch_sr_index(const double* weights, double* cw, int* idx, double* w_new, int W, double zeta,
                            const double* zeta_dev, hipStream_t s);

@@ -6,8 +6,9 @@
 //                         (trot/prop/afqmc.py:99-103)
 //   weight_update_kernel  phaseless importance, masking, population shift
 //                         (trot/prop/afqmc.py:108-134)
-//   chol_rinv_kernel      Gram matrix -> R^-1 of its Cholesky factor (CholQR)
+//   chol_rinv_kernel      Gram matrix -> R^-1 of its Cholesky factor (CholQR), breakdown flags
 //                         (trot/walkers.py:137-179)
+//   qr_fallback_kernel    Gram-Schmidt QR (twice) of the walkers whose CholQR broke down
 //   sr_index_kernel       comb stochastic reconfiguration indices
 //                         (trot/walkers.py:249-293)
 //   gather_rows_kernel    walker / cache gather after SR
@@ -335,8 +336,13 @@ __global__ __launch_bounds__(1024) void update_grouped_kernel(const UpdProb* __r
 // ---------------------------------------------------------------------------
 // dprod (optional): product of the Cholesky diagonal, prod_k L_kk = |det R| of w = Q R, written
 // (accumulate = 0) or multiplied in (accumulate = 1, the second CholQR pass).  rinv null: diagonal only.
+// flags (optional): 1 for a walker whose Cholesky breaks down, i.e. some pivot is not finite or falls below
+// CHOLQR_PIVOT_TOL times its Gram diagonal entry (written by the first pass, or-ed in by the second); such a
+// walker is redone by qr_fallback_kernel.
+#define CHOLQR_PIVOT_TOL 1.0e-8
 __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict__ s, double* __restrict__ rinv,
-                                                        int n, double* __restrict__ dprod, int accumulate) {
+                                                        int n, double* __restrict__ dprod, int accumulate,
+                                                        int* __restrict__ flags) {
   extern __shared__ double sh[];
   cplx* a = reinterpret_cast<cplx*>(sh);
   cplx* x = a + n * n;
@@ -346,8 +352,12 @@ __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict
   for (int e = t; e < n * n; e += 256) a[e] = cload(sw, e);
   __syncthreads();
   double dp = 1.0;
+  bool broken = false;
   for (int k = 0; k < n; ++k) {
-    const double d = sqrt(a[k * n + k].re);
+    const double piv = a[k * n + k].re;
+    // the Gram diagonal S_kk is still in global memory; pivot^2 >= sigma_min^2 while S_kk <= sigma_max^2
+    broken = broken || !(piv > CHOLQR_PIVOT_TOL * sw[2 * ((long)k * n + k)]) || !isfinite(piv);
+    const double d = sqrt(piv);
     dp *= d;
     __syncthreads();
     if (t == 0) a[k * n + k] = {d, 0.0};
@@ -363,6 +373,7 @@ __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict
     __syncthreads();
   }
   if (dprod && t == 0) dprod[w] = accumulate ? dprod[w] * dp : dp;
+  if (flags && t == 0) flags[w] = (accumulate ? flags[w] : 0) | (broken ? 1 : 0);
   if (!rinv) return;
   // R X = I, X upper: column j by thread j (back substitution)
   for (int e = t; e < n * n; e += 256) x[e] = {0.0, 0.0};
@@ -378,6 +389,68 @@ __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict
   __syncthreads();
   double* ow = rinv + 2 * w * n * n;
   for (int e = t; e < n * n; e += 256) cstore(ow, e, x[e]);
+}
+
+// ---------------------------------------------------------------------------
+// QR of the walkers whose Cholesky-QR broke down (flags[w] != 0; other workgroups return at once).  Classical
+// Gram-Schmidt with reorthogonalisation ("twice is enough": orthonormal to rounding for any numerically
+// full-rank w, unlike Cholesky-QR, which needs kappa(w) below ~1e8): column k is projected out of the
+// orthonormal columns before it twice, then normalised, so R_kk = |residual| is real and >= 0, the phase
+// convention of trot/walkers.py:141-148.  src (N x n, read) -> q (N x n, written); rinv (optional) set to the
+// identity for the walker, so that a following q rinv product keeps Q; dprod (optional) = prod_k R_kk.
+// A column in the span of the ones before it (exact rank deficiency) comes out as zeros and R_kk = 0.
+// One workgroup of 4 waves per walker; coefficients one per wave with a shuffle reduction (fixed order).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void qr_fallback_kernel(const double* __restrict__ src, double* __restrict__ q,
+                                                          const int* __restrict__ flags, int N, int n,
+                                                          double* __restrict__ rinv, double* __restrict__ dprod) {
+  extern __shared__ double sh[];
+  __shared__ double red[256];
+  cplx* coef = reinterpret_cast<cplx*>(sh);
+  const long w = blockIdx.x;
+  if (!flags[w]) return;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const long base = w * (long)N * n;
+  for (long e = t; e < (long)N * n; e += 256) cstore(q, base + e, cload(src, base + e));
+  __syncthreads();
+  double dp = 1.0;
+  for (int k = 0; k < n; ++k) {
+    for (int pass = 0; pass < 2 && k > 0; ++pass) {
+      for (int j = wave; j < k; j += 4) {
+        cplx acc = {0.0, 0.0};
+        for (int i = lane; i < N; i += 64) {
+          const cplx qj = cload(q, base + (long)i * n + j), qk = cload(q, base + (long)i * n + k);
+          acc = cadd(acc, cmul(cconj(qj), qk));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          acc.re += __shfl_xor(acc.re, off);
+          acc.im += __shfl_xor(acc.im, off);
+        }
+        if (lane == 0) coef[j] = acc;
+      }
+      __syncthreads();
+      for (int i = t; i < N; i += 256) {
+        cplx x = cload(q, base + (long)i * n + k);
+        for (int j = 0; j < k; ++j) x = csub(x, cmul(coef[j], cload(q, base + (long)i * n + j)));
+        cstore(q, base + (long)i * n + k, x);
+      }
+      __syncthreads();
+    }
+    double part = 0.0;
+    for (int i = t; i < N; i += 256) {
+      const cplx x = cload(q, base + (long)i * n + k);
+      part += x.re * x.re + x.im * x.im;
+    }
+    const double r = sqrt(block_sum<256>(part, red));
+    dp *= r;
+    const double inv = r > 0.0 ? 1.0 / r : 0.0;
+    for (int i = t; i < N; i += 256) cstore(q, base + (long)i * n + k, cscale(cload(q, base + (long)i * n + k), inv));
+    __syncthreads();
+  }
+  if (dprod && t == 0) dprod[w] = dp;
+  if (rinv)
+    for (int e = t; e < n * n; e += 256) cstore(rinv, w * (long)n * n + e, {e / n == e % n ? 1.0 : 0.0, 0.0});
 }
 
 // ---------------------------------------------------------------------------
@@ -502,13 +575,22 @@ hipError_t launch_weight_update(double* weights, const double* ov_old, const dou
 }
 
 hipError_t launch_chol_rinv(const double* gram, double* rinv, int n, int W, hipStream_t s, double* dprod,
-                            int accumulate) {
+                            int accumulate, int* flags) {
   if (W <= 0) return hipSuccess;
   if (n < 1 || n > TROT_MAX_NOCC) return hipErrorInvalidValue;
   const size_t lds = (size_t)2 * n * n * 16;
   hipError_t e = set_lds((const void*)chol_rinv_kernel, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(chol_rinv_kernel, dim3(W), dim3(256), lds, s, gram, rinv, n, dprod, accumulate);
+  hipLaunchKernelGGL(chol_rinv_kernel, dim3(W), dim3(256), lds, s, gram, rinv, n, dprod, accumulate, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_qr_fallback(const double* src, double* q, const int* flags, int N, int n, int W, double* rinv,
+                              double* dprod, hipStream_t s) {
+  if (W <= 0) return hipSuccess;
+  if (n < 1 || n > N) return hipErrorInvalidValue;
+  const size_t lds = (size_t)n * 16;
+  hipLaunchKernelGGL(qr_fallback_kernel, dim3(W), dim3(256), lds, s, src, q, flags, N, n, rinv, dprod);
   return hipGetLastError();
 }
 

@@ -414,6 +414,26 @@ int trot_update_problem(const double* det_a, const double* det_b, double* overla
                         double h0p_re, double h0p_im, double w_floor, double w_cap, double damping, void* desc);
 int trot_update_grouped(const void* probs, int nprob, void* stream);
 
+/* ---- building blocks of the hand-written adjoint (reverse mode) of the step and the block energy ----
+ * trot_gemm_batched: C[b] = alpha op(A[b]) op(B[b]) + beta C[b] for b < batch on FP64 MFMA, op TROT_OP_N (as
+ *   stored), TROT_OP_T (transpose), TROT_OP_C (conjugate transpose), TROT_OP_R (conjugate); A is M x K and B
+ *   K x N after op; real or interleaved-complex operands (conjugation of a real operand is an error), complex C
+ *   whenever either operand is complex; leading dimensions and batch strides in elements (a stride of 0
+ *   shares the operand), alpha complex, beta real.
+ * trot_inverse_det: minv[b] = m[b]^-1 and det[b] = det m[b] (complex n x n, n <= 64), the kernel of
+ *   trot_overlap_green.
+ * Replaces: the matrix products and the inverse that jax.grad takes through afqmc_step and the local energy
+ * (trot/prop/afqmc.py:79-144, trot/prop/utils.py:20-29, trot/meas/rhf.py:21-25, 128-138; the reference is
+ * differentiable end to end, README.md:11). */
+#define TROT_OP_N 0
+#define TROT_OP_T 1
+#define TROT_OP_C 2
+#define TROT_OP_R 3
+int trot_gemm_batched(int opa, int opb, int a_complex, int b_complex, int M, int N, int K, double alpha_re,
+                      double alpha_im, const double* A, long lda, long sA, const double* B, long ldb, long sB,
+                      double beta, double* C, long ldc, long sC, int batch, void* stream);
+int trot_inverse_det(const double* m, int W, int n, double* minv, double* det, void* stream);
+
 /* ---- CISD trial, restricted walkers (FP64) ----
  * The reference determinant occupies the first nocc orbitals; the active block is occupied orbitals
  * core .. nocc-1 (na = nocc - core) and virtuals nocc .. nocc+nvir-1; ci1 (na, nvir) and ci2 (na, nvir,

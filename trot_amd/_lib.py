@@ -75,6 +75,9 @@ _SIGS = {
     "trot_gather_rows": (_i, [_vp, _vp, _vp, _i, _l, _vp]),
     "trot_sr_comb_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _sz, _vp]),
     "trot_sr_comb_dev_noalloc": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "trot_gemm_batched": (_i, [_i, _i, _i, _i, _i, _i, _i, _d, _d, _vp, _l, _l, _vp, _l, _l, _d, _vp, _l, _l, _i,
+                               _vp]),
+    "trot_inverse_det": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
     "trot_gemm_problem_bytes": (_sz, []),
     "trot_gemm_problem": (_l, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _vp]),
     "trot_gemm_f64_grouped": (_i, [_i, _vp, _vp, _i, _i, _vp]),
@@ -378,6 +381,75 @@ def field_shift(fields: torch.Tensor, fb: torch.Tensor, mf: torch.Tensor, sqrt_d
     _check(lib().trot_field_shift(_p(fields), _p(fb), _p(mf), float(sqrt_dt), W, G, _p(xbar), _p(shift), _p(fbt),
                                   _stream()), "trot_field_shift")
     return xbar, shift, fbt
+
+
+_OPS = {"N": 0, "T": 1, "C": 2, "R": 3}
+
+
+def _mat(x: torch.Tensor, name: str):
+    """(batch or None, rows, cols, leading dim, batch stride, complex) of a 2-D or 3-D row-major operand view."""
+    if not isinstance(x, torch.Tensor) or x.device.type != "cuda":
+        raise ValueError(f"{name}: must be a GPU tensor")
+    if x.dtype not in (F64, C128):
+        raise TypeError(f"{name}: expected float64 or complex128, got {x.dtype}")
+    if x.dim() not in (2, 3):
+        raise ValueError(f"{name}: expected a 2-D or 3-D tensor, got shape {tuple(x.shape)}")
+    rows, cols = x.shape[-2], x.shape[-1]
+    if x.stride(-1) != 1 and cols > 1:
+        raise ValueError(f"{name}: rows must be contiguous")
+    ld = max(x.stride(-2), cols) if rows > 1 else cols
+    if x.stride(-2) < cols and rows > 1:
+        raise ValueError(f"{name}: overlapping rows")
+    batch = x.shape[0] if x.dim() == 3 else None
+    sb = x.stride(0) if (x.dim() == 3 and x.shape[0] > 1) else 0
+    return batch, rows, cols, ld, sb, x.is_complex()
+
+
+def gemm_batched(a: torch.Tensor, b: torch.Tensor, opa: str = "N", opb: str = "N", alpha: complex = 1.0,
+                 beta: float = 0.0, out: torch.Tensor | None = None) -> torch.Tensor:
+    """out[k] = alpha op(a[k]) op(b[k]) + beta out[k] on FP64 MFMA (trot_gemm_batched).
+
+    op: "N" as stored, "T" transpose, "C" conjugate transpose, "R" conjugate.  a and b are 2-D (shared by
+    every batch entry) or 3-D (batch, rows, cols) views with contiguous rows; out (batch, M, N), complex when
+    either operand is, may be a strided view (e.g. a column block of a wider matrix).
+    """
+    ba, ra, ca, lda, sa, ac = _mat(a, "a")
+    bb, rb, cb, ldb, sb, bc = _mat(b, "b")
+    if ba is not None and bb is not None and ba != bb:
+        raise ValueError(f"batch sizes differ: {ba} vs {bb}")
+    batch = ba if ba is not None else (bb if bb is not None else 1)
+    M, K = (ra, ca) if opa in ("N", "R") else (ca, ra)
+    Kb, Nc = (rb, cb) if opb in ("N", "R") else (cb, rb)
+    if K != Kb:
+        raise ValueError(f"inner dimensions differ: op(a) is {M}x{K}, op(b) is {Kb}x{Nc}")
+    cplx = ac or bc
+    dt = C128 if cplx else F64
+    if out is None:
+        if beta != 0.0:
+            raise ValueError("beta != 0 needs an output tensor")
+        out = torch.empty((batch, M, Nc), dtype=dt, device=a.device)
+    if out.dtype != dt or out.device != a.device:
+        raise TypeError(f"out: expected {dt} on {a.device}")
+    bo, ro, co, ldc, sc, _ = _mat(out, "out")
+    if (bo or 1) != batch or ro != M or co != Nc:
+        raise ValueError(f"out: expected shape ({batch}, {M}, {Nc}), got {tuple(out.shape)}")
+    if batch > 1 and sc == 0:
+        raise ValueError("out: needs a batch dimension")
+    al = complex(alpha)
+    _check(lib().trot_gemm_batched(_OPS[opa], _OPS[opb], int(ac), int(bc), M, Nc, K, al.real, al.imag, _p(a), lda, sa,
+                                   _p(b), ldb, sb, float(beta), _p(out), ldc, sc, batch, _stream()),
+           "trot_gemm_batched")
+    return out
+
+
+def inverse_det(m: torch.Tensor):
+    """(m^-1, det m) of a batch of complex n x n matrices (trot_inverse_det)."""
+    W, n, _ = m.shape
+    _req(m, "m", C128, (W, n, n))
+    minv = torch.empty_like(m)
+    det = torch.empty((W,), dtype=C128, device=m.device)
+    _check(lib().trot_inverse_det(_p(m), W, n, _p(minv), _p(det), _stream()), "trot_inverse_det")
+    return minv, det
 
 
 def vhs_build(xbar: torch.Tensor, chol_flat: torch.Tensor) -> torch.Tensor:

@@ -39,8 +39,9 @@ struct GemmP {
 
 // AKC: A[i][k] at i*lda + k (K contiguous), else at k*lda + i.
 // BNC: B[k][j] at k*ldb + j (N contiguous), else at j*ldb + k.
+// CONJA / CONJB: the complex operand enters conjugated.
 // ASYM: A is a symmetric matrix in sym-block storage (common.hpp); lda = nb.
-template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, bool ASYM = false>
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, bool ASYM = false, bool CONJB = false>
 struct TileF64 {
   static constexpr int BK = 16;
   static constexpr int PA = AC ? 2 : 1;
@@ -105,7 +106,7 @@ struct TileF64 {
         if (BC) {
           double2 v = ok ? *reinterpret_cast<const double2*>(B + 2 * off) : make_double2(0.0, 0.0);
           rb[r][0] = v.x;
-          rb[r][PB - 1] = v.y;
+          rb[r][PB - 1] = CONJB ? -v.y : v.y;
         } else {
           rb[r][0] = ok ? B[off] : 0.0;
         }
@@ -196,10 +197,10 @@ struct TileF64 {
 };
 
 // One output tile (m0, n0) of batch b of problem p (nbatch batches): the whole work of a workgroup.
-template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM>
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM, bool CONJB = false>
 __device__ __forceinline__ void gemm_f64_tile(const GemmP& p, int b, int nbatch, int m0, int n0,
-                                              typename TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM>::Smem& sm) {
-  typedef TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM> T;
+                                              typename TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM, CONJB>::Smem& sm) {
+  typedef TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM, CONJB> T;
   const double* A = p.A + (AC ? 2 : 1) * (long)b * p.sA;
   const double* B = p.B + (BC ? 2 : 1) * (long)b * p.sB;
   typename T::Acc acc;
@@ -245,11 +246,12 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmP& p, int b, int nbatch,
       }
 }
 
-template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false>
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false,
+          bool CONJB = false>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
-  __shared__ typename TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM>::Smem sm;
-  gemm_f64_tile<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM>(p, blockIdx.z, gridDim.z, blockIdx.y * BM,
-                                                            blockIdx.x * BN, sm);
+  __shared__ typename TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM, CONJB>::Smem sm;
+  gemm_f64_tile<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM, CONJB>(p, blockIdx.z, gridDim.z, blockIdx.y * BM,
+                                                                   blockIdx.x * BN, sm);
 }
 
 // Many independent problems in one launch: problem i owns workgroups start[i] .. start[i+1] - 1, laid out
@@ -287,11 +289,12 @@ static hipError_t launch_gemm_f64_grouped(const GemmP* probs, const int* start, 
   return hipGetLastError();
 }
 
-template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false>
+template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false,
+          bool CONJB = false>
 static hipError_t launch_gemm_f64(const GemmP& p, int batch, hipStream_t s) {
   if (p.M <= 0 || p.N <= 0 || batch <= 0) return hipSuccess;
   dim3 grid((p.N + BN - 1) / BN, (p.M + BM - 1) / BM, batch);
   if (grid.y > 65535 || grid.z > 65535) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM>), grid, dim3(256), 0, s, p);
+  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM, CONJB>), grid, dim3(256), 0, s, p);
   return hipGetLastError();
 }

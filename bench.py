@@ -337,6 +337,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(syn, mixed, args, dt)
 
     cisd = None if args.no_block else cisd_rates(args)
+    differentiation = None if args.no_block else differentiation_rates()
 
     out = {
         "metric": METRIC,
@@ -374,6 +375,7 @@ def main():
         "launch_bound_case": small,
         "free_projection": fp_rates,
         "cisd": cisd,
+        "differentiation": differentiation,
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
@@ -589,6 +591,66 @@ def free_projection_rates(syn, env, args) -> dict:
         out[f"{tag}_walker_steps_per_s"] = round(W / dt, 1)
         log(f"free projection {tag}: {1e3 * dt:.2f} ms/step")
         del st, pctx, mctx, pops, mops
+    return out
+
+
+def differentiation_rates(steps: int = 40, every: int = 20) -> dict:
+    """The differentiation form at its sizes (N=128, n=16, G=640, 512 walkers, dt = 0.005, seed 3, L / sqrt(N)):
+    ms per step of the product's FP64 step, of the tangent propagation (trot_amd.diff, one direction), and of
+    the reverse-mode adjoint (trot_amd.adjoint: forward with checkpoints every ``every`` steps, recomputation,
+    reverse; the full dE/dh1), FP64 and FP32 Cholesky contractions."""
+    import torch
+
+    from trot_amd import adjoint, diff, meas, prop, synthetic, trial
+    from trot_amd.core import HamChol, System
+
+    N, n, G, W, dt = 128, 16, 640, 512, 0.005
+    syn = synthetic.make_synthetic(N, n, G, W, 3, 256.0, chol_scale=1.0 / np.sqrt(N))
+    rng = np.random.default_rng(33)
+    a = rng.standard_normal((N, N))
+    D = 0.05 * (np.triu(a) + np.triu(a, 1).T)
+    params = prop.QmcParams(dt=dt, n_walkers=W, seed=3, n_prop_steps=steps)
+    w0 = torch.from_numpy(syn.walkers).cuda()
+    fields = [torch.from_numpy(synthetic.fields(3, k, W, G)).cuda() for k in range(steps)]
+    out = {"workload": "n128_o16_g640_w512", "steps": steps, "checkpoint_every": every}
+
+    sys_ = System(N, (n, n), "restricted")
+    ham = HamChol(syn.h0, syn.h1, syn.chol).to_device()
+    tr = trial.RhfTrial(syn.trial)
+    mops, tops = meas.make_rhf_meas_ops(sys_), trial.make_rhf_trial_ops(sys_)
+    pops = prop.make_prop_ops("restricted", "restricted")
+    pctx, mctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params), mops.build_meas_ctx(ham, tr)
+    st = pops.init_prop_state(sys=sys_, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops, params=params,
+                              initial_walkers=w0)
+    kw = dict(params=params, ham_data=ham, trial_data=tr, trial_ops=tops, meas_ops=mops, meas_ctx=mctx,
+              prop_ctx=pctx)
+    st = pops.step(st, fields=fields[0], **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st = pops.step(st, fields=fields[k], **kw)
+    torch.cuda.synchronize()
+    out["primal_fp64_ms_per_step"] = round(1e3 * (time.perf_counter() - t0) / steps, 3)
+    del st
+
+    tc = diff.build_ctx(syn.h0, syn.h1, syn.chol, syn.trial, D, 0.0, params)
+    diff.run(tc, w0, lambda k: fields[k], 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_t = diff.run(tc, w0, lambda k: fields[k], steps)
+    torch.cuda.synchronize()
+    out["tangent_ms_per_step"] = round(1e3 * (time.perf_counter() - t0) / steps, 3)
+    for prec in ("f64", "f32"):
+        ac = adjoint.build_ctx(syn.h0, syn.h1, syn.chol, syn.trial, params, precision=prec)
+        adjoint.run(ac, w0, lambda k: fields[k], 2, checkpoint_every=every)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = adjoint.run(ac, w0, lambda k: fields[k], steps, checkpoint_every=every)
+        torch.cuda.synchronize()
+        out[f"adjoint_{prec}_ms_per_step"] = round(1e3 * (time.perf_counter() - t0) / steps, 3)
+        out[f"adjoint_{prec}_dE_dlambda"] = float(np.sum(res.grad_h1.numpy() * D))
+    out["tangent_dE_dlambda"] = res_t.denergy
+    log(f"differentiation: {out}")
     return out
 
 

@@ -338,6 +338,7 @@ def main():
 
     cisd = None if args.no_block else cisd_rates(args)
     differentiation = None if args.no_block else differentiation_rates()
+    batched = None if args.no_block else batch_rates()
 
     out = {
         "metric": METRIC,
@@ -376,6 +377,7 @@ def main():
         "free_projection": fp_rates,
         "cisd": cisd,
         "differentiation": differentiation,
+        "batch": batched,
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
@@ -591,6 +593,84 @@ def free_projection_rates(syn, env, args) -> dict:
         out[f"{tag}_walker_steps_per_s"] = round(W / dt, 1)
         log(f"free projection {tag}: {1e3 * dt:.2f} ms/step")
         del st, pctx, mctx, pops, mops
+    return out
+
+
+def _fragment(i, N, n, G, W, dt, chol_scale):
+    import torch
+
+    from trot_amd import batch, meas, prop, synthetic, trial
+    from trot_amd.core import HamChol, System
+
+    syn = synthetic.make_synthetic(N, n, G, W, 100 + i, 256.0, nocc_b=n - 1, chol_scale=chol_scale)
+    sys_ = System(N, (n, n - 1), "unrestricted")
+    ham = HamChol(syn.h0, syn.h1, syn.chol).to_device()
+    tr = trial.UhfTrial(syn.trial, syn.trial_b)
+    mops, tops = meas.make_uhf_meas_ops(sys_), trial.make_uhf_trial_ops(sys_)
+    pops = prop.make_prop_ops("restricted", "unrestricted")
+    params = prop.QmcParams(dt=dt, n_walkers=W, seed=100 + i)
+    pctx, mctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params), mops.build_meas_ctx(ham, tr)
+    w0 = (torch.from_numpy(syn.walkers).cuda(), torch.from_numpy(syn.walkers_b).cuda())
+    st = pops.init_prop_state(sys=sys_, ham_data=ham, trial_ops=tops, trial_data=tr, meas_ops=mops, params=params,
+                              initial_walkers=w0)
+    return batch.Fragment(sys=sys_, ham=ham, trial=tr, trial_ops=tops, meas_ops=mops, prop_ops=pops, params=params,
+                          prop_ctx=pctx, meas_ctx=mctx, state=st)
+
+
+def batch_rates(steps: int = 20) -> dict:
+    """The batching form: 32 fragments (N ~ U[40, 200] seed 4, n = round(N/5), G = 4N, UHF trial (n, n-1), 128
+    walkers each, dt = 0.005, L / sqrt(N)), FP64: one fragment after another with eager launches, one HIP graph
+    of per-fragment launches on 4 branches, and one HIP graph of grouped launches (autotuned Taylor tiles).
+    frac = FP64 MFMA ideal of the step's GEMMs (VHS on the stored half of V, force bias, Taylor, one-body)
+    over the grouped step's time."""
+    import torch
+
+    from trot_amd import _lib, batch, synthetic
+
+    W, dt = 128, 0.005
+    sizes = synthetic.fragment_sizes(32, seed=4)
+    out = {"workload": "32 fragments, N ~ U[40, 200], n = N/5, G = 4N, UHF, 128 walkers each", "steps": steps}
+    fields = None
+    for mode in ("eager", "graph_4_branches", "grouped_graph"):
+        frags = [_fragment(i, N, n, G, W, dt, 1.0 / np.sqrt(N)) for i, (N, n, G) in enumerate(sizes)]
+        if fields is None:
+            fields = [[torch.from_numpy(synthetic.fields(100 + i, s, W, f.n_fields)).cuda() for i, f in
+                       enumerate(frags)] for s in range(steps + 3)]
+        if mode == "eager":
+            for s in range(2):
+                for i, f in enumerate(frags):
+                    f.state = f.step(fields[s][i])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for s in range(2, steps + 2):
+                for i, f in enumerate(frags):
+                    f.state = f.step(fields[s][i])
+            torch.cuda.synchronize()
+        else:
+            fb = batch.FragmentBatch(frags, grouped=mode == "grouped_graph")
+            if mode == "grouped_graph":
+                out["taylor_tiles"] = {f"{k[1]}x{k[2]}": v for k, v in sorted(fb.autotune().items())}
+            flat = [fb.pack_fields([x.cpu().numpy() for x in fs]) for fs in fields]
+            fb.capture(flat[0])
+            fb.step(flat[1])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for s in range(2, steps + 2):
+                fb.step(flat[s])
+            torch.cuda.synchronize()
+            del fb
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        out[f"{mode}_ms_per_step"] = round(ms, 3)
+        log(f"batch {mode}: {ms:.2f} ms per step of all 32 fragments")
+        del frags
+        torch.cuda.empty_cache()
+    symcols = lambda N: 64 * ((N + 7) // 8) * ((N + 7) // 8 + 1) // 2
+    flop = sum(4 * W * G * symcols(N) + 4 * W * G * N * (2 * n - 1) + 8 * W * N * N * (2 * n - 1) * 6
+               + 8 * W * N * N * (2 * n - 1) for N, n, G in sizes)
+    out["gemm_flop_per_step"] = float(flop)
+    out["fp64_t_star_ms"] = round(1e3 * flop / (PEAK["f64"] * 1e12), 3)
+    out["grouped_fp64_frac"] = round(out["fp64_t_star_ms"] / out["grouped_graph_ms_per_step"], 4)
+    out["speedup_grouped_vs_eager"] = round(out["eager_ms_per_step"] / out["grouped_graph_ms_per_step"], 3)
     return out
 
 

@@ -381,6 +381,9 @@ int trot_local_energy_lp(const float* rot_chol_a, const float* rot_chol_b, const
 #define TROT_GOP_VHS 4
 #define TROT_GOP_TAYLOR_NARROW 5
 #define TROT_GOP_FORCE_BIAS 6 /* problems from trot_force_bias_problem, not trot_gemm_problem */
+#define TROT_GOP_TAYLOR_64x64 7    /* TROT_GOP_TAYLOR on 64 x 64 tiles (same bits; the tile shapes of */
+#define TROT_GOP_TAYLOR_64x32 8    /* trot_autotune_taylor: 64x64, 64x32, TAYLOR = 32x64, NARROW = 32x32) */
+#define TROT_GOP_ONE_BODY_NARROW 9 /* TROT_GOP_ONE_BODY on 64 x 32 tiles (n <= 32; same bits) */
 size_t trot_gemm_problem_bytes(void);
 long trot_gemm_problem(int op, const double* a, const double* b, double* c, double* d, int W, int N, int n,
                        double coef, void* desc);

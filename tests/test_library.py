@@ -121,14 +121,18 @@ def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
     assert h.trot_orthogonalize(None, 4, 8, 2, None, d, big, None) == -1
     # grouped launches: the descriptors are filled on the host, the launches refuse empty groups
     desc = ctypes.create_string_buffer(h.trot_gemm_problem_bytes())
-    assert h.trot_gemm_problem(9, d, d, d, d, 4, 32, 8, 0.1, desc) == -1  # unknown operation
+    assert h.trot_gemm_problem(99, d, d, d, d, 4, 32, 8, 0.1, desc) == -1  # unknown operation
     assert h.trot_gemm_problem(1, d, d, d, None, 4, 32, 8, 0.1, desc) == -1  # Taylor needs the running sum
     assert h.trot_gemm_problem(0, d, d, d, None, 0, 32, 8, 0.0, desc) == -1  # no walkers
     wide = h.trot_gemm_problem(1, d, d, d, d, 4, 128, 16, 0.1, desc)
     narrow = h.trot_gemm_problem(5, d, d, d, d, 4, 128, 16, 0.1, desc)
     assert 0 < wide <= narrow  # the same problem in 32 x 64 and 32 x 32 tiles
+    # the four tile shapes of the tuner as grouped Taylor problems: 128 / 64 or 32 row tiles x 16 / 32 or 64 columns
+    assert [h.trot_gemm_problem(op, d, d, d, d, 4, 128, 16, 0.1, desc) for op in (7, 8, 1, 5)] == [8, 8, 16, 16]
+    assert h.trot_gemm_problem(9, d, d, d, None, 4, 128, 16, 0.0, desc) == 8  # one-body on 64 x 32 tiles
+    assert h.trot_gemm_problem(7, d, d, d, None, 4, 128, 16, 0.1, desc) == -1
     assert h.trot_gemm_f64_grouped(0, d, d, 0, 1, None) == -1
-    assert h.trot_gemm_f64_grouped(9, d, d, 1, 1, None) == -1
+    assert h.trot_gemm_f64_grouped(99, d, d, 1, 1, None) == -1
     idesc = ctypes.create_string_buffer(h.trot_inv_problem_bytes())
     assert h.trot_inv_problem(d, d, d, 4, 65, idesc) == -2  # nocc above the kernels' limit
     assert h.trot_inv_problem(d, d, d, 5, 8, idesc) == 5  # one inverse per walker

@@ -1060,7 +1060,10 @@ def taylor_apply_lp_symblk(vpk32: torch.Tensor, w: torch.Tensor, sqrt_dt: float,
 # ---------------------------------------------------------------------------
 # grouped launches over many walker sets
 # ---------------------------------------------------------------------------
-GOP = {"one_body": 0, "taylor": 1, "overlap": 2, "green": 3, "vhs": 4, "taylor_narrow": 5, "force_bias": 6}
+GOP = {"one_body": 0, "taylor": 1, "overlap": 2, "green": 3, "vhs": 4, "taylor_narrow": 5, "force_bias": 6,
+       "taylor_64x64": 7, "taylor_64x32": 8, "one_body_narrow": 9}
+# trot_autotune_taylor's shapes (0 = 64x64, 1 = 64x32, 2 = 32x64, 3 = 32x32) as grouped Taylor operations
+TAYLOR_GOP_OF_SHAPE = {0: "taylor_64x64", 1: "taylor_64x32", 2: "taylor", 3: "taylor_narrow"}
 
 
 class Grouped:

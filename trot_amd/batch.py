@@ -107,7 +107,7 @@ class _GroupedPlan:
     problem of the fragment's own entry point, so the step is bit-identical to stepping the fragments
     one at a time."""
 
-    def __init__(self, frags: List[Fragment], views):
+    def __init__(self, frags: List[Fragment], views, tiles: dict | None = None):
         from .trial import _coeffs
 
         n_terms = {int(f.params.n_exp_terms) for f in frags}
@@ -117,11 +117,14 @@ class _GroupedPlan:
         self.per = []
         dev = "cuda"
         vhs = _lib.Grouped("gemm", "vhs")
-        ob1 = _lib.Grouped("gemm", "one_body")
+        # one-body products on 64 x 32 tiles for n <= 32 (same bits as 64 x 64)
+        ob1 = {op: _lib.Grouped("gemm", op) for op in ("one_body", "one_body_narrow")}
         cp = _lib.Grouped("copy")
-        tay = [_lib.Grouped("gemm", "taylor") for _ in range(self.n_terms)]
-        tayn = [_lib.Grouped("gemm", "taylor_narrow") for _ in range(self.n_terms)]  # n <= 32: 32 x 32 tiles
-        ob2 = _lib.Grouped("gemm", "one_body")
+        # Taylor terms, grouped by tile shape: the autotuned shape of the fragment's (walkers, N, n) when
+        # FragmentBatch.autotune ran, else 32 x 32 for n <= 32 and 32 x 64 above (every shape: the same bits)
+        tay = [{op: _lib.Grouped("gemm", op) for op in _lib.TAYLOR_GOP_OF_SHAPE.values()} for _ in range(self.n_terms)]
+        ob2 = {op: _lib.Grouped("gemm", op) for op in ("one_body", "one_body_narrow")}
+        tiles = tiles or {}
         ovl = _lib.Grouped("gemm", "overlap")
         inv = _lib.Grouped("inv")
         grn = _lib.Grouped("gemm", "green")
@@ -155,13 +158,16 @@ class _GroupedPlan:
                 m = torch.empty((W, n, n), dtype=torch.complex128, device=dev)
                 minv = torch.empty_like(m)
                 det = torch.empty((W,), dtype=torch.complex128, device=dev)
-                ob1.add_gemm(ctx.exp_h1_half, w, t, None, W, N, n)
+                obop = "one_body_narrow" if n <= 32 else "one_body"
+                shape = tiles.get((W, N, n, True), 3 if n <= 32 else 2)
+                tayop = _lib.TAYLOR_GOP_OF_SHAPE[shape]
+                ob1[obop].add_gemm(ctx.exp_h1_half, w, t, None, W, N, n)
                 cp.add_copy(t, t0)
                 prev, nxt = t0, t1
                 for k in range(1, self.n_terms + 1):
-                    (tayn if n <= 32 else tay)[k - 1].add_gemm(b["v"], prev, nxt, t, W, N, n, ctx.sqrt_dt / k)
+                    tay[k - 1][tayop].add_gemm(b["v"], prev, nxt, t, W, N, n, ctx.sqrt_dt / k)
                     prev, nxt = nxt, prev
-                ob2.add_gemm(ctx.exp_h1_half, t, w, None, W, N, n)
+                ob2[obop].add_gemm(ctx.exp_h1_half, t, w, None, W, N, n)
                 ovl.add_gemm(c, w, m, None, W, N, n)
                 inv.add_inv(m, minv, det, W, n)
                 grn.add_gemm(w, minv, greens[si], None, W, N, n)
@@ -192,8 +198,10 @@ class _GroupedPlan:
                            getattr(p, "weight_floor", 1.0e-3), getattr(p, "weight_cap", 100.0),
                            getattr(p, "pop_control_damping", 0.1))
             self.per.append(b)
-        terms = [g for pair in zip(tay, tayn) for g in pair if g.descs]
-        self.ops = [g.finish() for g in (vhs, ob1, cp, *terms, ob2, ovl, inv, grn)]
+        terms = [g for groups in tay for g in groups.values() if g.descs]
+        first = [g for g in ob1.values() if g.descs]
+        last = [g for g in ob2.values() if g.descs]
+        self.ops = [g.finish() for g in (vhs, *first, cp, *terms, *last, ovl, inv, grn)]
         # spin a's sums write fb before spin b's one-slice GEMMs and sums add to it
         self.pre = [g.finish() for g in (fb_a, sum_a, fb_b, sum_b, shf) if g.descs]
         self.post = [upd.finish()]
@@ -222,7 +230,8 @@ class FragmentBatch:
     ``grouped`` (default: when every fragment runs the FP64 step on sym-block V): every operation of the
     step runs as one launch over all fragments (force bias and its slice sums, shifted fields, VHS,
     one-body, Taylor terms, overlaps, inverses, Green's functions, weight update; the trot_*_grouped
-    entry points); the results stay bit-identical to the one-by-one run.
+    entry points; one launch per tile shape where fragments differ); the results stay bit-identical to the
+    one-by-one run.  ``autotune`` before ``capture`` picks every fragment's Taylor tile shape by timing.
     """
 
     def __init__(self, fragments: List[Fragment], n_streams: int = 4, grouped: bool | None = None):
@@ -251,6 +260,7 @@ class FragmentBatch:
             raise ValueError("grouped step: every fragment must run the FP64 step on sym-block V")
         self.grouped = can if grouped is None else bool(grouped)
         self.plan = None
+        self.tiles = {}
 
     def autotune(self):
         """Pick the FP64 Taylor-term tile shape of every fragment's (walkers, N, n) per spin by timing
@@ -264,6 +274,7 @@ class FragmentBatch:
                 key = (f.n_walkers, f.prop_ctx.norb, n, sym)
                 if key not in picked:
                     picked[key] = _lib.autotune_taylor(*key)[0]
+        self.tiles.update(picked)  # the grouped step's Taylor launches take these shapes
         return picked
 
     def field_views(self):
@@ -296,7 +307,7 @@ class FragmentBatch:
                 torch.float64).contiguous()
             f.state = f.state._replace(pop_control_ene_shift=scal[0], e_estimate=scal[1])
             f._scal = scal
-        self.plan = _GroupedPlan(self.fragments, views)
+        self.plan = _GroupedPlan(self.fragments, views, self.tiles)
         for f, b in zip(self.fragments, self.plan.per):
             b["scal"] = f._scal
 

@@ -338,8 +338,8 @@ int trot_set_taylor_tile(int W, int N, int n, int symblk, int shape) {
 
 int trot_get_taylor_tile(int W, int N, int n, int symblk) { return taylor_tile(symblk ? 1 : 0, W, N, n); }
 
-// Times one Taylor term (K = 1: one product and the accumulation) in each tile shape on zeroed scratch
-// operands carved from ws, three launches per shape after one warm-up, and records the fastest for
+// Times one Taylor term (K = 1: one product and the accumulation) in each tile shape on scratch
+// operands carved from ws (filled with values in [-1, 1)), three launches per shape after one warm-up, and records the fastest for
 // (W, N, n, storage).  Synchronizes the stream: not for use while the stream is being captured.
 int trot_autotune_taylor(int W, int N, int n, int symblk, void* ws, size_t ws_bytes, void* stream, int* shape_out,
                          float* ms_out) {
@@ -357,7 +357,8 @@ int trot_autotune_taylor(int W, int N, int n, int symblk, void* ws, size_t ws_by
   double* w = c.take<double>((size_t)W * N * n * 2);
   double* t0 = c.take<double>((size_t)W * N * n * 2);
   double* t1 = c.take<double>((size_t)W * N * n * 2);
-  HIPCHK(hipMemsetAsync(ws, 0, trot_autotune_taylor_workspace(W, N, n, symblk), s));
+  // operands of size O(1) (a hashed pattern): MFMA timings on all-zero operands run fast and mislead
+  HIPCHK(launch_fill_pattern((double*)ws, (long)(trot_autotune_taylor_workspace(W, N, n, symblk) / 8), s));
   GemmP p{};
   p.A = v; p.lda = symblk ? (N + 7) / 8 : N; p.sA = vcols;
   p.B = t0; p.ldb = n; p.sB = (long)N * n;
@@ -778,7 +779,9 @@ size_t trot_copy_problem_bytes(void) { return sizeof(CopyProb); }
 // number of workgroups it takes in trot_gemm_f64_grouped, or a negative status.  Arguments, by op:
 //   TROT_GOP_ONE_BODY  a = exp(-dt/2 h1_eff) (N x N), b = w_in, c = w_out
 //   TROT_GOP_TAYLOR    a = V (sym-block), b = previous term, c = next term, d = running sum, coef = sqrt(dt)/k
-//                      (32 x 64 tiles; TROT_GOP_TAYLOR_NARROW the same with 32 x 32 tiles, for n <= 32)
+//                      (32 x 64 tiles; TROT_GOP_TAYLOR_NARROW 32 x 32, TROT_GOP_TAYLOR_64x64 and _64x32 the
+//                      other shapes of trot_autotune_taylor; every shape gives the same bits)
+//   TROT_GOP_ONE_BODY_NARROW  TROT_GOP_ONE_BODY on 64 x 32 tiles (n <= 32), the same bits
 //   TROT_GOP_OVERLAP   a = trial (N x n), b = walkers, c = C^T w (W, n, n)
 //   TROT_GOP_GREEN     a = walkers, b = (C^T w)^-1, c = green_t (W, N, n)
 //   TROT_GOP_VHS       a = xbar (W, G), b = chol (G, N2), c = vhs (W, N2); n = G, N = N2 (N2 < 2^31)
@@ -799,6 +802,13 @@ long trot_gemm_problem(int op, const double* a, const double* b, double* c, doub
       p = problem_taylor_symblk(a, b, c, d, N, n, coef);
       wg = gemm_f64_workgroups<32, 32>(p, W);
       break;
+    case TROT_GOP_TAYLOR_64x64:
+    case TROT_GOP_TAYLOR_64x32:
+      if (!d) return fail(TROT_ERR_INVALID, "gemm_problem: Taylor needs the running sum");
+      p = problem_taylor_symblk(a, b, c, d, N, n, coef);
+      wg = op == TROT_GOP_TAYLOR_64x64 ? gemm_f64_workgroups<64, 64>(p, W) : gemm_f64_workgroups<64, 32>(p, W);
+      break;
+    case TROT_GOP_ONE_BODY_NARROW: p = problem_one_body(a, b, c, N, n); wg = gemm_f64_workgroups<64, 32>(p, W); break;
     case TROT_GOP_OVERLAP: p = problem_overlap(a, b, c, N, n); wg = gemm_f64_workgroups<64, 64>(p, W); break;
     case TROT_GOP_GREEN: p = problem_green(a, b, c, N, n); wg = gemm_f64_workgroups<64, 64>(p, W); break;
     case TROT_GOP_VHS: p = problem_vhs(a, b, c, n, N, W); wg = gemm_f64_workgroups<64, 64>(p, 1); break;
@@ -825,6 +835,17 @@ int trot_gemm_f64_grouped(int op, const void* probs, const int* start, int nprob
     case TROT_GOP_TAYLOR_NARROW:
       HIPCHK((launch_gemm_f64_grouped<32, 32, true, true, true, true, false, EPI_TAYLOR, true>(P, start, nprob, total_wg,
                                                                                            s)));
+      break;
+    case TROT_GOP_TAYLOR_64x64:
+      HIPCHK((launch_gemm_f64_grouped<64, 64, true, true, true, true, false, EPI_TAYLOR, true>(P, start, nprob, total_wg,
+                                                                                           s)));
+      break;
+    case TROT_GOP_TAYLOR_64x32:
+      HIPCHK((launch_gemm_f64_grouped<64, 32, true, true, true, true, false, EPI_TAYLOR, true>(P, start, nprob, total_wg,
+                                                                                           s)));
+      break;
+    case TROT_GOP_ONE_BODY_NARROW:
+      HIPCHK((launch_gemm_f64_grouped<64, 32, false, true, true, true, false, EPI_STORE>(P, start, nprob, total_wg, s)));
       break;
     case TROT_GOP_OVERLAP:
       HIPCHK((launch_gemm_f64_grouped<64, 64, false, true, false, true, false, EPI_STORE>(P, start, nprob, total_wg, s)));

@@ -69,6 +69,7 @@ hipError_t launch_qr_fallback(const double* src, double* q, const int* flags, in
 // zeta_dev: comb offset read on the device when non-null (else zeta)
 hipError_t launch_sr_index(const double* weights, double* cw, int* idx, double* w_new, int W, double zeta,
                            const double* zeta_dev, hipStream_t s);
+hipError_t launch_fill_pattern(double* x, long n, hipStream_t s);
 hipError_t launch_gather_rows(const double* src, double* dst, const int* idx, int W, long row, hipStream_t s);
 hipError_t launch_energy(const double* rc_a, const double* rc_b, const double* gt_a, const double* gt_b,
                          const double* h1t_a, const double* h1t_b, int na, int nb, int N, int G, int W, double h0,

@@ -594,6 +594,25 @@ hipError_t launch_qr_fallback(const double* src, double* q, const int* flags, in
   return hipGetLastError();
 }
 
+// x[i] = a value in [-1, 1) hashed from i (deterministic, non-zero operands for timing).
+__global__ __launch_bounds__(256) void fill_pattern_kernel(double* __restrict__ x, long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+    unsigned long long h = (unsigned long long)i * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 32;
+    x[i] = (double)(h >> 11) * (1.0 / 4503599627370496.0) - 1.0;  // 2^-52 (h >> 11) - 1
+  }
+}
+
+hipError_t launch_fill_pattern(double* x, long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_pattern_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_sr_index(const double* weights, double* cw, int* idx, double* w_new, int W, double zeta,
                            const double* zeta_dev, hipStream_t s) {
   if (W <= 0) return hipSuccess;

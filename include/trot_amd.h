@@ -32,7 +32,7 @@ extern "C" {
 
 #define TROT_OK 0
 #define TROT_ERR_INVALID (-1)     /* bad pointer or size */
-#define TROT_ERR_UNSUPPORTED (-2) /* valid input outside what the kernels cover (e.g. nocc > 64) */
+#define TROT_ERR_UNSUPPORTED (-2) /* valid input outside what the kernels cover (e.g. nocc > 256) */
 #define TROT_ERR_HIP (-3)         /* HIP runtime error (launch or configuration) */
 #define TROT_ERR_WORKSPACE (-4)   /* workspace smaller than the *_workspace() query */
 
@@ -43,6 +43,10 @@ extern "C" {
 
 const char* trot_version(void);
 const char* trot_last_error(void);
+
+/* nocc: up to 256 per spin for the overlap / Green's function, force bias, local energy (FP64, FP32 and the
+ * FP32-tile split-BF16 form), QR and the step; up to 64 for trot_local_energy_b3, trot_rot_chol_b3, the block
+ * observables, the grouped inverse and the CISD entry points. */
 
 /* Overlap matrix m = C^T w, det(m) and GT = w m^-1 for every walker.
  * Replaces: overlap_r / _det (trot/trial/rhf.py:35-47) for the determinant and

@@ -285,8 +285,8 @@ def test_prop_ctx(gpu):
 def test_bad_sizes_raise(gpu):
     from trot_amd import _lib
 
-    w = torch.zeros((2, 80, 65), dtype=torch.complex128, device="cuda")
-    c = torch.zeros((80, 65), dtype=torch.float64, device="cuda")
+    w = torch.zeros((2, 300, 257), dtype=torch.complex128, device="cuda")  # nocc above 256
+    c = torch.zeros((300, 257), dtype=torch.float64, device="cuda")
     with pytest.raises(ValueError):
         _lib.overlap_green(c, w)
     with pytest.raises(ValueError):

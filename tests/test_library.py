@@ -47,20 +47,23 @@ def test_error_codes_without_gpu():
     assert b"null" in h.trot_last_error()
     # nocc above the kernels' limit
     dummy = ctypes.c_void_p(16)
-    assert h.trot_overlap_green(dummy, dummy, 4, 100, 65, dummy, None, dummy, 1 << 30, None) == -2
+    assert h.trot_overlap_green(dummy, dummy, 4, 300, 257, dummy, None, dummy, 1 << 30, None) == -2  # n > 256
     # workspace smaller than the query
     need = h.trot_overlap_green_workspace(4, 8, 2)
     assert need > 0
     assert h.trot_overlap_green(dummy, dummy, 4, 8, 2, dummy, None, dummy, need - 1, None) == -4
     assert b"workspace" in h.trot_last_error()
     assert h.trot_local_energy_workspace(8, 100) >= 8 * 4 * 16
+    assert h.trot_local_energy(dummy, None, dummy, None, dummy, None, 4, 300, 257, 0, 3, 0.0, dummy, dummy, 1 << 30,
+                               None) == -2
+    assert h.trot_inverse_det(dummy, 4, 257, ctypes.c_void_p(32), dummy, None) == -2
     assert h.trot_force_bias(None, None, 1, 1, 1, 2.0, 0, None, None) == -1
     assert h.trot_sr_comb(dummy, 0.5, 0, dummy, dummy, dummy, 4096, None) == -1
     assert h.trot_sr_comb_dev_noalloc(dummy, None, 8, dummy, dummy, None) == -1
     assert h.trot_sr_comb_dev_noalloc(dummy, dummy, 262145, dummy, dummy, None) == -2  # above the static scratch
     assert h.trot_one_body_apply(dummy, dummy, 1, 4, 2, dummy, None) == -1  # in place is rejected
     assert h.trot_orthonormalize(dummy, 2, 8, 70, dummy, 1 << 30, None) == -1  # n > N
-    assert h.trot_orthonormalize(dummy, 2, 100, 70, dummy, 1 << 30, None) == -2  # n > 64
+    assert h.trot_orthonormalize(dummy, 2, 300, 257, dummy, 1 << 30, None) == -2  # n > 256
 
 
 def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
@@ -116,7 +119,7 @@ def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
     assert h.trot_fp_weights(d, None, None, 4, d, d, None) == -1
     assert h.trot_fp_unnorm(d, d, None, 4, None) == -1
     assert h.trot_block_energy_fp(d, d, d, 4, 0.0, 0.0, d, None) == -1  # dt must be positive
-    assert h.trot_qr_norm(d, 4, 100, 65, d, d, big, None) == -2
+    assert h.trot_qr_norm(d, 4, 300, 257, d, d, big, None) == -2
     assert h.trot_qr_norm(d, 4, 8, 2, d, d, 0, None) == -4
     assert h.trot_orthogonalize(None, 4, 8, 2, None, d, big, None) == -1
     # grouped launches: the descriptors are filled on the host, the launches refuse empty groups

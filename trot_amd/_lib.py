@@ -16,7 +16,8 @@ import torch
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libtrot_amd.so"
 
 TROT_OK = 0
-MAX_NOCC = 64
+MAX_NOCC = 64  # the register-resident inverse, Cholesky-QR, split-BF16 exchange blocks, observables, CISD
+MAX_NOCC_BIG = 256  # the general inverse, tiled FP64 / FP32 exchange and Gram-Schmidt QR above 64
 
 
 class TrotError(RuntimeError):
@@ -214,8 +215,8 @@ def overlap_green(trial: torch.Tensor, walkers: torch.Tensor, want_green: bool =
     W, N, n = walkers.shape
     _req(walkers, "walkers", C128, (W, N, n))
     _req(trial, "trial", F64, (N, n))
-    if n > MAX_NOCC:
-        raise ValueError(f"nocc={n} > {MAX_NOCC} is not supported by the GPU kernels")
+    if n > MAX_NOCC_BIG:
+        raise ValueError(f"nocc={n} > {MAX_NOCC_BIG} is not supported by the GPU kernels")
     L = lib()
     det = torch.empty((W,), dtype=C128, device=walkers.device)
     gt = torch.empty((W, N, n), dtype=C128, device=walkers.device) if want_green else None
@@ -533,8 +534,8 @@ def local_energy(rot_chol_a, green_t_a, rot_h1t_a, h0: float, rot_chol_b=None, g
         _req(green_t_b, "green_t_b", C128, (W, N, nb))
         _req(rot_chol_b, "rot_chol_b", F64, (G, nb, N))
         _req(rot_h1t_b, "rot_h1t_b", F64, (N, nb))
-    if max(na, nb) > MAX_NOCC:
-        raise ValueError(f"nocc > {MAX_NOCC} is not supported by the GPU energy kernel")
+    if max(na, nb) > MAX_NOCC_BIG:
+        raise ValueError(f"nocc > {MAX_NOCC_BIG} is not supported by the GPU energy kernel")
     L = lib()
     out = torch.empty((W,), dtype=C128, device=green_t_a.device)
     ws = _ws(L.trot_local_energy_workspace(W, G), green_t_a.device)
@@ -598,8 +599,8 @@ def weighted_mean(x: torch.Tensor, weights: torch.Tensor, wsum: torch.Tensor) ->
 def orthonormalize(w: torch.Tensor) -> torch.Tensor:
     W, N, n = w.shape
     _req(w, "walkers", C128, (W, N, n))
-    if n > MAX_NOCC:
-        raise ValueError(f"nocc={n} > {MAX_NOCC} is not supported")
+    if n > MAX_NOCC_BIG:
+        raise ValueError(f"nocc={n} > {MAX_NOCC_BIG} is not supported")
     out = w.clone()
     L = lib()
     ws = _ws(L.trot_orthonormalize_workspace(W, N, n), w.device)
@@ -611,8 +612,8 @@ def orthogonalize(w: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     """(Q, det_r): Q with real positive diag(R) and det_r = prod diag(R) (real, (W,))."""
     W, N, n = w.shape
     _req(w, "walkers", C128, (W, N, n))
-    if n > MAX_NOCC:
-        raise ValueError(f"nocc={n} > {MAX_NOCC} is not supported")
+    if n > MAX_NOCC_BIG:
+        raise ValueError(f"nocc={n} > {MAX_NOCC_BIG} is not supported")
     out = w.clone()
     det_r = torch.empty((W,), dtype=F64, device=w.device)
     L = lib()
@@ -625,8 +626,8 @@ def qr_norm(w: torch.Tensor) -> torch.Tensor:
     """|det R| of w = QR per walker (W,) FP64, from the Cholesky factor of w^H w."""
     W, N, n = w.shape
     _req(w, "walkers", C128, (W, N, n))
-    if n > MAX_NOCC:
-        raise ValueError(f"nocc={n} > {MAX_NOCC} is not supported")
+    if n > MAX_NOCC_BIG:
+        raise ValueError(f"nocc={n} > {MAX_NOCC_BIG} is not supported")
     det_r = torch.empty((W,), dtype=F64, device=w.device)
     L = lib()
     ws = _ws(L.trot_qr_norm_workspace(W, N, n), w.device)
@@ -1000,8 +1001,8 @@ def local_energy_lp(rot_chol_a32, green_t_a, rot_h1t_a, h0: float, prec: str, ro
         _req(green_t_b, "green_t_b", C128, (W, N, nb))
         _req(rot_chol_b32, "rot_chol_b (f32)", F32, (G, nb, N))
         _req(rot_h1t_b, "rot_h1t_b", F64, (N, nb))
-    if max(na, nb) > MAX_NOCC:
-        raise ValueError(f"nocc > {MAX_NOCC} is not supported by the GPU energy kernel")
+    if max(na, nb) > MAX_NOCC_BIG:
+        raise ValueError(f"nocc > {MAX_NOCC_BIG} is not supported by the GPU energy kernel")
     L = lib()
     out = torch.empty((W,), dtype=C128, device=green_t_a.device)
     ws = _ws(L.trot_local_energy_workspace(W, G), green_t_a.device)

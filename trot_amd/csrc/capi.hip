@@ -113,7 +113,7 @@ int trot_overlap_green(const double* trial, const double* walkers, int W, int N,
                        double* green_t, void* ws, size_t ws_bytes, void* stream) {
   if (!trial || !walkers || !det) return fail(TROT_ERR_INVALID, "overlap_green: null pointer");
   if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "overlap_green: bad sizes");
-  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "overlap_green: nocc > 64 unsupported");
+  if (n > TROT_MAX_NOCC_BIG) return fail(TROT_ERR_UNSUPPORTED, "overlap_green: nocc > 256 unsupported");
   Carve c{(char*)ws, ws_bytes, 0};
   double* m = c.take<double>((size_t)W * n * n * 2);
   double* minv = c.take<double>((size_t)W * n * n * 2);
@@ -482,7 +482,8 @@ int trot_local_energy(const double* rot_chol_a, const double* rot_chol_b, const 
   if (green_t_b && (!rot_h1t_b || (G > 0 && !rot_chol_b)))
     return fail(TROT_ERR_INVALID, "local_energy: second spin incomplete");
   if (W < 1 || N < 1 || na < 1 || G < 0) return fail(TROT_ERR_INVALID, "local_energy: bad sizes");
-  if (na > 64 || (green_t_b && nb > 64)) return fail(TROT_ERR_UNSUPPORTED, "local_energy: nocc > 64");
+  if (na > TROT_MAX_NOCC_BIG || (green_t_b && nb > TROT_MAX_NOCC_BIG))
+    return fail(TROT_ERR_UNSUPPORTED, "local_energy: nocc > 256");
   if (ws_bytes < trot_local_energy_workspace(W, G)) return fail(TROT_ERR_WORKSPACE, "local_energy: workspace");
   HIPCHK(launch_energy(rot_chol_a, rot_chol_b, green_t_a, green_t_b, rot_h1t_a, rot_h1t_b, na, nb, N, G, W, h0,
                        kEnergyGB, (double*)ws, energy, (hipStream_t)stream));
@@ -519,7 +520,11 @@ int trot_orthonormalize(double* walkers, int W, int N, int n, void* ws, size_t w
 int trot_orthogonalize(double* walkers, int W, int N, int n, double* det_r, void* ws, size_t ws_bytes, void* stream) {
   if (!walkers) return fail(TROT_ERR_INVALID, "orthonormalize: null pointer");
   if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "orthonormalize: bad sizes");
-  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "orthonormalize: nocc > 64");
+  if (n > TROT_MAX_NOCC_BIG) return fail(TROT_ERR_UNSUPPORTED, "orthonormalize: nocc > 256");
+  if (n > TROT_MAX_NOCC) {  // Gram-Schmidt with reorthogonalisation for every walker, in place
+    HIPCHK(launch_qr_fallback(walkers, walkers, nullptr, N, n, W, nullptr, det_r, (hipStream_t)stream));
+    return TROT_OK;
+  }
   Carve c{(char*)ws, ws_bytes, 0};
   double* gram = c.take<double>((size_t)W * n * n * 2);
   double* rinv = c.take<double>((size_t)W * n * n * 2);
@@ -558,13 +563,17 @@ size_t trot_qr_norm_workspace(int W, int N, int n) {
 int trot_qr_norm(const double* walkers, int W, int N, int n, double* det_r, void* ws, size_t ws_bytes, void* stream) {
   if (!walkers || !det_r) return fail(TROT_ERR_INVALID, "qr_norm: null pointer");
   if (W < 1 || N < 1 || n < 1 || n > N) return fail(TROT_ERR_INVALID, "qr_norm: bad sizes");
-  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "qr_norm: nocc > 64");
+  if (n > TROT_MAX_NOCC_BIG) return fail(TROT_ERR_UNSUPPORTED, "qr_norm: nocc > 256");
   Carve c{(char*)ws, ws_bytes, 0};
   double* gram = c.take<double>((size_t)W * n * n * 2);
   double* q = c.take<double>((size_t)W * N * n * 2);
   int* flags = c.take<int>((size_t)W);
   if (!gram || !q || !flags) return fail(TROT_ERR_WORKSPACE, "qr_norm: workspace too small");
   hipStream_t s = (hipStream_t)stream;
+  if (n > TROT_MAX_NOCC) {  // |det R| from Gram-Schmidt for every walker
+    HIPCHK(launch_qr_fallback(walkers, q, nullptr, N, n, W, nullptr, det_r, s));
+    return TROT_OK;
+  }
   GemmP p{};
   p.A = walkers; p.lda = n; p.sA = (long)N * n;
   p.B = walkers; p.ldb = n; p.sB = (long)N * n;

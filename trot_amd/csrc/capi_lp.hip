@@ -157,7 +157,8 @@ int trot_local_energy_lp(const float* rot_chol_a, const float* rot_chol_b, const
   if (green_t_b && (!rot_h1t_b || (G > 0 && !rot_chol_b)))
     return fail(TROT_ERR_INVALID, "local_energy_lp: second spin incomplete");
   if (W < 1 || N < 1 || na < 1 || G < 0) return fail(TROT_ERR_INVALID, "local_energy_lp: bad sizes");
-  if (na > 64 || (green_t_b && nb > 64)) return fail(TROT_ERR_UNSUPPORTED, "local_energy_lp: nocc > 64");
+  if (na > TROT_MAX_NOCC_BIG || (green_t_b && nb > TROT_MAX_NOCC_BIG))
+    return fail(TROT_ERR_UNSUPPORTED, "local_energy_lp: nocc > 256");
   if (!mode_of(prec)) return fail(TROT_ERR_INVALID, "local_energy_lp: unknown precision");
   if (ws_bytes < trot_local_energy_workspace(W, G)) return fail(TROT_ERR_WORKSPACE, "local_energy_lp: workspace");
   HIPCHK(launch_energy_lp(rot_chol_a, rot_chol_b, green_t_a, green_t_b, rot_h1t_a, rot_h1t_b, na, nb, N, G, W, h0,

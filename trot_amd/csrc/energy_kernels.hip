@@ -7,7 +7,8 @@
 // c = 2 c^a, K = 2 K^a, which is 2 sum c^2 - sum K of the reference.
 //
 // exchange_kernel: grid (ceil(G / GB), W).  Each workgroup takes one walker
-// and GB Cholesky vectors; every f_g tile (<= 64 x 64) is produced on MFMA
+// and GB Cholesky vectors; every f_g tile (<= 64 x 64; exchange_big_kernel: 64 x 64 tile pairs for
+// nocc up to 256) is produced on MFMA
 // (FP64, FP32 or split-BF16), parked in LDS for the transpose, reduced in
 // FP64, and dropped: f is never written to HBM (the reference's "high" mode
 // materialises all of it, "low" scans g; both give the same sums).
@@ -107,6 +108,103 @@ __global__ __launch_bounds__(256) void exchange_kernel(const typename ExOps<Tile
   if (threadIdx.x == 0) cstore(partial, w * gridDim.x + gblk, {0.5 * sr, 0.5 * si});
 }
 
+// nocc > 64: f_g in 64 x 64 tiles.  c = sum_P tr f_PP; K = sum_P sum f_PP o f_PP^T + 2 sum_{P<Q} sum f_PQ o f_QP^T
+// (the (P, Q) and (Q, P) terms of sum_pq f_pq f_qp are equal); tile (P, Q) in F1, tile (Q, P) in F2 (aliasing
+// the GEMM staging).  Every sum runs in a fixed order.
+template <typename Tile>
+__device__ __forceinline__ void exchange_one_big(const typename ExOps<Tile>::TA* __restrict__ rc,
+                                                 const double* __restrict__ gt, int n, int N, typename Tile::Smem& sm,
+                                                 cplx* F2, cplx* F1, double* red, cplx& c, cplx& k) {
+  const int nt = (n + 63) / 64;
+  const int t = threadIdx.x;
+  double cr = 0, ci = 0, kr = 0, ki = 0;
+  for (int P = 0; P < nt; ++P)
+    for (int Q = P; Q < nt; ++Q) {
+      typename Tile::Acc acc;
+      for (int pass = 0; pass < (P == Q ? 1 : 2); ++pass) {
+        const int r0 = 64 * (pass ? Q : P), c0 = 64 * (pass ? P : Q);
+        Tile::run(rc, N, gt, n, n, n, N, r0, c0, sm, acc);
+        __syncthreads();  // sm is aliased by F2
+        cplx* F = pass ? F2 : F1;
+#pragma unroll
+        for (int i = 0; i < Tile::TM; ++i)
+#pragma unroll
+          for (int j = 0; j < Tile::TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int pp = Tile::row_of(i, r), qq = Tile::col_of(j);
+              F[pp * 64 + qq] = {(double)acc[0][i][j][r], (double)acc[1][i][j][r]};
+            }
+        __syncthreads();
+      }
+      const int rp = min(64, n - 64 * P), rq = min(64, n - 64 * Q);
+      if (P == Q) {
+        if (t < rp) {
+          cr += F1[t * 64 + t].re;
+          ci += F1[t * 64 + t].im;
+        }
+        for (int e = t; e < rp * rp; e += 256) {
+          const int pp = e / rp, qq = e - (e / rp) * rp;
+          const cplx v = cmul(F1[pp * 64 + qq], F1[qq * 64 + pp]);
+          kr += v.re;
+          ki += v.im;
+        }
+      } else {
+        for (int e = t; e < rp * rq; e += 256) {
+          const int pp = e / rq, qq = e - (e / rq) * rq;
+          const cplx v = cmul(F1[pp * 64 + qq], F2[qq * 64 + pp]);  // f[64P + pp, 64Q + qq] f[64Q + qq, 64P + pp]
+          kr += 2.0 * v.re;
+          ki += 2.0 * v.im;
+        }
+      }
+      __syncthreads();  // F1 / F2 / sm are rewritten by the next tile pair
+    }
+  cr = block_sum<256>(cr, red);
+  ci = block_sum<256>(ci, red);
+  kr = block_sum<256>(kr, red);
+  ki = block_sum<256>(ki, red);
+  c = {cr, ci};
+  k = {kr, ki};
+}
+
+template <typename Tile>
+__global__ __launch_bounds__(256) void exchange_big_kernel(const typename ExOps<Tile>::TA* __restrict__ rc_a,
+                                                           const typename ExOps<Tile>::TA* __restrict__ rc_b,
+                                                           const double* __restrict__ gt_a,
+                                                           const double* __restrict__ gt_b, int na, int nb, int N,
+                                                           int G, int GB, double* __restrict__ partial) {
+  __shared__ union {
+    typename Tile::Smem sm;
+    cplx F[64 * 64];
+  } u;
+  __shared__ cplx F1[64 * 64];
+  __shared__ double red[256];
+  const long w = blockIdx.y;
+  const int gblk = blockIdx.x;
+  const int g0 = gblk * GB;
+  const int g1 = min(G, g0 + GB);
+  const double* ga = gt_a + 2 * w * (long)N * na;
+  const double* gb = gt_b ? gt_b + 2 * w * (long)N * nb : nullptr;
+  double sr = 0, si = 0;
+  for (int g = g0; g < g1; ++g) {
+    cplx ca, ka, cb, kb;
+    if (na > 64) exchange_one_big<Tile>(rc_a + (long)g * na * N, ga, na, N, u.sm, u.F, F1, red, ca, ka);
+    else exchange_one<Tile>(rc_a + (long)g * na * N, ga, na, N, u.sm, u.F, red, ca, ka);
+    if (gb) {
+      if (nb > 64) exchange_one_big<Tile>(rc_b + (long)g * nb * N, gb, nb, N, u.sm, u.F, F1, red, cb, kb);
+      else exchange_one<Tile>(rc_b + (long)g * nb * N, gb, nb, N, u.sm, u.F, red, cb, kb);
+    } else {
+      cb = ca;
+      kb = ka;
+    }
+    const cplx cs = cadd(ca, cb);
+    const cplx c2 = cmul(cs, cs);
+    sr += c2.re - ka.re - kb.re;
+    si += c2.im - ka.im - kb.im;
+  }
+  if (threadIdx.x == 0) cstore(partial, w * gridDim.x + gblk, {0.5 * sr, 0.5 * si});
+}
+
 // E[w] = h0 + sum_s fac_s sum_{k,p} GT_s[w,k,p] rot_h1T_s[k,p] + sum_blocks partial[w, :]
 __global__ __launch_bounds__(256) void energy_finalize_kernel(const double* __restrict__ gt_a,
                                                               const double* __restrict__ gt_b,
@@ -152,9 +250,12 @@ static hipError_t launch_energy_t(const typename ExOps<Tile>::TA* rc_a, const ty
                                   int na, int nb, int N, int G, int W, double h0, int gb, double* partial,
                                   double* energy, hipStream_t s) {
   if (W <= 0) return hipSuccess;
-  if (na > 64 || nb > 64 || na < 1 || (gt_b && nb < 1)) return hipErrorInvalidValue;
+  if (na > TROT_MAX_NOCC_BIG || nb > TROT_MAX_NOCC_BIG || na < 1 || (gt_b && nb < 1)) return hipErrorInvalidValue;
   const int nblk = (G + gb - 1) / gb;
-  if (G > 0) {
+  if (G > 0 && (na > 64 || (gt_b && nb > 64))) {
+    hipLaunchKernelGGL(exchange_big_kernel<Tile>, dim3(nblk, W), dim3(256), 0, s, rc_a, rc_b, gt_a, gt_b, na, nb, N,
+                       G, gb, partial);
+  } else if (G > 0) {
     hipLaunchKernelGGL(exchange_kernel<Tile>, dim3(nblk, W), dim3(256), 0, s, rc_a, rc_b, gt_a, gt_b, na, nb, N, G,
                        gb, partial);
   }

@@ -84,7 +84,7 @@ int trot_gemm_batched(int opa, int opb, int a_complex, int b_complex, int M, int
 int trot_inverse_det(const double* m, int W, int n, double* minv, double* det, void* stream) {
   if (!m || !minv || !det || m == minv) return fail(TROT_ERR_INVALID, "inverse_det: bad pointers");
   if (W < 0 || n < 1) return fail(TROT_ERR_INVALID, "inverse_det: bad sizes");
-  if (n > TROT_MAX_NOCC) return fail(TROT_ERR_UNSUPPORTED, "inverse_det: n > 64");
+  if (n > TROT_MAX_NOCC_BIG) return fail(TROT_ERR_UNSUPPORTED, "inverse_det: n > 256");
   HIPCHK(launch_inv_det(m, minv, det, n, W, (hipStream_t)stream));
   return TROT_OK;
 }

@@ -3,7 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
-#define TROT_MAX_NOCC 64
+#define TROT_MAX_NOCC 64       /* register-resident inverse, Cholesky-QR, exchange blocks */
+#define TROT_MAX_NOCC_BIG 256  /* the general inverse, tiled exchange and Gram-Schmidt QR above 64 */
 
 hipError_t launch_inv_det(const double* m, double* minv, double* det, int n, int W, hipStream_t s);
 

@@ -392,7 +392,8 @@ __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// QR of the walkers whose Cholesky-QR broke down (flags[w] != 0; other workgroups return at once).  Classical
+// QR of the walkers whose Cholesky-QR broke down (flags[w] != 0; other workgroups return at once; null flags:
+// every walker, the QR for n > 64, beyond Cholesky-QR's LDS-resident Gram matrix; q may equal src).  Classical
 // Gram-Schmidt with reorthogonalisation ("twice is enough": orthonormal to rounding for any numerically
 // full-rank w, unlike Cholesky-QR, which needs kappa(w) below ~1e8): column k is projected out of the
 // orthonormal columns before it twice, then normalised, so R_kk = |residual| is real and >= 0, the phase
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(256) void qr_fallback_kernel(const double* __restri
   __shared__ double red[256];
   cplx* coef = reinterpret_cast<cplx*>(sh);
   const long w = blockIdx.x;
-  if (!flags[w]) return;
+  if (flags && !flags[w]) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const long base = w * (long)N * n;
   for (long e = t; e < (long)N * n; e += 256) cstore(q, base + e, cload(src, base + e));
@@ -544,10 +545,21 @@ hipError_t launch_copy_grouped(const CopyProb* probs, int nprob, long max_bytes,
   return hipGetLastError();
 }
 
+#define INV_LDS_MAX 96  // inv_det_big_kernel keeps the matrix in LDS up to this n (147 KB)
+__global__ void inv_det_big_kernel(const double* __restrict__ m, double* __restrict__ minv, double* __restrict__ det,
+                                   int n);
+
 hipError_t launch_inv_det(const double* m, double* minv, double* det, int n, int W, hipStream_t s) {
   if (W <= 0) return hipSuccess;
-  if (n < 1 || n > TROT_MAX_NOCC) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(inv_det_kernel, dim3(W), dim3(256), 0, s, m, minv, det, n);
+  if (n < 1 || n > TROT_MAX_NOCC_BIG) return hipErrorInvalidValue;
+  if (n <= TROT_MAX_NOCC) {
+    hipLaunchKernelGGL(inv_det_kernel, dim3(W), dim3(256), 0, s, m, minv, det, n);
+    return hipGetLastError();
+  }
+  const size_t lds = (n <= INV_LDS_MAX ? (size_t)n * n * 16 : 0) + (size_t)n * 16 + 256 * 8 + 256 * 4 + (size_t)n * 4;
+  hipError_t e = set_lds((const void*)inv_det_big_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(inv_det_big_kernel, dim3(W), dim3(256), lds, s, m, minv, det, n);
   return hipGetLastError();
 }
 
@@ -592,6 +604,100 @@ hipError_t launch_qr_fallback(const double* src, double* q, const int* flags, in
   const size_t lds = (size_t)n * 16;
   hipLaunchKernelGGL(qr_fallback_kernel, dim3(W), dim3(256), lds, s, src, q, flags, N, n, rinv, dprod);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Jordan inverse and determinant for n > 64 (inv_det_kernel covers n <= 64 in registers): partial
+// pivoting on |re| + |im| with the first index on ties (LAPACK's izamax), rows swapped in place, the inverse
+// built in place and its columns swapped back at the end; det = prod(pivots) (-1)^swaps.  One workgroup of
+// 256 threads per matrix.  The matrix lives in LDS when it fits (n <= INV_LDS_MAX), else in place in the
+// workgroup's own n x n block of minv (m and minv must not overlap).  Fixed reduction order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void inv_det_big_kernel(const double* __restrict__ m, double* __restrict__ minv,
+                                                          double* __restrict__ det, int n) {
+  extern __shared__ double sh[];
+  const long w = blockIdx.x;
+  const int t = threadIdx.x;
+  const long nn = (long)n * n;
+  const bool in_lds = n <= INV_LDS_MAX;
+  char* base = reinterpret_cast<char*>(sh) + (in_lds ? nn * 16 : 0);
+  cplx* a = in_lds ? reinterpret_cast<cplx*>(sh) : reinterpret_cast<cplx*>(minv + 2 * w * nn);
+  cplx* colk = reinterpret_cast<cplx*>(base);
+  double* rv = reinterpret_cast<double*>(base + (long)n * 16);
+  int* ri = reinterpret_cast<int*>(base + (long)n * 16 + 256 * 8);
+  int* ipiv = ri + 256;
+  const cplx* mw = reinterpret_cast<const cplx*>(m + 2 * w * nn);
+  for (long e = t; e < nn; e += 256) a[e] = mw[e];
+  __syncthreads();
+  cplx d = {1.0, 0.0};  // thread 0
+  for (int k = 0; k < n; ++k) {
+    double best = -1.0;
+    int bi = n;
+    for (int i = k + t; i < n; i += 256) {
+      const double v = cabs1(a[(long)i * n + k]);
+      if (v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    rv[t] = best;
+    ri[t] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) {
+        const double v2 = rv[t + s];
+        const int i2 = ri[t + s];
+        if (v2 > rv[t] || (v2 == rv[t] && i2 < ri[t])) {
+          rv[t] = v2;
+          ri[t] = i2;
+        }
+      }
+      __syncthreads();
+    }
+    int piv_row = ri[0];
+    if (piv_row >= n) piv_row = k;  // no comparable entry (NaN column): keep the row
+    __syncthreads();
+    if (t == 0) ipiv[k] = piv_row;
+    if (piv_row != k)
+      for (int j = t; j < n; j += 256) {
+        const cplx x = a[(long)k * n + j];
+        a[(long)k * n + j] = a[(long)piv_row * n + j];
+        a[(long)piv_row * n + j] = x;
+      }
+    __syncthreads();
+    const cplx piv = a[(long)k * n + k];
+    if (t == 0) {
+      d = cmul(d, piv);
+      if (piv_row != k) d = {-d.re, -d.im};
+    }
+    const cplx ip = cinv(piv);
+    for (int i = t; i < n; i += 256) colk[i] = i == k ? cplx{0.0, 0.0} : a[(long)i * n + k];
+    __syncthreads();
+    for (int j = t; j < n; j += 256) a[(long)k * n + j] = cmul(j == k ? cplx{1.0, 0.0} : a[(long)k * n + j], ip);
+    __syncthreads();
+    for (long e = t; e < nn; e += 256) {
+      const int i = (int)(e / n), j = (int)(e - (long)i * n);
+      if (i == k) continue;
+      const cplx x = j == k ? cplx{0.0, 0.0} : a[e];
+      a[e] = csub(x, cmul(colk[i], a[(long)k * n + j]));
+    }
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    const int q = ipiv[k];
+    if (q != k)
+      for (int i = t; i < n; i += 256) {
+        const cplx x = a[(long)i * n + k];
+        a[(long)i * n + k] = a[(long)i * n + q];
+        a[(long)i * n + q] = x;
+      }
+    __syncthreads();
+  }
+  if (in_lds) {
+    cplx* o = reinterpret_cast<cplx*>(minv + 2 * w * nn);
+    for (long e = t; e < nn; e += 256) o[e] = a[e];
+  }
+  if (t == 0) cstore(det, w, d);
 }
 
 // x[i] = a value in [-1, 1) hashed from i (deterministic, non-zero operands for timing).

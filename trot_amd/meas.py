@@ -232,7 +232,8 @@ class Engine:
                 return _lib.local_energy(sa.rot_chol, greens[0], sa.rot_h1t, h0)
             return _lib.local_energy(sa.rot_chol, greens[0], sa.rot_h1t, h0, sb.rot_chol, greens[1], sb.rot_h1t)
         p = self.precision
-        if p == "bf16x3":
+        n_max = max(g.shape[2] for g in greens if g is not None)
+        if p == "bf16x3" and n_max <= _lib.MAX_NOCC:  # the batched exchange GEMM works on 64-row blocks
             G = sa.rot_chol.shape[0]
             if self.kind == "rr":
                 return _lib.local_energy_b3(self._b3(sa), G, greens[0], sa.rot_h1t, h0)

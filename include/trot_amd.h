@@ -441,6 +441,21 @@ int trot_gemm_batched(int opa, int opb, int a_complex, int b_complex, int M, int
                       double beta, double* C, long ldc, long sC, int batch, void* stream);
 int trot_inverse_det(const double* m, int W, int n, double* minv, double* det, void* stream);
 
+/* The whole FP64 propagation of one small walker set as one grouped problem: w <- E . T_K(i sqrt(dt) V) . E w with
+ * V in sym-block storage (W, trot_symblk_cols(N)) complex128 and E (N, N) real, w (W, N, n) in place, sum a
+ * (W, N, n) scratch (the Taylor running sum).  trot_fused_prop_grouped runs one workgroup per walker of every
+ * problem: the one-body product, the n_terms Taylor terms and the last one-body product with the current operand
+ * resident in LDS, each output element accumulated by the same MFMA sequence as trot_one_body_apply and
+ * trot_taylor_apply_symblk (bit-identical results).  trot_fused_prop_problem returns W, or TROT_ERR_UNSUPPORTED
+ * when pad16(N) pad16(n) 16 bytes exceed 160 KiB, N > 256 or n > 48; max_lds is the largest of those byte counts
+ * over the problems of a launch.
+ * Replaces: _apply_trotter_r / _apply_trotter_u per walker set (trot/prop/chol_afqmc_ops.py:198-222) for the
+ * batching form's fragments (the reference steps fragments one at a time). */
+size_t trot_fused_prop_problem_bytes(void);
+long trot_fused_prop_problem(const double* vsym, const double* expm, double* w, double* sum, int W, int N, int n,
+                             double sqrt_dt, int n_terms, void* desc);
+int trot_fused_prop_grouped(const void* probs, const int* start, int nprob, int total_wg, long max_lds, void* stream);
+
 /* ---- CISD trial, restricted walkers (FP64) ----
  * The reference determinant occupies the first nocc orbitals; the active block is occupied orbitals
  * core .. nocc-1 (na = nocc - core) and virtuals nocc .. nocc+nvir-1; ci1 (na, nvir) and ci2 (na, nvir,

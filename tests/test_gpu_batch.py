@@ -4,9 +4,14 @@ Fragments as the batching case defines them (trot_amd.synthetic.fragment_sizes: 
 seed 4, n = round(N/5), G = 4N), each a synthetic system by the scale-up recipe (seed 100 + i, tau 256)
 with a UHF trial of (n, n - 1) occupied orbitals and 128 complex128 walkers, dt = 0.005.  The same 100
 steps with the same pre-drawn fields run once fragment by fragment (eager launches) and once through
-FragmentBatch (one captured graph replayed per step, fragments on four parallel branches, Taylor tile
-shapes autotuned per fragment shape); walkers, weights, overlaps, population shifts and node counts must
-be bit-identical after steps 1, 10 and 100.
+FragmentBatch (one captured graph replayed per step: grouped launches over all fragments, with the whole
+propagation of every fragment in one fused launch or as one-body / Taylor-term launches with autotuned
+tile shapes, or the fragments on four parallel branches); walkers, weights, overlaps, population shifts
+and node counts must be bit-identical after steps 1, 10 and 100.
+
+The fused propagation alone (trot_fused_prop_grouped) is also checked bit for bit against
+trot_one_body_apply, trot_taylor_apply_symblk and trot_one_body_apply on walker sets from N = 9, n = 1 to
+N = 256, n = 32 (sixteen 16-row tiles, the largest operand that fits in LDS) in one launch.
 """
 
 from __future__ import annotations
@@ -53,14 +58,14 @@ def _snapshot(st):
         st.node_encounters.cpu().numpy()]
 
 
-@pytest.mark.parametrize("grouped", [True, False])
-def test_fragment_batch_bit_identical(gpu, grouped):
+@pytest.mark.parametrize("grouped,fused", [(True, True), (True, False), (False, False)])
+def test_fragment_batch_bit_identical(gpu, grouped, fused):
     from trot_amd import batch, synthetic
 
-    print(f"[batch] grouped launches: {grouped}", flush=True)
+    print(f"[batch] grouped launches: {grouped}, fused propagation: {fused}", flush=True)
     frags = _fragments()
     init = [f.state for f in frags]
-    fb = batch.FragmentBatch(frags, grouped=grouped)
+    fb = batch.FragmentBatch(frags, grouped=grouped, fused=fused)
     tiles = fb.autotune()  # per (walkers, N, n): one of the four FP64 Taylor tile shapes, timed on the GPU
     fields = [[synthetic.fields(100 + i, s, W, f.n_fields) for i, f in enumerate(frags)] for s in range(STEPS)]
     checkpoints = {1, 10, STEPS}
@@ -95,7 +100,9 @@ def test_fragment_batch_bit_identical(gpu, grouped):
     torch.cuda.synchronize()
     t_graph = time.perf_counter() - t0
     n_finite = sum(bool(np.all(np.isfinite(ref[(i, STEPS)][-4]))) for i in range(len(frags)))
-    print(f"[batch] grouped {grouped}: 32 fragments x {STEPS} steps: eager one-by-one {t_eager:.2f} s, "
+    if grouped:
+        assert fb.plan.n_fused == (2 * len(frags) if fused else 0)
+    print(f"[batch] grouped {grouped} fused {fused}: 32 fragments x {STEPS} steps: eager one-by-one {t_eager:.2f} s, "
           f"graph {t_graph:.2f} s for {STEPS - 1} steps ({1e3 * t_graph / (STEPS - 1):.2f} ms/step); "
           f"fragments with finite weights after {STEPS} steps: {n_finite}; Taylor tiles "
           f"{sorted(set(tiles.values()))} over {len(tiles)} shapes", flush=True)
@@ -104,3 +111,37 @@ def test_fragment_batch_bit_identical(gpu, grouped):
         for x, y in zip(a, b):
             assert np.array_equal(x, y, equal_nan=True), key
     assert all(np.all(np.isfinite(ref[(i, 1)][-4])) for i in range(len(frags)))
+
+
+def test_fused_propagation_bit_identical(gpu):
+    from trot_amd import _lib
+
+    shapes = [(3, 9, 1), (5, 40, 8), (4, 123, 25), (2, 256, 32), (3, 200, 40), (6, 64, 48)]
+    g = _lib.Grouped("fused")
+    runs = []
+    sqrt_dt, K = 0.07, 6
+    for s, (W, N, n) in enumerate(shapes):
+        gen = torch.Generator().manual_seed(s)
+        sym = lambda: (lambda a: (a + a.transpose(1, 2)) / N)(torch.randn((W, N, N), generator=gen,
+                                                                           dtype=torch.float64))
+        v = torch.complex(_lib.pack_symblk(sym().cuda()), _lib.pack_symblk(sym().cuda())).contiguous()
+        e = torch.linalg.matrix_exp(-0.01 * sym()[0]).contiguous().cuda()
+        w = torch.randn((W, N, n, 2), generator=gen, dtype=torch.float64)
+        w = torch.view_as_complex(w).contiguous().cuda()
+        ref = _lib.one_body_apply(e, _lib.taylor_apply_symblk(v, _lib.one_body_apply(e, w), sqrt_dt, K))
+        wf, acc = w.clone(), torch.empty_like(w)
+        assert g.add_fused(v, e, wf, acc, W, N, n, sqrt_dt, K)
+        runs.append((W, N, n, ref, wf))
+    # operands beyond LDS (N x n padded, complex FP64, over 160 KiB) or beyond three 16-column tiles are refused
+    big = torch.empty((1, 256, 48), dtype=torch.complex128, device="cuda")
+    vb = torch.empty((1, _lib.symblk_cols(256)), dtype=torch.complex128, device="cuda")
+    eb = torch.empty((256, 256), dtype=torch.float64, device="cuda")
+    assert not g.add_fused(vb, eb, big, torch.empty_like(big), 1, 256, 48, sqrt_dt, K)
+    wide = torch.empty((1, 64, 49), dtype=torch.complex128, device="cuda")
+    assert not g.add_fused(vb, eb, wide, torch.empty_like(wide), 1, 64, 49, sqrt_dt, K)
+    print(f"[fused] {len(shapes)} walker sets, largest LDS {g.max_lds} bytes", flush=True)
+    g.finish().launch()
+    torch.cuda.synchronize()
+    for W, N, n, ref, wf in runs:
+        assert torch.equal(wf, ref), (W, N, n)
+        assert torch.isfinite(torch.view_as_real(wf)).all()

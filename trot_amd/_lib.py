@@ -79,6 +79,9 @@ _SIGS = {
     "trot_gemm_batched": (_i, [_i, _i, _i, _i, _i, _i, _i, _d, _d, _vp, _l, _l, _vp, _l, _l, _d, _vp, _l, _l, _i,
                                _vp]),
     "trot_inverse_det": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
+    "trot_fused_prop_problem_bytes": (_sz, []),
+    "trot_fused_prop_problem": (_l, [_vp, _vp, _vp, _vp, _i, _i, _i, _d, _i, _vp]),
+    "trot_fused_prop_grouped": (_i, [_vp, _vp, _i, _i, _l, _vp]),
     "trot_gemm_problem_bytes": (_sz, []),
     "trot_gemm_problem": (_l, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _vp]),
     "trot_gemm_f64_grouped": (_i, [_i, _vp, _vp, _i, _i, _vp]),
@@ -1067,6 +1070,11 @@ GOP = {"one_body": 0, "taylor": 1, "overlap": 2, "green": 3, "vhs": 4, "taylor_n
 TAYLOR_GOP_OF_SHAPE = {0: "taylor_64x64", 1: "taylor_64x32", 2: "taylor", 3: "taylor_narrow"}
 
 
+def fused_prop_lds(N: int, n: int) -> int:
+    """Bytes of LDS the fused propagation of a walker set takes: its N x n operand, padded to 16, complex FP64."""
+    return ((N + 15) // 16 * 16) * ((n + 15) // 16 * 16) * 16
+
+
 class Grouped:
     """Descriptors of one operation over many walker sets, gathered into device memory, and its launch.
 
@@ -1079,7 +1087,9 @@ class Grouped:
         L = lib()
         self.size = {"gemm": L.trot_gemm_problem_bytes, "inv": L.trot_inv_problem_bytes,
                      "copy": L.trot_copy_problem_bytes, "sum": L.trot_sum_problem_bytes,
-                     "shift": L.trot_shift_problem_bytes, "update": L.trot_update_problem_bytes}[kind]()
+                     "shift": L.trot_shift_problem_bytes, "update": L.trot_update_problem_bytes,
+                     "fused": L.trot_fused_prop_problem_bytes}[kind]()
+        self.max_lds = 0
         self.descs, self.counts, self.keep = [], [], []
         self.max_bytes = 0
         self.probs = self.start = None
@@ -1136,6 +1146,19 @@ class Grouped:
                                          float(w_cap), float(damping), buf), "trot_update_problem")
         self._add(buf, 1, (det_a, det_b, overlaps, shift, fbt, weights, scal, nodes))
 
+    def add_fused(self, vsym, expm, w, acc, W: int, N: int, n: int, sqrt_dt: float, n_terms: int):
+        """The whole FP64 propagation E . T_K . E of one walker set (trot_fused_prop_problem), or False when the
+        set is too large for its LDS-resident series."""
+        buf = ctypes.create_string_buffer(self.size)
+        r = lib().trot_fused_prop_problem(_p(vsym), _p(expm), _p(w), _p(acc), int(W), int(N), int(n), float(sqrt_dt),
+                                          int(n_terms), buf)
+        if r == -2:
+            return False
+        _check(0 if r > 0 else int(r), "trot_fused_prop_problem")
+        self._add(buf, r, (vsym, expm, w, acc))
+        self.max_lds = max(self.max_lds, fused_prop_lds(N, n))
+        return True
+
     def add_copy(self, src: torch.Tensor, dst: torch.Tensor):
         if src.numel() != dst.numel() or src.dtype != dst.dtype or not (src.is_contiguous() and dst.is_contiguous()):
             raise ValueError("copy: matching contiguous tensors needed")
@@ -1175,5 +1198,8 @@ class Grouped:
                    "trot_shift_grouped")
         elif self.kind == "update":
             _check(L.trot_update_grouped(_p(self.probs), n, _stream()), "trot_update_grouped")
+        elif self.kind == "fused":
+            _check(L.trot_fused_prop_grouped(_p(self.probs), _p(self.start), n, self.total, self.max_lds, _stream()),
+                   "trot_fused_prop_grouped")
         else:
             _check(L.trot_copy_grouped(_p(self.probs), n, self.max_bytes, _stream()), "trot_copy_grouped")

@@ -107,7 +107,7 @@ class _GroupedPlan:
     problem of the fragment's own entry point, so the step is bit-identical to stepping the fragments
     one at a time."""
 
-    def __init__(self, frags: List[Fragment], views, tiles: dict | None = None):
+    def __init__(self, frags: List[Fragment], views, tiles: dict | None = None, fused: bool = True):
         from .trial import _coeffs
 
         n_terms = {int(f.params.n_exp_terms) for f in frags}
@@ -124,6 +124,9 @@ class _GroupedPlan:
         # FragmentBatch.autotune ran, else 32 x 32 for n <= 32 and 32 x 64 above (every shape: the same bits)
         tay = [{op: _lib.Grouped("gemm", op) for op in _lib.TAYLOR_GOP_OF_SHAPE.values()} for _ in range(self.n_terms)]
         ob2 = {op: _lib.Grouped("gemm", op) for op in ("one_body", "one_body_narrow")}
+        # the whole propagation E . T_K . E of a walker set in one workgroup per walker when its operand fits in
+        # LDS (trot_fused_prop_problem); bigger sets take the launches above
+        fus = _lib.Grouped("fused")
         tiles = tiles or {}
         ovl = _lib.Grouped("gemm", "overlap")
         inv = _lib.Grouped("inv")
@@ -158,16 +161,17 @@ class _GroupedPlan:
                 m = torch.empty((W, n, n), dtype=torch.complex128, device=dev)
                 minv = torch.empty_like(m)
                 det = torch.empty((W,), dtype=torch.complex128, device=dev)
-                obop = "one_body_narrow" if n <= 32 else "one_body"
-                shape = tiles.get((W, N, n, True), 3 if n <= 32 else 2)
-                tayop = _lib.TAYLOR_GOP_OF_SHAPE[shape]
-                ob1[obop].add_gemm(ctx.exp_h1_half, w, t, None, W, N, n)
-                cp.add_copy(t, t0)
-                prev, nxt = t0, t1
-                for k in range(1, self.n_terms + 1):
-                    tay[k - 1][tayop].add_gemm(b["v"], prev, nxt, t, W, N, n, ctx.sqrt_dt / k)
-                    prev, nxt = nxt, prev
-                ob2[obop].add_gemm(ctx.exp_h1_half, t, w, None, W, N, n)
+                if not (fused and fus.add_fused(b["v"], ctx.exp_h1_half, w, t, W, N, n, ctx.sqrt_dt, self.n_terms)):
+                    obop = "one_body_narrow" if n <= 32 else "one_body"
+                    shape = tiles.get((W, N, n, True), 3 if n <= 32 else 2)
+                    tayop = _lib.TAYLOR_GOP_OF_SHAPE[shape]
+                    ob1[obop].add_gemm(ctx.exp_h1_half, w, t, None, W, N, n)
+                    cp.add_copy(t, t0)
+                    prev, nxt = t0, t1
+                    for k in range(1, self.n_terms + 1):
+                        tay[k - 1][tayop].add_gemm(b["v"], prev, nxt, t, W, N, n, ctx.sqrt_dt / k)
+                        prev, nxt = nxt, prev
+                    ob2[obop].add_gemm(ctx.exp_h1_half, t, w, None, W, N, n)
                 ovl.add_gemm(c, w, m, None, W, N, n)
                 inv.add_inv(m, minv, det, W, n)
                 grn.add_gemm(w, minv, greens[si], None, W, N, n)
@@ -201,7 +205,10 @@ class _GroupedPlan:
         terms = [g for groups in tay for g in groups.values() if g.descs]
         first = [g for g in ob1.values() if g.descs]
         last = [g for g in ob2.values() if g.descs]
-        self.ops = [g.finish() for g in (vhs, *first, cp, *terms, *last, ovl, inv, grn)]
+        copies = [cp] if cp.descs else []
+        fusedl = [fus] if fus.descs else []
+        self.n_fused = len(fus.descs)
+        self.ops = [g.finish() for g in (vhs, *fusedl, *first, *copies, *terms, *last, ovl, inv, grn)]
         # spin a's sums write fb before spin b's one-slice GEMMs and sums add to it
         self.pre = [g.finish() for g in (fb_a, sum_a, fb_b, sum_b, shf) if g.descs]
         self.post = [upd.finish()]
@@ -232,9 +239,13 @@ class FragmentBatch:
     one-body, Taylor terms, overlaps, inverses, Green's functions, weight update; the trot_*_grouped
     entry points; one launch per tile shape where fragments differ); the results stay bit-identical to the
     one-by-one run.  ``autotune`` before ``capture`` picks every fragment's Taylor tile shape by timing.
+    ``fused`` (default): the propagation of every walker set whose N x n operand fits in LDS (N <= 256,
+    n <= 48) is ONE launch over all of them (trot_fused_prop_grouped: one workgroup per walker runs the
+    one-body product, all Taylor terms and the last one-body product), also bit-identical.
     """
 
-    def __init__(self, fragments: List[Fragment], n_streams: int = 4, grouped: bool | None = None):
+    def __init__(self, fragments: List[Fragment], n_streams: int = 4, grouped: bool | None = None,
+                 fused: bool = True):
         if not fragments:
             raise ValueError("FragmentBatch needs at least one fragment")
         if n_streams < 1:
@@ -261,6 +272,7 @@ class FragmentBatch:
         self.grouped = can if grouped is None else bool(grouped)
         self.plan = None
         self.tiles = {}
+        self.fused = bool(fused)
 
     def autotune(self):
         """Pick the FP64 Taylor-term tile shape of every fragment's (walkers, N, n) per spin by timing
@@ -307,7 +319,7 @@ class FragmentBatch:
                 torch.float64).contiguous()
             f.state = f.state._replace(pop_control_ene_shift=scal[0], e_estimate=scal[1])
             f._scal = scal
-        self.plan = _GroupedPlan(self.fragments, views, self.tiles)
+        self.plan = _GroupedPlan(self.fragments, views, self.tiles, self.fused)
         for f, b in zip(self.fragments, self.plan.per):
             b["scal"] = f._scal
 

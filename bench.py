@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--cisd-workload", default="cisd_n128_o16_g512_w1024")
     ap.add_argument("--cisd-only", action="store_true",
                     help="only the CISD-trial rates, printed as their own JSON line (profiling runs)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batching form's rates (32 fragments; profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device copies of 1 GiB (scale reference for rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -192,6 +194,9 @@ def main():
         return
     if args.cisd_only:
         print(json.dumps(cisd_rates(args)), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps(batch_rates()), flush=True)
         return
     import torch
 
@@ -620,7 +625,9 @@ def _fragment(i, N, n, G, W, dt, chol_scale):
 def batch_rates(steps: int = 20) -> dict:
     """The batching form: 32 fragments (N ~ U[40, 200] seed 4, n = round(N/5), G = 4N, UHF trial (n, n-1), 128
     walkers each, dt = 0.005, L / sqrt(N)), FP64: one fragment after another with eager launches, one HIP graph
-    of per-fragment launches on 4 branches, and one HIP graph of grouped launches (autotuned Taylor tiles).
+    of per-fragment launches on 4 branches, one HIP graph of grouped launches with the whole propagation of
+    every walker set in one fused launch (the default), and the same with the propagation as one-body / copy /
+    Taylor-term launches on autotuned tiles.
     frac = FP64 MFMA ideal of the step's GEMMs (VHS on the stored half of V, force bias, Taylor, one-body)
     over the grouped step's time."""
     import torch
@@ -631,11 +638,15 @@ def batch_rates(steps: int = 20) -> dict:
     sizes = synthetic.fragment_sizes(32, seed=4)
     out = {"workload": "32 fragments, N ~ U[40, 200], n = N/5, G = 4N, UHF, 128 walkers each", "steps": steps}
     fields = None
-    for mode in ("eager", "graph_4_branches", "grouped_graph"):
+    for mode in ("eager", "graph_4_branches", "grouped_graph", "grouped_graph_unfused"):
         frags = [_fragment(i, N, n, G, W, dt, 1.0 / np.sqrt(N)) for i, (N, n, G) in enumerate(sizes)]
         if fields is None:
             fields = [[torch.from_numpy(synthetic.fields(100 + i, s, W, f.n_fields)).cuda() for i, f in
                        enumerate(frags)] for s in range(steps + 3)]
+            # Taylor tile shapes of every (walkers, N, n), tuned once into the process-wide table that every
+            # mode's Taylor launches read
+            out["taylor_tiles"] = {f"{k[1]}x{k[2]}": v for k, v in
+                                   sorted(batch.FragmentBatch(frags, grouped=True).autotune().items())}
         if mode == "eager":
             for s in range(2):
                 for i, f in enumerate(frags):
@@ -647,9 +658,9 @@ def batch_rates(steps: int = 20) -> dict:
                     f.state = f.step(fields[s][i])
             torch.cuda.synchronize()
         else:
-            fb = batch.FragmentBatch(frags, grouped=mode == "grouped_graph")
-            if mode == "grouped_graph":
-                out["taylor_tiles"] = {f"{k[1]}x{k[2]}": v for k, v in sorted(fb.autotune().items())}
+            fb = batch.FragmentBatch(frags, grouped=mode != "graph_4_branches", fused=mode == "grouped_graph")
+            if mode != "graph_4_branches":
+                fb.autotune()  # the grouped plan takes the tile shapes of its own batch
             flat = [fb.pack_fields([x.cpu().numpy() for x in fs]) for fs in fields]
             fb.capture(flat[0])
             fb.step(flat[1])

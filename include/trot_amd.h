@@ -427,8 +427,9 @@ int trot_update_grouped(const void* probs, int nprob, void* stream);
  *   K x N after op; real or interleaved-complex operands (conjugation of a real operand is an error), complex C
  *   whenever either operand is complex; leading dimensions and batch strides in elements (a stride of 0
  *   shares the operand), alpha complex, beta real.
- * trot_inverse_det: minv[b] = m[b]^-1 and det[b] = det m[b] (complex n x n, n <= 64), the kernel of
- *   trot_overlap_green.
+ * trot_inverse_det: minv[b] = m[b]^-1 and det[b] = det m[b] (complex n x n, n <= 256, m != minv), the
+ *   kernels of trot_overlap_green (register-resident Gauss-Jordan up to 64, LDS-resident up to 96, in place
+ *   in minv above).
  * Replaces: the matrix products and the inverse that jax.grad takes through afqmc_step and the local energy
  * (trot/prop/afqmc.py:79-144, trot/prop/utils.py:20-29, trot/meas/rhf.py:21-25, 128-138; the reference is
  * differentiable end to end, README.md:11). */

@@ -65,13 +65,36 @@ class AdjCtx:
     A: np.ndarray  # -dt/2 h1_eff (host)
     C: torch.Tensor  # (N, n) float64 trial
     Lflat: torch.Tensor  # (G, N*N) float64
+    Lsym: torch.Tensor  # (G, symblk_cols(N)) float64: the stored 8 x 8 blocks of every symmetric L_g
+    sym_inv: torch.Tensor  # (N*N,) long: stored column of entry (i, k) of a symmetric matrix
+    sym_i1: torch.Tensor  # (cols,) long: entry (i, k) a stored column holds
+    sym_i2: torch.Tensor  # (cols,) long: its mirror (k, i)
+    sym_off: torch.Tensor  # (cols,) float64: 1 for columns of off-diagonal blocks, 0 on diagonal blocks
+    sym_valid: torch.Tensor  # (cols,) float64: 0 for the padding of the last block row
     Rt: torch.Tensor  # (G, N*n) float64: R'[g, j*n + p] = rot_chol[g, p, j]
     rot_chol: torch.Tensor  # (G, n, N) float64
     rot_h1t: torch.Tensor  # (N, n) float64 = h1 C
     params: prop.QmcParams
     precision: str = "f64"
-    Lflat32: torch.Tensor | None = None
     Rt32: torch.Tensor | None = None
+    Lsym32: torch.Tensor | None = None
+
+
+def _symblk_maps(N: int, device):
+    """Index maps between a dense symmetric N x N matrix and its sym-block storage (trot_pack_symblk's
+    layout): packing the matrices of flat indices (+1, so that padding reads 0) and of their transposes."""
+    idx = torch.arange(1, N * N + 1, dtype=F64, device=device).view(1, N, N)
+    p1 = _lib.pack_symblk(idx)[0]
+    p2 = _lib.pack_symblk(idx[0].T.contiguous().view(1, N, N))[0]
+    valid = p1 > 0
+    i1 = (p1 - 1).clamp(min=0).long()
+    i2 = (p2 - 1).clamp(min=0).long()
+    off = valid & ((i1 // N) // 8 != (i1 % N) // 8)
+    cols = torch.arange(p1.numel(), device=device)
+    inv = torch.zeros(N * N, dtype=torch.long, device=device)
+    inv[i1[valid]] = cols[valid]
+    inv[i2[valid]] = cols[valid]  # V is symmetric: either stored copy of a diagonal-block entry serves
+    return inv, i1, i2, off.to(F64), valid.to(F64)
 
 
 def build_ctx(h0: float, h1, chol, c: np.ndarray, params: prop.QmcParams, precision: str = "f64") -> AdjCtx:
@@ -95,11 +118,13 @@ def build_ctx(h0: float, h1, chol, c: np.ndarray, params: prop.QmcParams, precis
     ac = AdjCtx(N=N, n=n, G=G, dt=pctx.dt, sqrt_dt=pctx.sqrt_dt, n_terms=params.n_exp_terms, h0=float(h0),
                 h0_prop=complex(pctx.h0_prop), mf=pctx.mf_shifts, E=pctx.exp_h1_half.contiguous(),
                 A=(-0.5 * pctx.dt) * h1_eff.cpu().numpy(), C=torch.as_tensor(c, dtype=F64, device="cuda"),
-                Lflat=pctx.chol_flat, Rt=sp.rot_chol_t.reshape(G, N * n), rot_chol=sp.rot_chol,
+                Lflat=pctx.chol_flat, Lsym=_lib.pack_symblk(ham.chol), sym_inv=None, sym_i1=None, sym_i2=None,
+                sym_off=None, sym_valid=None, Rt=sp.rot_chol_t.reshape(G, N * n), rot_chol=sp.rot_chol,
                 rot_h1t=sp.rot_h1t, params=params, precision=precision)
+    ac.sym_inv, ac.sym_i1, ac.sym_i2, ac.sym_off, ac.sym_valid = _symblk_maps(N, ac.Lsym.device)
     if precision == "f32":
-        ac.Lflat32 = ac.Lflat.to(torch.float32).contiguous()
         ac.Rt32 = ac.Rt.to(torch.float32).contiguous()
+        ac.Lsym32 = ac.Lsym.to(torch.float32).contiguous()
     return ac
 
 
@@ -143,21 +168,28 @@ def _rt_apply(ac: AdjCtx, x: torch.Tensor) -> torch.Tensor:
 
 
 def _vhs(ac: AdjCtx, xbar: torch.Tensor) -> torch.Tensor:
+    """V = sum_g xbar_g L_g (W, N, N): the product over the stored half of the symmetric L_g (sym-block
+    storage, half the flops of the dense product), then unpacked."""
     W = xbar.shape[0]
     if ac.precision == "f32":
-        v = _lib.vhs_build_lp(xbar, ac.Lflat32, "f32", out_f64=True)
+        v = _lib.vhs_build_lp(xbar, ac.Lsym32, "f32", out_f64=True)
     else:
-        v = _lib.vhs_build(xbar, ac.Lflat)
-    return v.view(W, ac.N, ac.N)
+        v = _lib.vhs_build(xbar, ac.Lsym)
+    return v.index_select(1, ac.sym_inv).view(W, ac.N, ac.N)
 
 
 def _l_dot(ac: AdjCtx, vhat_conj: torch.Tensor) -> torch.Tensor:
-    """xbar^_g = <L_g, V^> = conj(sum_ij L_g,ij conj(V^)_ij), (W, G)."""
+    """xbar^_g = <L_g, V^> = conj(sum_ij L_g,ij conj(V^)_ij), (W, G).  L_g is symmetric, so the sum runs over
+    its stored blocks: P = conj(V^)_ik + conj(V^)_ki on off-diagonal blocks and conj(V^)_ik on diagonal ones,
+    contracted with the sym-block L (half the flops of the dense contraction)."""
     W = vhat_conj.shape[0]
+    flat = vhat_conj.reshape(W, ac.N * ac.N)
+    P = (flat.index_select(1, ac.sym_i1) + ac.sym_off * flat.index_select(1, ac.sym_i2)) * ac.sym_valid
+    cols = P.shape[1]
     if ac.precision == "f32":
-        r = _lib.force_bias_lp(vhat_conj, ac.Lflat32.view(ac.G, ac.N, ac.N), 1.0, "f32")
+        r = _lib.force_bias_lp(P.view(W, cols // 64, 64), ac.Lsym32.view(ac.G, cols // 64, 64), 1.0, "f32")
     else:
-        r = _lib.force_bias_flat(vhat_conj.reshape(W, ac.N * ac.N), ac.Lflat, 1.0)
+        r = _lib.force_bias_flat(P, ac.Lsym, 1.0)
     return r.conj()
 
 
@@ -246,12 +278,28 @@ def _green_adjoint(ac: AdjCtx, gt, minv, gt_hat, l_hat, w_hat):
     return _lib.gemm_batched(Y, minv, opb="C", beta=1.0, out=w_hat)
 
 
+def _k_slices(W: int, tiles: int, target_wg: int = 512) -> int:
+    """Walker slices for a split-K outer product: the largest divisor of W up to target_wg / tiles."""
+    s = max(1, min(W, target_wg // max(tiles, 1)))
+    while W % s:
+        s -= 1
+    return s
+
+
 def _acc_outer(acc: torch.Tensor, x: torch.Tensor, y: torch.Tensor) -> None:
-    """acc (N x N complex) += sum_w x_w y_w^H as one GEMM over K = W n (walkers moved to the column index)."""
+    """acc (N x N complex) += sum_w x_w y_w^H, a GEMM over K = W n (walkers moved to the column index).  The N x N
+    output has few tiles, so K is split over S walker slices (one batched GEMM of S partial products, about 512
+    workgroups) and the partial products are summed over the slices."""
     W, N, n = x.shape
     xp = x.permute(1, 0, 2).reshape(N, W * n)
     yp = y.permute(1, 0, 2).reshape(N, W * n)
-    _lib.gemm_batched(xp, yp, opb="C", beta=1.0, out=acc.view(1, N, N))
+    S = _k_slices(W, ((N + 63) // 64) ** 2)
+    if S == 1:
+        _lib.gemm_batched(xp, yp, opb="C", beta=1.0, out=acc.view(1, N, N))
+        return
+    kc = (W // S) * n
+    part = _lib.gemm_batched(xp.view(N, S, kc).permute(1, 0, 2), yp.view(N, S, kc).permute(1, 0, 2), opb="C")
+    acc += part.sum(0)
 
 
 def _energy_grad_gt(ac: AdjCtx, gt: torch.Tensor, chunk: int = 64) -> torch.Tensor:

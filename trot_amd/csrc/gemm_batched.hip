@@ -16,8 +16,11 @@
 namespace {
 
 // Storage of op(X) for the tile: K-contiguous A / N-contiguous B for N and R, the other way for T and C.
+// Products with at most 32 output columns (the walker side of n <= 32) run on 64 x 32 tiles, which do half
+// the MFMA work of a 64 x 64 tile on them; every tile shape accumulates K in the same order.
 template <bool AC, bool BC, bool AKC, bool BNC, bool CJA, bool CJB>
 hipError_t run(const GemmP& p, int batch, hipStream_t s) {
+  if (p.N <= 32) return launch_gemm_f64<64, 32, AC, BC, AKC, BNC, CJA, EPI_STORE, false, CJB>(p, batch, s);
   return launch_gemm_f64<64, 64, AC, BC, AKC, BNC, CJA, EPI_STORE, false, CJB>(p, batch, s);
 }
 

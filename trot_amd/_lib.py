@@ -213,8 +213,9 @@ F64 = torch.float64
 # ---------------------------------------------------------------------------
 
 
-def overlap_green(trial: torch.Tensor, walkers: torch.Tensor, want_green: bool = True):
-    """(det (W,) c128, green_t (W,N,n) c128 or None) for one spin."""
+def overlap_green(trial: torch.Tensor, walkers: torch.Tensor, want_green: bool = True,
+                  out_green: torch.Tensor | None = None):
+    """(det (W,) c128, green_t (W,N,n) c128 or None) for one spin; ``out_green`` receives green_t when given."""
     W, N, n = walkers.shape
     _req(walkers, "walkers", C128, (W, N, n))
     _req(trial, "trial", F64, (N, n))
@@ -222,7 +223,13 @@ def overlap_green(trial: torch.Tensor, walkers: torch.Tensor, want_green: bool =
         raise ValueError(f"nocc={n} > {MAX_NOCC_BIG} is not supported by the GPU kernels")
     L = lib()
     det = torch.empty((W,), dtype=C128, device=walkers.device)
-    gt = torch.empty((W, N, n), dtype=C128, device=walkers.device) if want_green else None
+    gt = None
+    if want_green:
+        if out_green is not None:
+            _req(out_green, "out_green", C128, (W, N, n))
+            gt = out_green
+        else:
+            gt = torch.empty((W, N, n), dtype=C128, device=walkers.device)
     ws = _ws(L.trot_overlap_green_workspace(W, N, n), walkers.device)
     _check(L.trot_overlap_green(_p(trial), _p(walkers), W, N, n, _p(det), _p(gt), _p(ws), ws.numel(), _stream()),
            "trot_overlap_green")

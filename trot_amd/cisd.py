@@ -288,7 +288,8 @@ class CisdEngine:
         self.precision = precision
         self.doubles_precision = doubles_precision
 
-    def overlap_green(self, walkers, trial_data: CisdTrial):
+    def overlap_green(self, walkers, trial_data: CisdTrial, out=None):
+        """``out`` is accepted for the engine interface; the CISD parts are allocated here."""
         if isinstance(walkers, tuple):
             raise ValueError("CISD trial supports restricted walkers only")
         return _overlap_parts(walkers, trial_data)

@@ -81,7 +81,10 @@ class _Recorded:
         if not isinstance(pctx.fields, (DeviceFields, StateFields)):
             raise ValueError("recording needs the device field stream (DeviceFields) on the propagation context")
         out = dict(kw)
-        out["prop_ctx"] = dataclasses.replace(pctx, fields=StateFields(self.rng))
+        # inside the recording the propagation may overwrite the static walkers it was given (nothing reads
+        # them after it), which saves copying the new walkers back into them at the end of every replay
+        out["prop_ctx"] = dataclasses.replace(pctx, fields=StateFields(self.rng),
+                                              low={**pctx.low, "walkers_in_place": True})
         return out
 
     def _run(self, state, kw, run, n_draws):

@@ -158,8 +158,8 @@ class Engine:
             spin.low["rot_chol_b3"] = _lib.rot_chol_b3(spin.rot_chol.contiguous())
         return spin.low["rot_chol_b3"]
 
-    def overlap_green(self, walkers, trial_data):
-        return overlap_green(walkers, trial_data, want_green=True)
+    def overlap_green(self, walkers, trial_data, out=None):
+        return overlap_green(walkers, trial_data, want_green=True, out=out)
 
     def observable_from_green(self, name: str, greens, meas_ctx, trial_data) -> torch.Tensor:
         """Per-walker block observables from the transposed half Green's functions.

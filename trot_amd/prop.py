@@ -287,11 +287,11 @@ class KernelTimer:
     def __init__(self):
         self.events: dict[str, list] = {}
 
-    def wrap(self, name: str, fn, *args):
+    def wrap(self, name: str, fn, *args, **kwargs):
         s = torch.cuda.Event(enable_timing=True)
         e = torch.cuda.Event(enable_timing=True)
         s.record()
-        out = fn(*args)
+        out = fn(*args, **kwargs)
         e.record()
         self.events.setdefault(name, []).append((s, e))
         return out
@@ -365,10 +365,12 @@ def _trotter_b3(ctx: CholAfqmcCtx, v: torch.Tensor, w: torch.Tensor, n_terms: in
     else:
         fn = _lib.trotter_b3_symblk
         args = (v, w.contiguous(), ctx.low["onebody_b3"], ctx.sqrt_dt, n_terms)
+    # a recorded graph step writes the new walkers over its static input buffer (run_b3 allows w_out == w_in)
+    kw = {"out": w} if ctx.low.get("walkers_in_place") and w.is_contiguous() else {}
     timer = ctx.low.get("timer")
     if timer is not None:
-        return timer.wrap("trotter", fn, *args)
-    return fn(*args)
+        return timer.wrap("trotter", fn, *args, **kw)
+    return fn(*args, **kw)
 
 
 def apply_trotter(walkers, xbar: torch.Tensor, ctx: CholAfqmcCtx, n_terms: int):
@@ -422,7 +424,9 @@ def afqmc_step(state: PropState, *, params, ham_data, trial_data, meas_ops: Meas
     fb = engine.force_bias_from_green(greens, meas_ctx)  # afqmc.py:95-98
     xbar, shift_term, fb_term = _lib.field_shift(fields, fb, prop_ctx.mf_shifts, prop_ctx.sqrt_dt)  # :99-103
     walkers_new = apply_trotter(state.walkers, xbar, prop_ctx, params.n_exp_terms)  # :104-106
-    overlaps_new, greens_new = engine.overlap_green(walkers_new, trial_data)  # :108-110
+    # inside a recorded graph the new Green's functions overwrite the state's (read only by the force bias above)
+    g_out = state.green if prop_ctx.low.get("walkers_in_place") and state.green is not None else None
+    overlaps_new, greens_new = engine.overlap_green(walkers_new, trial_data, out=g_out)  # :108-110
     weights = state.weights.clone()
     scal = torch.stack([state.pop_control_ene_shift, state.e_estimate]).to(torch.float64).contiguous()
     nodes = state.node_encounters.clone()

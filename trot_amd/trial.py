@@ -99,19 +99,21 @@ def _coeffs(trial_data):
     return trial_data.mo_coeff, trial_data.mo_coeff
 
 
-def overlap_green(walkers, trial_data, want_green: bool = True):
+def overlap_green(walkers, trial_data, want_green: bool = True, out=None):
     """Batched overlaps and transposed half Green's functions.
 
     Returns (overlap (W,), (GT_a, GT_b or None)).  Restricted walkers give
     det(C^T w)^2 (trot/trial/rhf.py:45-47); unrestricted give det_a det_b
-    (trot/trial/rhf.py:50-54, trot/trial/uhf.py:54-58).
+    (trot/trial/rhf.py:50-54, trot/trial/uhf.py:54-58).  ``out`` (GT_a, GT_b or None): buffers that receive
+    the Green's functions.
     """
     ca, cb = _coeffs(trial_data)
+    oa, ob = out if out is not None else (None, None)
     if isinstance(walkers, tuple):
-        det_a, gt_a = _lib.overlap_green(ca, walkers[0], want_green)
-        det_b, gt_b = _lib.overlap_green(cb, walkers[1], want_green)
+        det_a, gt_a = _lib.overlap_green(ca, walkers[0], want_green, oa)
+        det_b, gt_b = _lib.overlap_green(cb, walkers[1], want_green, ob)
         return _lib.overlap_combine(det_a, det_b), (gt_a, gt_b)
-    det_a, gt_a = _lib.overlap_green(ca, walkers, want_green)
+    det_a, gt_a = _lib.overlap_green(ca, walkers, want_green, oa)
     return _lib.overlap_combine(det_a, None), (gt_a, None)
 
 

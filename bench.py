@@ -601,7 +601,7 @@ def free_projection_rates(syn, env, args) -> dict:
     return out
 
 
-def _fragment(i, N, n, G, W, dt, chol_scale):
+def _fragment(i, N, n, G, W, dt, chol_scale, precision="f64"):
     import torch
 
     from trot_amd import batch, meas, prop, synthetic, trial
@@ -611,8 +611,12 @@ def _fragment(i, N, n, G, W, dt, chol_scale):
     sys_ = System(N, (n, n - 1), "unrestricted")
     ham = HamChol(syn.h0, syn.h1, syn.chol).to_device()
     tr = trial.UhfTrial(syn.trial, syn.trial_b)
-    mops, tops = meas.make_uhf_meas_ops(sys_), trial.make_uhf_trial_ops(sys_)
-    pops = prop.make_prop_ops("restricted", "unrestricted")
+    mops, tops = meas.make_uhf_meas_ops(sys_, precision=precision), trial.make_uhf_trial_ops(sys_)
+    if precision == "f64":
+        pops = prop.make_prop_ops("restricted", "unrestricted")
+    else:  # the headline's mixed form
+        pops = prop.make_prop_ops("restricted", "unrestricted", precision=precision, taylor_precision=precision,
+                                  taylor_hi_from=3)
     params = prop.QmcParams(dt=dt, n_walkers=W, seed=100 + i)
     pctx, mctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params), mops.build_meas_ctx(ham, tr)
     w0 = (torch.from_numpy(syn.walkers).cuda(), torch.from_numpy(syn.walkers_b).cuda())
@@ -627,7 +631,9 @@ def batch_rates(steps: int = 20) -> dict:
     walkers each, dt = 0.005, L / sqrt(N)), FP64: one fragment after another with eager launches, one HIP graph
     of per-fragment launches on 4 branches, one HIP graph of grouped launches with the whole propagation of
     every walker set in one fused launch (the default), and the same with the propagation as one-body / copy /
-    Taylor-term launches on autotuned tiles.
+    Taylor-term launches on autotuned tiles; then the same fragments in the headline's mixed form (split-BF16
+    GEMMs, Taylor terms 3-6 one bf16 pass) one after another and through the graph of per-fragment launches
+    (the grouped launches are FP64 only).
     frac = FP64 MFMA ideal of the step's GEMMs (VHS on the stored half of V, force bias, Taylor, one-body)
     over the grouped step's time."""
     import torch
@@ -638,8 +644,10 @@ def batch_rates(steps: int = 20) -> dict:
     sizes = synthetic.fragment_sizes(32, seed=4)
     out = {"workload": "32 fragments, N ~ U[40, 200], n = N/5, G = 4N, UHF, 128 walkers each", "steps": steps}
     fields = None
-    for mode in ("eager", "graph_4_branches", "grouped_graph", "grouped_graph_unfused"):
-        frags = [_fragment(i, N, n, G, W, dt, 1.0 / np.sqrt(N)) for i, (N, n, G) in enumerate(sizes)]
+    for mode in ("eager", "graph_4_branches", "grouped_graph", "grouped_graph_unfused", "mixed_eager",
+                 "mixed_graph_4_branches"):
+        prec = "bf16x3" if mode.startswith("mixed") else "f64"
+        frags = [_fragment(i, N, n, G, W, dt, 1.0 / np.sqrt(N), prec) for i, (N, n, G) in enumerate(sizes)]
         if fields is None:
             fields = [[torch.from_numpy(synthetic.fields(100 + i, s, W, f.n_fields)).cuda() for i, f in
                        enumerate(frags)] for s in range(steps + 3)]
@@ -647,7 +655,7 @@ def batch_rates(steps: int = 20) -> dict:
             # mode's Taylor launches read
             out["taylor_tiles"] = {f"{k[1]}x{k[2]}": v for k, v in
                                    sorted(batch.FragmentBatch(frags, grouped=True).autotune().items())}
-        if mode == "eager":
+        if mode in ("eager", "mixed_eager"):
             for s in range(2):
                 for i, f in enumerate(frags):
                     f.state = f.step(fields[s][i])
@@ -658,8 +666,8 @@ def batch_rates(steps: int = 20) -> dict:
                     f.state = f.step(fields[s][i])
             torch.cuda.synchronize()
         else:
-            fb = batch.FragmentBatch(frags, grouped=mode != "graph_4_branches", fused=mode == "grouped_graph")
-            if mode != "graph_4_branches":
+            fb = batch.FragmentBatch(frags, grouped=mode.startswith("grouped"), fused=mode == "grouped_graph")
+            if mode.startswith("grouped"):
                 fb.autotune()  # the grouped plan takes the tile shapes of its own batch
             flat = [fb.pack_fields([x.cpu().numpy() for x in fs]) for fs in fields]
             fb.capture(flat[0])
@@ -682,6 +690,8 @@ def batch_rates(steps: int = 20) -> dict:
     out["fp64_t_star_ms"] = round(1e3 * flop / (PEAK["f64"] * 1e12), 3)
     out["grouped_fp64_frac"] = round(out["fp64_t_star_ms"] / out["grouped_graph_ms_per_step"], 4)
     out["speedup_grouped_vs_eager"] = round(out["eager_ms_per_step"] / out["grouped_graph_ms_per_step"], 3)
+    out["mixed_speedup_vs_fp64_grouped"] = round(out["grouped_graph_ms_per_step"]
+                                                 / out["mixed_graph_4_branches_ms_per_step"], 3)
     return out
 
 

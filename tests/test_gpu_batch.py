@@ -9,6 +9,10 @@ propagation of every fragment in one fused launch or as one-body / Taylor-term l
 tile shapes, or the fragments on four parallel branches); walkers, weights, overlaps, population shifts
 and node counts must be bit-identical after steps 1, 10 and 100.
 
+Mixed precision: the same 32 fragments with split-BF16 GEMMs (Taylor terms 3-6 one bf16 pass) through the
+graph of per-fragment launches on four branches, bit-identical to the one-by-one eager mixed steps after 20
+steps.
+
 The fused propagation alone (trot_fused_prop_grouped) is also checked bit for bit against
 trot_one_body_apply, trot_taylor_apply_symblk and trot_one_body_apply on walker sets from N = 9, n = 1 to
 N = 256, n = 32 (sixteen 16-row tiles, the largest operand that fits in LDS) in one launch.
@@ -27,7 +31,7 @@ pytestmark = pytest.mark.gpu
 W, DT, STEPS = 128, 0.005, 100
 
 
-def _fragments(n_frag=32):
+def _fragments(n_frag=32, precision="f64"):
     from trot_amd import batch, meas, prop, synthetic, trial
     from trot_amd.core import HamChol, System
 
@@ -37,9 +41,13 @@ def _fragments(n_frag=32):
         sys_ = System(N, (n, n - 1), "unrestricted")
         ham = HamChol(syn.h0, syn.h1, syn.chol).to_device()
         tr = trial.UhfTrial(syn.trial, syn.trial_b)
-        mops = meas.make_uhf_meas_ops(sys_)
+        mops = meas.make_uhf_meas_ops(sys_, precision=precision)
         tops = trial.make_uhf_trial_ops(sys_)
-        pops = prop.make_prop_ops("restricted", "unrestricted")
+        if precision == "f64":
+            pops = prop.make_prop_ops("restricted", "unrestricted")
+        else:  # the headline's mixed form: split-BF16 GEMMs, Taylor terms 3-6 one bf16 pass
+            pops = prop.make_prop_ops("restricted", "unrestricted", precision=precision, taylor_precision=precision,
+                                      taylor_hi_from=3)
         params = prop.QmcParams(dt=DT, n_walkers=W, seed=100 + i)
         pctx = pops.build_prop_ctx(ham, tops.get_rdm1(tr), params)
         mctx = mops.build_meas_ctx(ham, tr)
@@ -145,3 +153,32 @@ def test_fused_propagation_bit_identical(gpu):
     for W, N, n, ref, wf in runs:
         assert torch.equal(wf, ref), (W, N, n)
         assert torch.isfinite(torch.view_as_real(wf)).all()
+
+
+def test_mixed_precision_fragments_bit_identical(gpu):
+    from trot_amd import batch, synthetic
+
+    steps = 20
+    print("[batch] split-BF16 fragments through the four-branch graph", flush=True)
+    frags = _fragments(precision="bf16x3")
+    init = [f.state for f in frags]
+    fb = batch.FragmentBatch(frags)
+    assert not fb.grouped  # the grouped launches are FP64-only; mixed fragments take the branch graph
+    fields = [[synthetic.fields(100 + i, s, W, f.n_fields) for i, f in enumerate(frags)] for s in range(steps)]
+    ref = []
+    for i, f in enumerate(frags):
+        st = f.state
+        for s in range(steps):
+            f.state = st
+            st = f.step(torch.from_numpy(fields[s][i]).cuda())
+        ref.append(_snapshot(st))
+    for f, st in zip(frags, init):
+        f.state = st
+    packed = [fb.pack_fields(fields[s]) for s in range(steps)]
+    fb.capture(packed[0])
+    for s in range(1, steps):
+        fb.step(packed[s])
+    for i, f in enumerate(frags):
+        for x, y in zip(ref[i], _snapshot(f.state)):
+            assert np.array_equal(x, y, equal_nan=True), i
+    assert all(np.all(np.isfinite(r[-4])) for r in ref)

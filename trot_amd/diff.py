@@ -92,10 +92,9 @@ def build_ctx(h0: float, h1: np.ndarray, chol: torch.Tensor, c: np.ndarray, D: n
     mctx = meas.make_rhf_meas_ops(sys_).build_meas_ctx(ham, tr)
     # h1_eff = E's generator: -2/dt log E is avoided by rebuilding it from its definition
     # (trot/prop/chol_afqmc_ops.py:90-100): h1 - 1/2 sum_g L_g L_g - sum_g Re(i mf_g) L_g
-    L = ham.chol if isinstance(ham.chol, torch.Tensor) else torch.as_tensor(ham.chol, device="cuda")
     mf = pctx.mf_shifts
-    h1_eff = torch.as_tensor(h1l, device="cuda") - 0.5 * torch.einsum("gij,gjk->ik", L, L) \
-        - torch.einsum("g,gij->ij", (1j * mf).real, L)
+    _, mfl, v0m = _lib.build_prop_tensors(ham.chol, prop._total_dm(tops.get_rdm1(tr)))
+    h1_eff = torch.as_tensor(h1l, device="cuda") - v0m - (1j * mfl).real
     A = (-0.5 * params.dt) * h1_eff.cpu().numpy()
     B = (-0.5 * params.dt) * np.asarray(D, dtype=np.float64)
     big = scipy.linalg.expm(np.block([[A, B], [np.zeros_like(A), A]]))
@@ -111,13 +110,16 @@ def build_ctx(h0: float, h1: np.ndarray, chol: torch.Tensor, c: np.ndarray, D: n
 
 
 def _green(tc: TangentCtx, w, dw):
-    """(overlap, GT, dGT, dlog overlap) for restricted walkers (overlap = det(C^T w)^2)."""
-    det, gt = _lib.overlap_green(tc.C, w)
-    m = torch.einsum("ip,wiq->wpq", tc.C.to(C128), w)
-    dm = torch.einsum("ip,wiq->wpq", tc.C.to(C128), dw)
-    minv = torch.linalg.inv(m)
-    dgt = torch.matmul(dw - torch.matmul(gt, dm), minv)
-    dlog = 2.0 * torch.diagonal(torch.matmul(minv, dm), dim1=1, dim2=2).sum(-1)
+    """(overlap, GT, dGT, dlog overlap) for restricted walkers (overlap = det(C^T w)^2), on the library's
+    batched FP64 GEMM and inverse: m = C^T w, GT = w m^-1, dGT = (dw - GT dm) m^-1, dlog = 2 tr(m^-1 dm)."""
+    m = _lib.gemm_batched(tc.C, w, opa="T")
+    dm = _lib.gemm_batched(tc.C, dw, opa="T")
+    minv, det = _lib.inverse_det(m)
+    gt = _lib.gemm_batched(w, minv)
+    x = dw.clone()
+    _lib.gemm_batched(gt, dm, alpha=-1.0, beta=1.0, out=x)
+    dgt = _lib.gemm_batched(x, minv)
+    dlog = 2.0 * (minv * dm.transpose(1, 2)).sum((1, 2))
     return det * det, gt, dgt, dlog
 
 
@@ -129,7 +131,7 @@ def _energy(tc: TangentCtx, gt, dgt):
     s = big / small if small > 0 else 1.0
     ep = _lib.local_energy(tc.rot_chol, (gt + s * dgt).contiguous(), tc.rot_h1t, tc.h0)
     em = _lib.local_energy(tc.rot_chol, (gt - s * dgt).contiguous(), tc.rot_h1t, tc.h0)
-    de = (ep - em) / (2.0 * s) + 2.0 * torch.einsum("wkp,kp->w", gt, tc.d_rot_h1t.to(C128))
+    de = (ep - em) / (2.0 * s) + 2.0 * (gt * tc.d_rot_h1t.to(C128)).sum((1, 2))
     return e, de
 
 

@@ -150,12 +150,13 @@ int trot_force_bias(const double* green_t, const double* rot_chol_t, int W, int 
   return TROT_OK;
 }
 
-// Split K for small walker counts: S slices of K, FP64 partial products in ws, summed in slice order.
-// S is a function of the shape only (same shape, same S, same rounding): enough slices for about
-// 1024 workgroups, each slice at least 256 long, at most 64 slices.
+// Split K: S slices of K, FP64 partial products in ws, summed in slice order.  S is a function of the
+// shape only (same shape, same S, same rounding): enough slices for about 2048 workgroups (8 per CU; at
+// W = 2048, G = 2560, N n = 32768 two slices of the 1280 tiles run the FP64 force bias 3.4 % faster per step
+// than none, and a 1024 target left it unsplit), each slice at least 256 long, at most 64 slices.
 static int force_bias_slices(int W, int G, long Nn) {
   const long tiles = (long)((G + 63) / 64) * ((W + 63) / 64);
-  long S = (1024 + tiles - 1) / tiles;
+  long S = (2048 + tiles - 1) / tiles;
   const long smax = Nn / 256;
   if (S > smax) S = smax;
   if (S > 64) S = 64;

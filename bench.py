@@ -626,6 +626,17 @@ def _fragment(i, N, n, G, W, dt, chol_scale, precision="f64"):
                           prop_ctx=pctx, meas_ctx=mctx, state=st)
 
 
+def _fragment_block_energy(f) -> float:
+    """Weighted mean of Re E_L over a fragment's walkers (its own energy kernels and weights)."""
+    from trot_amd.meas import _h0
+
+    eng = f.meas_ops.engine
+    _, g = eng.overlap_green(f.state.walkers, f.trial)
+    e = eng.energy_from_green(g, _h0(f.ham), f.meas_ctx).real
+    w = f.state.weights
+    return float((w * e).sum() / w.sum())
+
+
 def batch_rates(steps: int = 20) -> dict:
     """The batching form: 32 fragments (N ~ U[40, 200] seed 4, n = round(N/5), G = 4N, UHF trial (n, n-1), 128
     walkers each, dt = 0.005, L / sqrt(N)), FP64: one fragment after another with eager launches, one HIP graph
@@ -644,6 +655,7 @@ def batch_rates(steps: int = 20) -> dict:
     sizes = synthetic.fragment_sizes(32, seed=4)
     out = {"workload": "32 fragments, N ~ U[40, 200], n = N/5, G = 4N, UHF, 128 walkers each", "steps": steps}
     fields = None
+    blocks = {}
     for mode in ("eager", "graph_4_branches", "grouped_graph", "grouped_graph_unfused", "mixed_eager",
                  "mixed_graph_4_branches"):
         prec = "bf16x3" if mode.startswith("mixed") else "f64"
@@ -681,6 +693,10 @@ def batch_rates(steps: int = 20) -> dict:
         ms = 1e3 * (time.perf_counter() - t0) / steps
         out[f"{mode}_ms_per_step"] = round(ms, 3)
         log(f"batch {mode}: {ms:.2f} ms per step of all 32 fragments")
+        if mode in ("grouped_graph", "mixed_graph_4_branches"):
+            # every mode ran the same steps with the same fields from the same states: the block energy of
+            # each fragment (each form's own energy kernels, its own weights) after them
+            blocks[mode] = [_fragment_block_energy(f) for f in frags]
         del frags
         torch.cuda.empty_cache()
     symcols = lambda N: 64 * ((N + 7) // 8) * ((N + 7) // 8 + 1) // 2
@@ -692,6 +708,9 @@ def batch_rates(steps: int = 20) -> dict:
     out["speedup_grouped_vs_eager"] = round(out["eager_ms_per_step"] / out["grouped_graph_ms_per_step"], 3)
     out["mixed_speedup_vs_fp64_grouped"] = round(out["grouped_graph_ms_per_step"]
                                                  / out["mixed_graph_4_branches_ms_per_step"], 3)
+    d = [abs(m - f) * 1e3 for m, f in zip(blocks["mixed_graph_4_branches"], blocks["grouped_graph"])]
+    out["mixed_vs_fp64_block_energy_mHa"] = {"steps": steps + 2, "max_over_fragments": round(max(d), 6),
+                                             "mean_over_fragments": round(float(np.mean(d)), 6)}
     return out
 
 

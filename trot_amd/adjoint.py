@@ -64,7 +64,6 @@ class AdjCtx:
     E: torch.Tensor  # (N, N) float64, exp(-dt/2 h1_eff)
     A: np.ndarray  # -dt/2 h1_eff (host)
     C: torch.Tensor  # (N, n) float64 trial
-    Lflat: torch.Tensor  # (G, N*N) float64
     Lsym: torch.Tensor  # (G, symblk_cols(N)) float64: the stored 8 x 8 blocks of every symmetric L_g
     sym_inv: torch.Tensor  # (N*N,) long: stored column of entry (i, k) of a symmetric matrix
     sym_i1: torch.Tensor  # (cols,) long: entry (i, k) a stored column holds
@@ -118,7 +117,7 @@ def build_ctx(h0: float, h1, chol, c: np.ndarray, params: prop.QmcParams, precis
     ac = AdjCtx(N=N, n=n, G=G, dt=pctx.dt, sqrt_dt=pctx.sqrt_dt, n_terms=params.n_exp_terms, h0=float(h0),
                 h0_prop=complex(pctx.h0_prop), mf=pctx.mf_shifts, E=pctx.exp_h1_half.contiguous(),
                 A=(-0.5 * pctx.dt) * h1_eff.cpu().numpy(), C=torch.as_tensor(c, dtype=F64, device="cuda"),
-                Lflat=pctx.chol_flat, Lsym=_lib.pack_symblk(ham.chol), sym_inv=None, sym_i1=None, sym_i2=None,
+                Lsym=_lib.pack_symblk(ham.chol), sym_inv=None, sym_i1=None, sym_i2=None,
                 sym_off=None, sym_valid=None, Rt=sp.rot_chol_t.reshape(G, N * n), rot_chol=sp.rot_chol,
                 rot_h1t=sp.rot_h1t, params=params, precision=precision)
     ac.sym_inv, ac.sym_i1, ac.sym_i2, ac.sym_off, ac.sym_valid = _symblk_maps(N, ac.Lsym.device)

@@ -91,8 +91,10 @@ def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
     assert h.trot_local_energy_b3(d, None, d, None, d, None, 4, 8, 2, 0, 3, 0.0, d, d, need - 1, None) == -4
     assert h.trot_rot_chol_b3(d, 3, 65, 8, d, None) == -2
     assert h.trot_rot_chol_b3_bytes(5, 2, 8) == 2 * (8 * 64) * 128 * 2  # G padded to 4 blocks of 64 rows
-    # split-K force bias: no split (no workspace) at large W x G, slices for a small walker set
-    assert h.trot_force_bias_workspace(2048, 2560, 512 * 64) == 0
+    # split-K force bias: slices for about 2048 workgroups (two of the 1280 tiles at W = 2048, G = 2560: partial
+    # products of 2 W G complex numbers), none once the tiles alone reach that, slices for a small walker set
+    assert h.trot_force_bias_workspace(2048, 2560, 512 * 64) == 2 * 2048 * 2560 * 16
+    assert h.trot_force_bias_workspace(4096, 4096, 512 * 64) == 0
     assert h.trot_force_bias_workspace(128, 800, 200 * 40) > 0
     assert h.trot_force_bias_splitk(None, d, 4, 4, 4, 2.0, 0, d, None, 0, None) == -1
     assert h.trot_force_bias_splitk(d, d, 128, 800, 8000, 2.0, 0, d, None, 0, None) == -4
@@ -144,8 +146,9 @@ def test_error_codes_of_mixed_and_batching_entry_points_without_gpu():
     assert h.trot_copy_problem(d, d, 17, cdesc) == -1  # sizes are multiples of 16 bytes
     assert h.trot_copy_problem(d, d, 32, cdesc) == 0
     assert h.trot_copy_grouped(d, 0, 32, None) == -1
-    assert h.trot_force_bias_slices(128, 800, 8000) == 31  # ~1024 workgroups, slices of >= 256, <= 64 slices
-    assert h.trot_force_bias_slices(2048, 2560, 32768) == 1
+    assert h.trot_force_bias_slices(128, 800, 8000) == 31  # ~2048 workgroups, slices of >= 256, <= 64 slices
+    assert h.trot_force_bias_slices(2048, 2560, 32768) == 2
+    assert h.trot_force_bias_slices(4096, 4096, 32768) == 1
     assert h.trot_force_bias_slices(0, 8, 8) == -1
     gdesc = ctypes.create_string_buffer(h.trot_gemm_problem_bytes())
     assert h.trot_force_bias_problem(d, d, 128, 800, 8000, 1.0, 0, d, gdesc) == 26 * 31  # tiles x slices

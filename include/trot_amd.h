@@ -316,6 +316,49 @@ int trot_trotter_b3v(const void* v_planes, int hi_from, const void* onebody_plan
                      double* w_out, int W, int N, int n, double sqrt_dt, int n_terms, void* ws, size_t ws_bytes,
                      void* stream);
 
+/* The launches of trot_trotter_b3v as descriptors, for grouping them over many walker sets (the batching
+ * form's fragments in mixed precision).  trot_trotter_b3v_stages fills, in launch order, the kind
+ * (TROT_B3_STAGE_*), the workgroup count and the descriptor (trot_prop_b3_stage_bytes() bytes) of each launch
+ * the same call of trot_trotter_b3v would make, and returns their number (nothing is launched; ws stays in
+ * use).  trot_prop_b3_grouped runs one kind for nprob sets from a device copy of their descriptors and of the
+ * nprob + 1 prefix sums of their workgroup counts; each workgroup runs exactly the body it runs in its own
+ * set's launch, so a grouped sequence is bit-identical to trot_trotter_b3v per set.  TROT_B3_STAGE_COPY has
+ * no grouped form (TROT_ERR_UNSUPPORTED).
+ * Replaces: _apply_trotter_u per fragment (trot/prop/chol_afqmc_ops.py:211-222), the reference stepping
+ * fragments one at a time (trot/lno.py:19-141). */
+/* Grouped forms of the split-BF16 operand split (complex FP64, row-major), the wide split-BF16 GEMM of
+ * trot_gemm_b3 with complex A (its K slices included) and its slice reduction, for the batching form's force
+ * bias and VHS in mixed precision: *_problem fills one descriptor (*_problem_bytes() bytes) with the arguments
+ * of one call and returns its workgroup count (trot_b3_reduce_problem: 0 for a product without split K);
+ * *_grouped runs nprob problems from device copies of the descriptors and of the nprob + 1 prefix sums of
+ * their workgroup counts, each workgroup exactly as in its own call's launch (bit-identical).
+ * trot_gemm_b3_grouped runs problems of one output kind (c_kind as trot_gemm_b3's c_f64).
+ * Replaces: the per-fragment force bias and VHS (trot/meas/uhf.py:43-57, trot/prop/chol_afqmc_ops.py:83-87,
+ * 247-257) of fragments stepped one at a time (trot/lno.py:19-141). */
+size_t trot_split_bf16_problem_bytes(void);
+long trot_split_bf16_problem(const void* src, long ld, int rows, int K, void* dst, void* desc);
+int trot_split_bf16_grouped(const void* probs, const int* start, int nprob, int total_wg, void* stream);
+size_t trot_gemm_b3_problem_bytes(void);
+long trot_gemm_b3_problem(const void* a_planes, const void* b_planes, int M, int N, int K, void* C, int c_kind,
+                          long ldc, double alpha_re, double alpha_im, double beta, int ksplit, void* ws,
+                          size_t ws_bytes, void* desc);
+int trot_gemm_b3_grouped(int c_kind, const void* probs, const int* start, int nprob, int total_wg, void* stream);
+size_t trot_b3_reduce_problem_bytes(void);
+long trot_b3_reduce_problem(const void* gemm_desc, void* desc);
+int trot_b3_reduce_grouped(const void* probs, const int* start, int nprob, int total_wg, void* stream);
+
+#define TROT_B3_STAGE_SPLIT 0
+#define TROT_B3_STAGE_ONEBODY 1
+#define TROT_B3_STAGE_TERM_C64 2
+#define TROT_B3_STAGE_TERM3 3
+#define TROT_B3_STAGE_TERM1 4
+#define TROT_B3_STAGE_COPY 5
+size_t trot_prop_b3_stage_bytes(void);
+long trot_trotter_b3v_stages(const void* v_planes, int hi_from, const void* onebody_planes, const double* w_in,
+                             double* w_out, int W, int N, int n, double sqrt_dt, int n_terms, void* ws,
+                             size_t ws_bytes, int max_stages, int* kinds, long* workgroups, void* descs);
+int trot_prop_b3_grouped(int kind, const void* probs, const int* start, int nprob, int total_wg, void* stream);
+
 /* Split-BF16 planes of rot_chol (G, n, N) FP64 for trot_local_energy_b3, trot_rot_chol_b3_bytes(G, n, N) bytes;
  * made once per measurement context (trot/meas/rhf.py:308-319 builds rot_chol). */
 size_t trot_rot_chol_b3_bytes(int G, int n, int N);

@@ -9,9 +9,10 @@ propagation of every fragment in one fused launch or as one-body / Taylor-term l
 tile shapes, or the fragments on four parallel branches); walkers, weights, overlaps, population shifts
 and node counts must be bit-identical after steps 1, 10 and 100.
 
-Mixed precision: the same 32 fragments with split-BF16 GEMMs (Taylor terms 3-6 one bf16 pass) through the
-graph of per-fragment launches on four branches, bit-identical to the one-by-one eager mixed steps after 20
-steps.
+Mixed precision: the same 32 fragments with split-BF16 GEMMs (Taylor terms 3-6 one bf16 pass), through one
+graph of grouped launches (every launch of the split-BF16 propagation over all walker sets) and through the
+graph of per-fragment launches on four branches, each bit-identical to the one-by-one eager mixed steps after
+20 steps.
 
 The fused propagation alone (trot_fused_prop_grouped) is also checked bit for bit against
 trot_one_body_apply, trot_taylor_apply_symblk and trot_one_body_apply on walker sets from N = 9, n = 1 to
@@ -155,15 +156,16 @@ def test_fused_propagation_bit_identical(gpu):
         assert torch.isfinite(torch.view_as_real(wf)).all()
 
 
-def test_mixed_precision_fragments_bit_identical(gpu):
+@pytest.mark.parametrize("grouped", [True, False])
+def test_mixed_precision_fragments_bit_identical(gpu, grouped):
     from trot_amd import batch, synthetic
 
     steps = 20
-    print("[batch] split-BF16 fragments through the four-branch graph", flush=True)
+    print(f"[batch] split-BF16 fragments, grouped launches: {grouped}", flush=True)
     frags = _fragments(precision="bf16x3")
     init = [f.state for f in frags]
-    fb = batch.FragmentBatch(frags)
-    assert not fb.grouped  # the grouped launches are FP64-only; mixed fragments take the branch graph
+    fb = batch.FragmentBatch(frags, grouped=grouped)
+    assert fb.mixed and fb.grouped == grouped
     fields = [[synthetic.fields(100 + i, s, W, f.n_fields) for i, f in enumerate(frags)] for s in range(steps)]
     ref = []
     for i, f in enumerate(frags):

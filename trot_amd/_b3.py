@@ -67,7 +67,7 @@ def gemm_nt(a: Planes, b: Planes, out: torch.Tensor, alpha: complex = 1.0, beta:
     return out
 
 
-def gemm_nt_split(a: Planes, b: Planes) -> torch.Tensor:
+def gemm_nt_split(a: Planes, b: Planes, out: torch.Tensor | None = None) -> torch.Tensor:
     """V = a b^T for complex a, written as split complex64: an int32 (2, M, N) tensor of words
     bf16(re) | bf16(im) << 16, hi plane then lo plane (trot_gemm_b3 output kind 2)."""
     if b.complex or not a.complex:
@@ -75,7 +75,10 @@ def gemm_nt_split(a: Planes, b: Planes) -> torch.Tensor:
     if a.K != b.K:
         raise ValueError(f"gemm_nt_split: K mismatch {a.K} != {b.K}")
     M, N = a.rows, b.rows
-    out = torch.empty((2, M, N), dtype=torch.int32, device=a.data.device)
+    if out is None:
+        out = torch.empty((2, M, N), dtype=torch.int32, device=a.data.device)
+    elif out.dtype != torch.int32 or tuple(out.shape) != (2, M, N) or not out.is_contiguous():
+        raise ValueError(f"gemm_nt_split: out must be a contiguous int32 (2, {M}, {N}) tensor")
     L = lib()
     _check(L.trot_gemm_b3(_p(a.data), 1, _p(b.data), M, N, a.K, _p(out), 2, N, 1.0, 0.0, 0.0, 1, None, 0,
                           _stream()), "trot_gemm_b3")

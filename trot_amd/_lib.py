@@ -79,6 +79,18 @@ _SIGS = {
     "trot_gemm_batched": (_i, [_i, _i, _i, _i, _i, _i, _i, _d, _d, _vp, _l, _l, _vp, _l, _l, _d, _vp, _l, _l, _i,
                                _vp]),
     "trot_inverse_det": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
+    "trot_split_bf16_problem_bytes": (_sz, []),
+    "trot_split_bf16_problem": (_l, [_vp, _l, _i, _i, _vp, _vp]),
+    "trot_split_bf16_grouped": (_i, [_vp, _vp, _i, _i, _vp]),
+    "trot_gemm_b3_problem_bytes": (_sz, []),
+    "trot_gemm_b3_problem": (_l, [_vp, _vp, _i, _i, _i, _vp, _i, _l, _d, _d, _d, _i, _vp, _sz, _vp]),
+    "trot_gemm_b3_grouped": (_i, [_i, _vp, _vp, _i, _i, _vp]),
+    "trot_b3_reduce_problem_bytes": (_sz, []),
+    "trot_b3_reduce_problem": (_l, [_vp, _vp]),
+    "trot_b3_reduce_grouped": (_i, [_vp, _vp, _i, _i, _vp]),
+    "trot_prop_b3_stage_bytes": (_sz, []),
+    "trot_trotter_b3v_stages": (_l, [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _d, _i, _vp, _sz, _i, _vp, _vp, _vp]),
+    "trot_prop_b3_grouped": (_i, [_i, _vp, _vp, _i, _i, _vp]),
     "trot_fused_prop_problem_bytes": (_sz, []),
     "trot_fused_prop_problem": (_l, [_vp, _vp, _vp, _vp, _i, _i, _i, _d, _i, _vp]),
     "trot_fused_prop_grouped": (_i, [_vp, _vp, _i, _i, _l, _vp]),
@@ -985,6 +997,29 @@ def trotter_b3v(vsplit: torch.Tensor, w: torch.Tensor, onebody_planes: torch.Ten
     return out
 
 
+def trotter_b3v_stages(vsplit: torch.Tensor, w_in: torch.Tensor, onebody_planes: torch.Tensor, sqrt_dt: float,
+                       n_terms: int, hi_from: int, w_out: torch.Tensor, ws: torch.Tensor):
+    """The launches trotter_b3v would make (trot_trotter_b3v_stages): a list of (kind, workgroups, descriptor
+    bytes); ``ws`` (uint8, trot_taylor_b3_workspace bytes) must outlive their use."""
+    W, N, n = w_in.shape
+    _req(w_in, "walkers", C128, (W, N, n))
+    _req(w_out, "out", C128, (W, N, n))
+    _req(vsplit, "vhs (split planes)", torch.int32, (2, W, symblk_cols(N)))
+    L = lib()
+    if onebody_planes.dtype != torch.int16 or onebody_planes.numel() * 2 != L.trot_onebody_b3_planes_bytes(N):
+        raise ValueError("trotter_b3v_stages: onebody_planes must come from onebody_b3_planes for this N")
+    size = L.trot_prop_b3_stage_bytes()
+    mx = 4 + (W + 7) // 8 * max(int(n_terms), 1)  # split, two one-body factors, terms of chunks of >= 8 walkers
+    kinds = (ctypes.c_int * mx)()
+    wgs = (ctypes.c_long * mx)()
+    descs = ctypes.create_string_buffer(size * mx)
+    r = L.trot_trotter_b3v_stages(_p(vsplit), int(hi_from), _p(onebody_planes), _p(w_in), _p(w_out), W, N, n,
+                                  float(sqrt_dt), int(n_terms), _p(ws), ws.numel(), mx, kinds, wgs, descs)
+    _check(0 if r >= 0 else int(r), "trot_trotter_b3v_stages")
+    raw = descs.raw
+    return [(int(kinds[i]), int(wgs[i]), raw[i * size:(i + 1) * size]) for i in range(r)]
+
+
 def taylor_apply_lp(vhs32: torch.Tensor, w: torch.Tensor, sqrt_dt: float, n_terms: int, prec: str,
                     inplace: bool = False) -> torch.Tensor:
     W, N, n = w.shape
@@ -1073,6 +1108,7 @@ def taylor_apply_lp_symblk(vpk32: torch.Tensor, w: torch.Tensor, sqrt_dt: float,
 # ---------------------------------------------------------------------------
 GOP = {"one_body": 0, "taylor": 1, "overlap": 2, "green": 3, "vhs": 4, "taylor_narrow": 5, "force_bias": 6,
        "taylor_64x64": 7, "taylor_64x32": 8, "one_body_narrow": 9}
+TROT_B3_STAGE_COPY = 5  # a propagation launch kind without a grouped form (include/trot_amd.h)
 # trot_autotune_taylor's shapes (0 = 64x64, 1 = 64x32, 2 = 32x64, 3 = 32x32) as grouped Taylor operations
 TAYLOR_GOP_OF_SHAPE = {0: "taylor_64x64", 1: "taylor_64x32", 2: "taylor", 3: "taylor_narrow"}
 
@@ -1095,7 +1131,9 @@ class Grouped:
         self.size = {"gemm": L.trot_gemm_problem_bytes, "inv": L.trot_inv_problem_bytes,
                      "copy": L.trot_copy_problem_bytes, "sum": L.trot_sum_problem_bytes,
                      "shift": L.trot_shift_problem_bytes, "update": L.trot_update_problem_bytes,
-                     "fused": L.trot_fused_prop_problem_bytes}[kind]()
+                     "fused": L.trot_fused_prop_problem_bytes, "b3stage": L.trot_prop_b3_stage_bytes,
+                     "b3split": L.trot_split_bf16_problem_bytes, "b3gemm": L.trot_gemm_b3_problem_bytes,
+                     "b3reduce": L.trot_b3_reduce_problem_bytes}[kind]()
         self.max_lds = 0
         self.descs, self.counts, self.keep = [], [], []
         self.max_bytes = 0
@@ -1166,6 +1204,47 @@ class Grouped:
         self.max_lds = max(self.max_lds, fused_prop_lds(N, n))
         return True
 
+    def add_b3_stage(self, desc: bytes, workgroups: int, keep=()):
+        """One set's descriptor of a propagation stage (from trotter_b3v_stages), for kind ``self.op``."""
+        if len(desc) != self.size:
+            raise ValueError("add_b3_stage: descriptor size")
+        self.descs.append(desc)
+        self.counts.append(int(workgroups))
+        self.keep += list(keep)
+
+    def add_b3_split(self, src: torch.Tensor, dst: torch.Tensor):
+        """Split-BF16 planes of a complex128 row-major matrix (trot_split_bf16_problem) into ``dst`` (uint8)."""
+        rows, K = src.shape
+        _req(src, "src", C128, (rows, K))
+        if dst.dtype != torch.uint8 or dst.numel() < lib().trot_b3_planes_bytes(rows, K, 1):
+            raise ValueError("add_b3_split: dst too small")
+        buf = ctypes.create_string_buffer(self.size)
+        r = lib().trot_split_bf16_problem(_p(src), K, rows, K, _p(dst), buf)
+        _check(0 if r > 0 else int(r), "trot_split_bf16_problem")
+        self._add(buf, r, (src, dst))
+
+    def add_b3_gemm(self, a_data: torch.Tensor, b_data: torch.Tensor, M: int, N: int, K: int, out: torch.Tensor,
+                    ldc: int, alpha: complex, beta: float, ksplit: int, ws: torch.Tensor | None):
+        """One complex-A product of trot_gemm_b3 (kind ``self.op``); returns its descriptor for a reduction."""
+        buf = ctypes.create_string_buffer(self.size)
+        al = complex(alpha)
+        r = lib().trot_gemm_b3_problem(_p(a_data), _p(b_data), int(M), int(N), int(K), _p(out), int(self.op), int(ldc),
+                                       al.real, al.imag, float(beta), int(ksplit), _p(ws),
+                                       0 if ws is None else ws.numel(), buf)
+        _check(0 if r > 0 else int(r), "trot_gemm_b3_problem")
+        self._add(buf, r, (a_data, b_data, out) + (() if ws is None else (ws,)))
+        return buf.raw
+
+    def add_b3_reduce(self, gemm_desc: bytes) -> bool:
+        """The slice reduction of a split-K product from add_b3_gemm; False when it has no split K."""
+        buf = ctypes.create_string_buffer(self.size)
+        r = lib().trot_b3_reduce_problem(gemm_desc, buf)
+        _check(0 if r >= 0 else int(r), "trot_b3_reduce_problem")
+        if r == 0:
+            return False
+        self._add(buf, r, ())
+        return True
+
     def add_copy(self, src: torch.Tensor, dst: torch.Tensor):
         if src.numel() != dst.numel() or src.dtype != dst.dtype or not (src.is_contiguous() and dst.is_contiguous()):
             raise ValueError("copy: matching contiguous tensors needed")
@@ -1205,6 +1284,18 @@ class Grouped:
                    "trot_shift_grouped")
         elif self.kind == "update":
             _check(L.trot_update_grouped(_p(self.probs), n, _stream()), "trot_update_grouped")
+        elif self.kind == "b3split":
+            _check(L.trot_split_bf16_grouped(_p(self.probs), _p(self.start), n, self.total, _stream()),
+                   "trot_split_bf16_grouped")
+        elif self.kind == "b3gemm":
+            _check(L.trot_gemm_b3_grouped(int(self.op), _p(self.probs), _p(self.start), n, self.total, _stream()),
+                   "trot_gemm_b3_grouped")
+        elif self.kind == "b3reduce":
+            _check(L.trot_b3_reduce_grouped(_p(self.probs), _p(self.start), n, self.total, _stream()),
+                   "trot_b3_reduce_grouped")
+        elif self.kind == "b3stage":
+            _check(L.trot_prop_b3_grouped(int(self.op), _p(self.probs), _p(self.start), n, self.total, _stream()),
+                   "trot_prop_b3_grouped")
         elif self.kind == "fused":
             _check(L.trot_fused_prop_grouped(_p(self.probs), _p(self.start), n, self.total, self.max_lds, _stream()),
                    "trot_fused_prop_grouped")

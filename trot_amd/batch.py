@@ -97,6 +97,17 @@ def _groupable(f: Fragment) -> bool:
             and f.sys.walker_kind in ("restricted", "unrestricted") and max(f.sys.nelec) <= _lib.MAX_NOCC)
 
 
+def _groupable_mixed(f: Fragment) -> bool:
+    """The mixed step the grouped propagation covers: split-BF16 force bias, VHS written as split planes on
+    sym-block V and the whole propagation on split-BF16 MFMA (trotter_b3v), all walkers in one chunk."""
+    ctx = f.prop_ctx
+    eng = f.meas_ops.engine
+    return (ctx.precision == "bf16x3" and ctx.taylor_precision == "bf16x3" and bool(ctx.low.get("split_v"))
+            and ctx.walker_chunk <= 0 and eng.precision == "bf16x3"
+            and getattr(eng, "kind", "") in ("rr", "ru", "uu")
+            and f.sys.walker_kind in ("restricted", "unrestricted") and max(f.sys.nelec) <= _lib.MAX_NOCC)
+
+
 class _GroupedPlan:
     """Static buffers of every fragment and the grouped launches of one step (FP64, sym-block V).
 
@@ -226,6 +237,123 @@ class _GroupedPlan:
             g.launch()
 
 
+class _GroupedPlanMixed:
+    """The grouped step for fragments in the split-BF16 mixed form (``_groupable_mixed``).
+
+    One launch each over all fragments and spins: the split of the Green's functions into bf16 planes, the
+    split-BF16 force-bias GEMM (its K slices) and the slice sums of spin a then spin b, the shifted fields,
+    the split of the shifted fields, the VHS GEMM writing V as split planes, every launch of the split-BF16
+    propagation (trot_trotter_b3v_stages / trot_prop_b3_grouped: the split of the walkers, the two one-body
+    products and each Taylor term, walkers updated in place), C^T w, inverse and determinant, Green's
+    functions, and the weight update (the FP64 parts exactly as _GroupedPlan runs them).  Each walker set runs
+    the kernels and operands of its own step, so the results are bit-identical to stepping the fragments one
+    at a time."""
+
+    def __init__(self, frags: List[Fragment], views):
+        from . import _b3
+        from .trial import _coeffs
+
+        n_terms = {int(f.params.n_exp_terms) for f in frags}
+        if len(n_terms) != 1:
+            raise ValueError("grouped step: fragments must share n_exp_terms")
+        K = n_terms.pop()
+        dev = "cuda"
+        L = _lib.lib()
+        self.per = []
+        split_g, gemm_fb = _lib.Grouped("b3split"), _lib.Grouped("b3gemm", 1)
+        red_a, red_b = _lib.Grouped("b3reduce"), _lib.Grouped("b3reduce")
+        split_x, gemm_v = _lib.Grouped("b3split"), _lib.Grouped("b3gemm", 2)
+        shf, upd = _lib.Grouped("shift"), _lib.Grouped("update")
+        ovl, inv, grn = _lib.Grouped("gemm", "overlap"), _lib.Grouped("inv"), _lib.Grouped("gemm", "green")
+        stage_seq, stage_groups = None, []
+        for fi, f in enumerate(frags):
+            ctx, st = f.prop_ctx, f.state
+            W, G, N = f.n_walkers, f.n_fields, ctx.norb
+            planes = ctx.low["chol_b3"]
+            b = dict(xbar=torch.empty((W, G), dtype=torch.complex128, device=dev),
+                     shift=torch.empty((W,), dtype=torch.complex128, device=dev),
+                     fbt=torch.empty((W,), dtype=torch.complex128, device=dev),
+                     fb=torch.empty((W, G), dtype=torch.complex128, device=dev),
+                     v=torch.empty((2, W, planes.rows), dtype=torch.int32, device=dev), det=[], spins=[], ws=[])
+            walkers = st.walkers if isinstance(st.walkers, tuple) else (st.walkers,)
+            greens = [g for g in st.green if g is not None]
+            # the force bias as the split-BF16 engine runs it (trot_amd.meas, force_bias_from_green): per spin,
+            # the Green's function split into planes and an NT GEMM with 8 K slices; spin a's slice sums write
+            # alpha * fb, spin b's add to it
+            eng = f.meas_ops.engine
+            sa, sb = eng.spins(f.meas_ctx)
+            spin_fb = [(greens[0], sa, 2.0 if eng.kind == "rr" else 1.0, 0.0)]
+            if eng.kind != "rr":
+                spin_fb.append((greens[1], sb, 1.0, 1.0))
+            for si, (gt, s, al, be) in enumerate(spin_fb):
+                Nn = gt.shape[1] * gt.shape[2]
+                if "rot_chol_t_b3" not in s.low:
+                    s.low["rot_chol_t_b3"] = _b3.split(s.rot_chol_t.reshape(G, -1))
+                rcb = s.low["rot_chol_t_b3"]
+                gp = torch.empty(L.trot_b3_planes_bytes(W, Nn, 1), dtype=torch.uint8, device=dev)
+                split_g.add_b3_split(gt.view(W, Nn), gp)
+                ws = torch.empty(max(int(L.trot_gemm_b3_workspace(W, G, 1, 8)), 1), dtype=torch.uint8, device=dev)
+                desc = gemm_fb.add_b3_gemm(gp, rcb.data, W, G, Nn, b["fb"], G, al, be, 8, ws)
+                if not (red_a if si == 0 else red_b).add_b3_reduce(desc):
+                    raise ValueError("grouped mixed step: a force bias without split K would race between spins")
+                b["ws"] += [gp, ws]
+            shf.add_shift(views[fi], b["fb"], ctx.mf_shifts, ctx.sqrt_dt, W, G, b["xbar"], b["shift"], b["fbt"])
+            xp = torch.empty(L.trot_b3_planes_bytes(W, G, 1), dtype=torch.uint8, device=dev)
+            split_x.add_b3_split(b["xbar"], xp)
+            gemm_v.add_b3_gemm(xp, planes.data, W, planes.rows, G, b["v"], planes.rows, 1.0, 0.0, 1, None)
+            b["ws"].append(xp)
+            coeffs = _coeffs(f.trial)
+            hi_from = ctx.taylor_hi_from if ctx.taylor_hi_from > 0 else K + 1
+            for si, w in enumerate(walkers):
+                n = w.shape[2]
+                c = coeffs[si].contiguous()
+                m = torch.empty((W, n, n), dtype=torch.complex128, device=dev)
+                minv = torch.empty_like(m)
+                det = torch.empty((W,), dtype=torch.complex128, device=dev)
+                ws = torch.empty(L.trot_taylor_b3_workspace(W, N, n), dtype=torch.uint8, device=dev)
+                stages = _lib.trotter_b3v_stages(b["v"], w, ctx.low["onebody_b3"], ctx.sqrt_dt, K, hi_from, w, ws)
+                seq = [k for k, _, _ in stages]
+                if stage_seq is None:
+                    stage_seq = seq
+                    stage_groups = [_lib.Grouped("b3stage", k) for k in seq]
+                elif seq != stage_seq:
+                    raise ValueError("grouped mixed step: walker sets need the same propagation launches "
+                                     "(one walker chunk each)")
+                for g, (_, wg, desc) in zip(stage_groups, stages):
+                    g.add_b3_stage(desc, wg, (b["v"], w, ws, ctx.low["onebody_b3"]))
+                ovl.add_gemm(c, w, m, None, W, N, n)
+                inv.add_inv(m, minv, det, W, n)
+                grn.add_gemm(w, minv, greens[si], None, W, N, n)
+                b["det"].append(det)
+                b["spins"].append((m, minv, c))
+                b["ws"].append(ws)
+            p = f.params
+            upd.add_update(b["det"][0], b["det"][1] if len(b["det"]) > 1 else None, st.overlaps, b["shift"], b["fbt"],
+                           st.weights, f._scal, st.node_encounters, W, ctx.dt, ctx.h0_prop,
+                           getattr(p, "weight_floor", 1.0e-3), getattr(p, "weight_cap", 100.0),
+                           getattr(p, "pop_control_damping", 0.1))
+            self.per.append(b)
+        if stage_seq is not None and _lib.TROT_B3_STAGE_COPY in stage_seq:
+            raise ValueError("grouped mixed step: the propagation needs its one-body factors")
+        self.pre = [g.finish() for g in (split_g, gemm_fb, red_a, red_b) if g.descs] + [shf.finish()]
+        self.ops = ([split_x.finish(), gemm_v.finish()] + [g.finish() for g in stage_groups]
+                    + [g.finish() for g in (ovl, inv, grn)])
+        self.post = [upd.finish()]
+        self.n_fused = 0
+
+    def launch_pre(self):
+        for g in self.pre:
+            g.launch()
+
+    def launch(self):
+        for g in self.ops:
+            g.launch()
+
+    def launch_post(self):
+        for g in self.post:
+            g.launch()
+
+
 class FragmentBatch:
     """A list of fragments stepped together through one captured HIP graph.
 
@@ -266,9 +394,11 @@ class FragmentBatch:
         self.fields = torch.zeros(self.offsets[-1], dtype=torch.float64, device="cuda")
         self.graph = None
         self.steps_done = 0
-        can = all(_groupable(f) for f in fragments)
+        self.mixed = all(_groupable_mixed(f) for f in fragments)
+        can = self.mixed or all(_groupable(f) for f in fragments)
         if grouped and not can:
-            raise ValueError("grouped step: every fragment must run the FP64 step on sym-block V")
+            raise ValueError("grouped step: every fragment must run the FP64 step on sym-block V, or every "
+                             "fragment the split-BF16 step on split V planes")
         self.grouped = can if grouped is None else bool(grouped)
         self.plan = None
         self.tiles = {}
@@ -319,7 +449,10 @@ class FragmentBatch:
                 torch.float64).contiguous()
             f.state = f.state._replace(pop_control_ene_shift=scal[0], e_estimate=scal[1])
             f._scal = scal
-        self.plan = _GroupedPlan(self.fragments, views, self.tiles, self.fused)
+        if self.mixed:
+            self.plan = _GroupedPlanMixed(self.fragments, views)
+        else:
+            self.plan = _GroupedPlan(self.fragments, views, self.tiles, self.fused)
         for f, b in zip(self.fragments, self.plan.per):
             b["scal"] = f._scal
 

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(512, 1) void exchange_b3_kernel(B3P p, int n_mt, in
   int mt, nt;
   if (!b3_tile_of(blockIdx.x, n_mt, n_nt, 8, mt, nt)) return;
   f4_t acc[2][4][4];
-  b3w_mainloop(p, mt * B3W_BM, nt * B3W_BN, b_rows, smem, acc);  // ends on a barrier: smem is free
+  b3w_mainloop(p, mt * B3W_BM, nt * B3W_BN, b_rows, smem, acc, blockIdx.y);  // ends on a barrier: smem is free
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int w = mt * 2 + (wave >> 2), g = nt * 4 + (wave & 3);

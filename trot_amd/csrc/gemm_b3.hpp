@@ -60,9 +60,11 @@ __device__ __forceinline__ void split_bf16_raw(float x, bf16raw& hi, bf16raw& lo
 }
 
 // Epilogue of a wave's 64 x 64 tile at (mrow, ncol): D[i = 4 (l / 16) + r][j = l % 16] in register r.
+// slice: the K slice of a split-K product (its partial sums go to part[slice]); -1 takes blockIdx.y.
 template <bool AC, typename TC>
 __device__ __forceinline__ void b3_epilogue(const B3P& p, const f4_t (&acc)[AC ? 2 : 1][4][4], int mrow, int ncol,
-                                            int lane) {
+                                            int lane, int slice = -1) {
+  const long sl = slice < 0 ? (long)blockIdx.y : (long)slice;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -73,7 +75,7 @@ __device__ __forceinline__ void b3_epilogue(const B3P& p, const f4_t (&acc)[AC ?
         const int gj = ncol + j * 16 + (lane & 15);
         if (gi >= p.M || gj >= p.N) continue;
         if (p.part) {  // split K: raw partial sums, reduced by b3_reduce_kernel
-          const long po = ((long)blockIdx.y * p.M + gi) * p.N + gj;
+          const long po = (sl * p.M + gi) * p.N + gj;
           if constexpr (AC) {
             reinterpret_cast<float2*>(p.part)[po] = make_float2(acc[0][i][j][r], acc[1][i][j][r]);
           } else {
@@ -242,7 +244,7 @@ __device__ __forceinline__ bool b3_tile_of(int b, int n_mt, int n_nt, int GM, in
 // Main loop of the wide form: acc[q][i][j] (q = re, im of A) for the wave's 64 x 64 sub-tile
 // (rows wm * 64 .., columns wn * 64 ..; wm = wave / 4, wn = wave % 4) of the tile at (m0, n0).
 __device__ __forceinline__ void b3w_mainloop(const B3P& p, int m0, int n0, int b_rows, char* smem,
-                                             f4_t (&acc)[2][4][4]) {
+                                             f4_t (&acc)[2][4][4], int slice) {
   constexpr int BM = B3W_BM, BN = B3W_BN, BK = B3W_BK;
   typedef bf16raw AsT[4][BM * BK];
   typedef bf16raw BsT[2][BN * BK];
@@ -286,7 +288,7 @@ __device__ __forceinline__ void b3w_mainloop(const B3P& p, int m0, int n0, int b
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[q][i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
-  const int kbeg = blockIdx.y * p.k_slice;
+  const int kbeg = slice * p.k_slice;
   const int nk = min(p.k_slice, p.Kp - kbeg) / BK;
   const int seg = lane >> 4;
   if (nk > 0) {
@@ -347,15 +349,45 @@ __global__ __launch_bounds__(512, 1) void gemm_b3w_kernel(B3P p, int n_mt, int n
   if (!b3_tile_of(blockIdx.x, n_mt, n_nt, 8, mt, nt)) return;
   const int m0 = mt * B3W_BM, n0 = nt * B3W_BN;
   f4_t acc[2][4][4];
-  b3w_mainloop(p, m0, n0, b_rows, smem, acc);
+  b3w_mainloop(p, m0, n0, b_rows, smem, acc, blockIdx.y);
   const int wave = threadIdx.x >> 6;
   b3_epilogue<true, TC>(p, acc, m0 + (wave >> 2) * 64, n0 + (wave & 3) * 64, threadIdx.x & 63);
 }
 
+// The wide GEMMs of many problems in one launch (the batching form's force bias and VHS in mixed precision):
+// problem i owns workgroups start[i] .. start[i+1] - 1, S slices of its tiles_pad (a multiple of 8) tile
+// workgroups, and runs exactly the workgroup body of its own launch (same tile order, same K slices).
+struct B3GProb {
+  B3P p;
+  int n_mt, n_nt, b_rows, S, tiles_pad, c_kind;
+  double ar, ai, beta;  // FP64 alpha and beta of the slice reduction (split K)
+};
+
+template <typename TC>
+__global__ __launch_bounds__(512, 1) void gemm_b3w_grouped_kernel(const B3GProb* __restrict__ probs,
+                                                                  const int* __restrict__ start, int nprob) {
+  __shared__ __attribute__((aligned(16))) char smem[B3W_SMEM];
+  int lo = 0, hi = nprob;  // start[lo] <= g < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= (int)blockIdx.x) lo = mid; else hi = mid;
+  }
+  const B3GProb q = probs[lo];
+  const int local = blockIdx.x - start[lo];
+  const int slice = local / q.tiles_pad, tb = local - slice * q.tiles_pad;
+  int mt, nt;
+  if (!b3_tile_of(tb, q.n_mt, q.n_nt, 8, mt, nt)) return;
+  const int m0 = mt * B3W_BM, n0 = nt * B3W_BN;
+  f4_t acc[2][4][4];
+  b3w_mainloop(q.p, m0, n0, q.b_rows, smem, acc, slice);
+  const int wave = threadIdx.x >> 6;
+  b3_epilogue<true, TC>(q.p, acc, m0 + (wave >> 2) * 64, n0 + (wave & 3) * 64, threadIdx.x & 63, slice);
+}
+
 // C (complex128, M x N, ldc) = alpha * sum_s part[s] (+ beta C), slices summed in order in FP64.
-static __global__ void b3_reduce_c128_kernel(const float* __restrict__ part, int S, int M, int N, double* __restrict__ C,
-                                      long ldc, double ar, double ai, double beta) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b3_reduce_c128_body(const float* __restrict__ part, int S, int M, int N,
+                                                    double* __restrict__ C, long ldc, double ar, double ai, double beta,
+                                                    long e) {
   if (e >= (long)M * N) return;
   const int i = (int)(e / N), j = (int)(e - (e / N) * N);
   double sr = 0.0, si = 0.0;
@@ -369,13 +401,17 @@ static __global__ void b3_reduce_c128_kernel(const float* __restrict__ part, int
   *c = make_double2(ar * sr - ai * si + beta * o.x, ar * si + ai * sr + beta * o.y);
 }
 
+static __global__ void b3_reduce_c128_kernel(const float* __restrict__ part, int S, int M, int N, double* __restrict__ C,
+                                             long ldc, double ar, double ai, double beta) {
+  b3_reduce_c128_body(part, S, M, N, C, ldc, ar, ai, beta, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Split a row-major FP64 matrix (rows x K, leading dim ld; complex if CX) into bf16 planes of
 // (rows_pad, Kp), zero padded.  Planes: [re_hi, re_lo, im_hi, im_lo] or [hi, lo].
 // TRANS: element (r, k) of the source sits at k * ld + r (column-major source).
 template <bool CX, bool TRANS, typename TS>
-__global__ void split_planes_kernel(const TS* __restrict__ src, long ld, int rows, int K, int rows_pad, int Kp,
-                                    bf16raw* __restrict__ dst) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void split_planes_body(const TS* __restrict__ src, long ld, int rows, int K, int rows_pad,
+                                                  int Kp, bf16raw* __restrict__ dst, long e) {
   const long total = (long)rows_pad * Kp;
   if (e >= total) return;
   const int r = (int)(e / Kp), k = (int)(e - (e / Kp) * Kp);
@@ -399,4 +435,10 @@ __global__ void split_planes_kernel(const TS* __restrict__ src, long ld, int row
     dst[2 * total + e] = h;
     dst[3 * total + e] = l;
   }
+}
+
+template <bool CX, bool TRANS, typename TS>
+__global__ void split_planes_kernel(const TS* __restrict__ src, long ld, int rows, int K, int rows_pad, int Kp,
+                                    bf16raw* __restrict__ dst) {
+  split_planes_body<CX, TRANS, TS>(src, ld, rows, K, rows_pad, Kp, dst, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }

@@ -24,6 +24,8 @@
 // its T planes and the mirrored V blocks in that XCD's L2.
 // Complex x complex = 4 real products x 3 passes: re += Ar.Br + Ai.(-Bi), im += Ar.Bi + Ai.Br
 // (the -Bi fragment is a sign flip of its bf16 words); real E x complex = 2 x 3 passes.
+#include <vector>
+
 #include "capi_util.hpp"
 #include "common.hpp"
 #include "gemm_b3.hpp"
@@ -60,8 +62,8 @@ __device__ __forceinline__ unsigned pack2(bf16raw a, bf16raw b) { return (unsign
 // REAL_A: product (E - I) . planes (one body); else V . planes (one Taylor term).
 // VM (Taylor only): 0 V as complex64, split here; 1 V as split planes (hi, lo), three passes; 2 the hi
 // planes of V and of the term only, one pass (bf16 products for the late, small terms).
-template <bool REAL_A, int VM = 0>
-__global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
+template <bool REAL_A, int VM>
+__device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
   constexpr int BM = PB_BM, BN = PB_BN, BK = PB_BK;
   constexpr int PA = REAL_A ? 2 : 4;
   constexpr bool HIONLY = !REAL_A && VM == 2;  // lo planes neither loaded nor multiplied
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
 
   const int n_rt = p.Np / BM;
   const int tpw = n_rt * (p.np_ / BN);
-  const int b = blockIdx.x;
+  // b: the workgroup's index within this problem's grid
   const int grp = b >> 3;
   const int tile = grp % tpw;
   const long walker = (long)(grp / tpw) * 8 + (b & 7);
@@ -320,13 +322,38 @@ __global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
     }
 }
 
+template <bool REAL_A, int VM = 0>
+__global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
+  prop_b3_body<REAL_A, VM>(p, blockIdx.x);
+}
+
+// The propagation kernels of many walker sets in one launch: problem i owns workgroups start[i] ..
+// start[i+1] - 1 (a multiple of 8 each, so a workgroup's XCD is that of its place in its own grid) and
+// runs exactly the workgroup body of its own launch.
+__device__ __forceinline__ int b3_problem_of(const int* __restrict__ start, int nprob, int g) {
+  int lo = 0, hi = nprob;  // start[lo] <= g < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= g) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <bool REAL_A, int VM>
+__global__ __launch_bounds__(256, 2) void prop_b3_grouped_kernel(const PropB3P* __restrict__ probs,
+                                                                const int* __restrict__ start, int nprob) {
+  const int pi = b3_problem_of(start, nprob, blockIdx.x);
+  const PropB3P p = probs[pi];
+  prop_b3_body<REAL_A, VM>(p, blockIdx.x - start[pi]);
+}
+
 // Transposed planes (W, 4, np, Np) of walkers w (W, N, n) complex128, zero padded.  Grid (Np/64, np/64, W),
 // 256 threads; a 64 x 64 tile goes through LDS so that reads run along p and writes along i.
-__global__ __launch_bounds__(256) void split_transpose_w_kernel(const double* __restrict__ w, bf16raw* __restrict__ tp,
-                                                                int N, int n, int Np, int np_) {
+__device__ __forceinline__ void split_transpose_w_body(const double* __restrict__ w, bf16raw* __restrict__ tp, int N,
+                                                       int n, int Np, int np_, int bx, int by, long bz) {
   __shared__ double2 tile[64][65];
-  const int i0 = blockIdx.x * 64, p0 = blockIdx.y * 64;
-  const long b = blockIdx.z;
+  const int i0 = bx * 64, p0 = by * 64;
+  const long b = bz;
   const int t = threadIdx.x;
   const double2* wb = reinterpret_cast<const double2*>(w) + b * (long)N * n;
   for (int e = t; e < 64 * 64; e += 256) {
@@ -349,6 +376,22 @@ __global__ __launch_bounds__(256) void split_transpose_w_kernel(const double* __
     base[2 * plane + o] = h;
     base[3 * plane + o] = l;
   }
+}
+
+__global__ __launch_bounds__(256) void split_transpose_w_kernel(const double* __restrict__ w, bf16raw* __restrict__ tp,
+                                                                int N, int n, int Np, int np_) {
+  split_transpose_w_body(w, tp, N, n, Np, np_, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Grouped: problem i (a PropB3P with win = walkers, tn = planes, W, N, n, Np, np_) owns workgroups
+// start[i] .. start[i+1] - 1, laid out as its own grid (x fastest, then y, then z = walker).
+__global__ __launch_bounds__(256) void split_transpose_grouped_kernel(const PropB3P* __restrict__ probs,
+                                                                      const int* __restrict__ start, int nprob) {
+  const int pi = b3_problem_of(start, nprob, blockIdx.x);
+  const PropB3P p = probs[pi];
+  const int local = blockIdx.x - start[pi];
+  const int nx = p.Np / 64, ny = p.np_ / 64;
+  split_transpose_w_body(p.win, p.tn, p.N, p.n, p.Np, p.np_, local % nx, (local / nx) % ny, local / (nx * ny));
 }
 
 // (E - I) split into planes [hi, lo] of (Np, Np), zero padded.
@@ -395,7 +438,23 @@ struct VSrc {
   int hi_from;  // terms k >= hi_from: one bf16 pass on the hi planes of V and of the term
 };
 
-// w_out <- [E .] T_K(i sqrt(dt) V) [. E] w_in; one-body factors when ep is non-null.
+// One launch of the propagation: its kind and parameters.  SPLIT: walkers win -> planes tn; ONEBODY: the
+// (E - I) product; TERM_C64 / TERM3 / TERM1: a Taylor term with V complex64, split V three passes, split V
+// one pass; COPY: win -> w (no one-body factors and no terms).
+enum { B3_SPLIT = 0, B3_ONEBODY = 1, B3_TERM_C64 = 2, B3_TERM3 = 3, B3_TERM1 = 4, B3_COPY = 5 };
+struct B3Stage {
+  int kind;
+  PropB3P P;
+};
+
+long b3_stage_workgroups(const B3Stage& st) {
+  const PropB3P& P = st.P;
+  if (st.kind == B3_SPLIT) return (long)(P.Np / 64) * (P.np_ / 64) * P.W;
+  if (st.kind == B3_COPY) return 0;
+  return (long)((P.W + 7) / 8) * 8 * (P.Np / PB_BM) * (P.np_ / PB_BN);
+}
+
+// w_out <- [E .] T_K(i sqrt(dt) V) [. E] w_in; one-body factors when ep is non-null, as a list of launches.
 //
 // The split of w_in into planes and both one-body factors run once over all walkers (E is shared, so
 // these launches are large and short-lived per walker); the Taylor series runs over walker chunks whose
@@ -403,41 +462,52 @@ struct VSrc {
 // Infinity Cache across the whole series, while each launch still has at least 512 workgroups (two per
 // CU).  Measured at N = 512, n = 64, W = 2048 with the one-body factors inside the chunks: 128 walkers per
 // chunk 10.5 ms, all at once 11.9 ms, 64 per chunk 14.3 ms (fewer workgroups than CUs can hold).
-int run_b3(const VSrc& v, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
-           double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, hipStream_t s) {
+int b3_stages(const VSrc& v, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
+              double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, std::vector<B3Stage>& out) {
   const int Np = pad_up(N, PB_BM), np_ = pad_up(n, PB_BN), nb = (N + 7) / 8;
   Carve cv{(char*)ws, ws_bytes, 0};
   bf16raw* bufs[2] = {cv.take<bf16raw>((size_t)W * 4 * np_ * Np), cv.take<bf16raw>((size_t)W * 4 * np_ * Np)};
   float* S = cv.take<float>((size_t)W * Np * np_ * 2);
   if (!bufs[0] || !bufs[1] || !S) return fail(TROT_ERR_WORKSPACE, "propagation b3: workspace");
-  if (!ep && n_terms == 0) {
-    if (w_out != w_in) HIPCHK(hipMemcpyAsync(w_out, w_in, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
-    return TROT_OK;
-  }
-  hipLaunchKernelGGL(split_transpose_w_kernel, dim3(Np / 64, np_ / 64, W), dim3(256), 0, s, w_in, bufs[0], N, n, Np,
-                     np_);
-  HIPCHK(hipGetLastError());
   const long vcols = symblk_count(nb) * 64;
   PropB3P base{};
   base.vcols = vcols;
   base.ep = ep;
+  base.W = W;
   base.N = N;
   base.n = n;
   base.Np = Np;
   base.np_ = np_;
   base.nb = nb;
+  if (!ep && n_terms == 0) {
+    if (w_out != w_in) {
+      B3Stage st{B3_COPY, base};
+      st.P.win = w_in;
+      st.P.w = w_out;
+      out.push_back(st);
+    }
+    return TROT_OK;
+  }
+  {
+    B3Stage st{B3_SPLIT, base};
+    st.P.win = w_in;
+    st.P.tn = bufs[0];
+    out.push_back(st);
+  }
   int cur = 0;
   if (ep) {  // first one-body factor, all walkers: w_out = w_in + (E - I) w_in, with its planes
-    PropB3P P = base;
-    P.W = W;
-    P.tp = bufs[0];
-    P.tn = bufs[1];
-    P.win = w_in;
-    P.w = w_out;
-    if (int rc = launch_prop<true>(P, s)) return rc;
+    B3Stage st{B3_ONEBODY, base};
+    st.P.tp = bufs[0];
+    st.P.tn = bufs[1];
+    st.P.win = w_in;
+    st.P.w = w_out;
+    out.push_back(st);
     cur = 1;
   } else if (w_out != w_in) {
-    HIPCHK(hipMemcpyAsync(w_out, w_in, (size_t)W * N * n * 16, hipMemcpyDeviceToDevice, s));
+    B3Stage st{B3_COPY, base};
+    st.P.win = w_in;
+    st.P.w = w_out;
+    out.push_back(st);
   }
   const double per_walker = 8.0 * vcols + 2.0 * 4 * np_ * Np * 2 + 8.0 * Np * np_;
   const int tpw = (Np / PB_BM) * (np_ / PB_BN);
@@ -466,22 +536,46 @@ int run_b3(const VSrc& v, const bf16raw* ep, const double* w_in, double* w_out, 
       // the next term, when it is one bf16 pass on the hi planes, never reads the lo planes of this output
       P.tn_hi = !v.c64 && k < n_terms && k + 1 >= v.hi_from;
       P.c = (float)(sqrt_dt / k);
-      const int rc = v.c64 ? launch_prop<false, 0>(P, s)
-                           : (k >= v.hi_from ? launch_prop<false, 2>(P, s) : launch_prop<false, 1>(P, s));
-      if (rc) return rc;
+      out.push_back(B3Stage{v.c64 ? B3_TERM_C64 : (k >= v.hi_from ? B3_TERM1 : B3_TERM3), P});
       c ^= 1;
     }
     cur_end = c;
   }
   if (ep) {  // second one-body factor, all walkers, on the planes of the propagated walkers
-    PropB3P P = base;
-    P.W = W;
-    P.tp = bufs[cur_end];
-    P.tn = nullptr;
-    P.win = w_out;
-    P.w = w_out;
-    if (int rc = launch_prop<true>(P, s)) return rc;
+    B3Stage st{B3_ONEBODY, base};
+    st.P.tp = bufs[cur_end];
+    st.P.tn = nullptr;
+    st.P.win = w_out;
+    st.P.w = w_out;
+    out.push_back(st);
   }
+  return TROT_OK;
+}
+
+int launch_stage(const B3Stage& st, hipStream_t s) {
+  const PropB3P& P = st.P;
+  switch (st.kind) {
+    case B3_SPLIT:
+      hipLaunchKernelGGL(split_transpose_w_kernel, dim3(P.Np / 64, P.np_ / 64, P.W), dim3(256), 0, s, P.win, P.tn, P.N,
+                         P.n, P.Np, P.np_);
+      HIPCHK(hipGetLastError());
+      return TROT_OK;
+    case B3_ONEBODY: return launch_prop<true>(P, s);
+    case B3_TERM_C64: return launch_prop<false, 0>(P, s);
+    case B3_TERM3: return launch_prop<false, 1>(P, s);
+    case B3_TERM1: return launch_prop<false, 2>(P, s);
+    default:
+      HIPCHK(hipMemcpyAsync(P.w, P.win, (size_t)P.W * P.N * P.n * 16, hipMemcpyDeviceToDevice, s));
+      return TROT_OK;
+  }
+}
+
+int run_b3(const VSrc& v, const bf16raw* ep, const double* w_in, double* w_out, int W, int N, int n,
+           double sqrt_dt, int n_terms, void* ws, size_t ws_bytes, hipStream_t s) {
+  std::vector<B3Stage> stages;
+  if (int rc = b3_stages(v, ep, w_in, w_out, W, N, n, sqrt_dt, n_terms, ws, ws_bytes, stages)) return rc;
+  for (const B3Stage& st : stages)
+    if (int rc = launch_stage(st, s)) return rc;
   return TROT_OK;
 }
 
@@ -549,6 +643,65 @@ int trot_trotter_b3v(const void* v_planes, int hi_from, const void* onebody_plan
   const unsigned* vh = (const unsigned*)v_planes;
   return run_b3(VSrc{nullptr, vh, vh + (size_t)W * vcols, hi_from}, (const bf16raw*)onebody_planes, w_in, w_out, W,
                 N, n, sqrt_dt, n_terms, ws, ws_bytes, (hipStream_t)stream);
+}
+
+
+size_t trot_prop_b3_stage_bytes(void) { return sizeof(PropB3P); }
+
+// The launches trot_trotter_b3v makes, as descriptors for grouped launches over many walker sets: kinds[i]
+// (TROT_B3_STAGE_*), workgroups[i] and descs (max_stages * trot_prop_b3_stage_bytes() bytes) of each, in
+// order.  Returns the number of launches, or an error.  Nothing is launched; ws must stay allocated while
+// the descriptors are used.
+long trot_trotter_b3v_stages(const void* v_planes, int hi_from, const void* onebody_planes, const double* w_in,
+                             double* w_out, int W, int N, int n, double sqrt_dt, int n_terms, void* ws,
+                             size_t ws_bytes, int max_stages, int* kinds, long* workgroups, void* descs) {
+  if (!v_planes || !onebody_planes || !w_in || !w_out || !kinds || !workgroups || !descs)
+    return fail(TROT_ERR_INVALID, "trotter_b3v_stages: null pointer");
+  if (int rc = check_sizes("trotter_b3v_stages", W, N, n, n_terms)) return rc;
+  if (hi_from < 1) return fail(TROT_ERR_INVALID, "trotter_b3v_stages: hi_from must be >= 1");
+  if (ws_bytes < b3_workspace(W, N, n)) return fail(TROT_ERR_WORKSPACE, "trotter_b3v_stages: workspace");
+  const long vcols = symblk_count((N + 7) / 8) * 64;
+  const unsigned* vh = (const unsigned*)v_planes;
+  std::vector<B3Stage> stages;
+  if (int rc = b3_stages(VSrc{nullptr, vh, vh + (size_t)W * vcols, hi_from}, (const bf16raw*)onebody_planes, w_in,
+                         w_out, W, N, n, sqrt_dt, n_terms, ws, ws_bytes, stages))
+    return rc;
+  if ((long)stages.size() > max_stages) return fail(TROT_ERR_INVALID, "trotter_b3v_stages: more launches than max_stages");
+  for (size_t i = 0; i < stages.size(); ++i) {
+    kinds[i] = stages[i].kind;
+    workgroups[i] = b3_stage_workgroups(stages[i]);
+    reinterpret_cast<PropB3P*>(descs)[i] = stages[i].P;
+  }
+  return (long)stages.size();
+}
+
+// One launch of a propagation stage kind over many walker sets: probs (device) holds nprob descriptors from
+// trot_trotter_b3v_stages of that kind, start (device) the nprob + 1 prefix sums of their workgroup counts.
+// Each workgroup runs exactly the body it runs in its own set's launch (bit-identical results).
+int trot_prop_b3_grouped(int kind, const void* probs, const int* start, int nprob, int total_wg, void* stream) {
+  if (!probs || !start || nprob < 1 || total_wg < 1) return fail(TROT_ERR_INVALID, "prop_b3_grouped: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const PropB3P* P = (const PropB3P*)probs;
+  switch (kind) {
+    case B3_SPLIT:
+      hipLaunchKernelGGL(split_transpose_grouped_kernel, dim3(total_wg), dim3(256), 0, s, P, start, nprob);
+      break;
+    case B3_ONEBODY:
+      hipLaunchKernelGGL((prop_b3_grouped_kernel<true, 0>), dim3(total_wg), dim3(256), 0, s, P, start, nprob);
+      break;
+    case B3_TERM_C64:
+      hipLaunchKernelGGL((prop_b3_grouped_kernel<false, 0>), dim3(total_wg), dim3(256), 0, s, P, start, nprob);
+      break;
+    case B3_TERM3:
+      hipLaunchKernelGGL((prop_b3_grouped_kernel<false, 1>), dim3(total_wg), dim3(256), 0, s, P, start, nprob);
+      break;
+    case B3_TERM1:
+      hipLaunchKernelGGL((prop_b3_grouped_kernel<false, 2>), dim3(total_wg), dim3(256), 0, s, P, start, nprob);
+      break;
+    default: return fail(TROT_ERR_UNSUPPORTED, "prop_b3_grouped: this stage kind has no grouped launch");
+  }
+  HIPCHK(hipGetLastError());
+  return TROT_OK;
 }
 
 }  // extern "C"

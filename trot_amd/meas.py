@@ -197,7 +197,8 @@ class Engine:
         _lib.rdm1_from_green(mb, cb, dm, 1)
         return dm[0]
 
-    def force_bias_from_green(self, greens, meas_ctx) -> torch.Tensor:
+    def force_bias_from_green(self, greens, meas_ctx, out: torch.Tensor | None = None) -> torch.Tensor:
+        """(W, G) force bias from the transposed half Green's functions; ``out`` (split-BF16 form) receives it."""
         sa, sb = self.spins(meas_ctx)
         alpha = 2.0 if self.kind == "rr" else 1.0  # trot/meas/rhf.py:124-125 (both spins); 150-153; uhf.py:55-57
         if self.precision == "f64":
@@ -210,7 +211,7 @@ class Engine:
             # NT GEMM on split-bf16 planes: A = GT (W x N n, complex), B = R' (G x N n), split K
             W = greens[0].shape[0]
             G = sa.rot_chol_t.shape[0]
-            fb = torch.empty((W, G), dtype=torch.complex128, device=greens[0].device)
+            fb = out if out is not None else torch.empty((W, G), dtype=torch.complex128, device=greens[0].device)
             for i, (g, s) in enumerate(((greens[0], sa), (greens[1], sb))):
                 if i == 1 and self.kind == "rr":
                     break

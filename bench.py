@@ -643,8 +643,8 @@ def batch_rates(steps: int = 20) -> dict:
     of per-fragment launches on 4 branches, one HIP graph of grouped launches with the whole propagation of
     every walker set in one fused launch (the default), and the same with the propagation as one-body / copy /
     Taylor-term launches on autotuned tiles; then the same fragments in the headline's mixed form (split-BF16
-    GEMMs, Taylor terms 3-6 one bf16 pass) one after another and through the graph of per-fragment launches
-    (the grouped launches are FP64 only).
+    GEMMs, Taylor terms 3-6 one bf16 pass) one after another, through the graph of per-fragment launches on
+    4 branches and through the graph of grouped launches.
     frac = FP64 MFMA ideal of the step's GEMMs (VHS on the stored half of V, force bias, Taylor, one-body)
     over the grouped step's time."""
     import torch
@@ -657,7 +657,7 @@ def batch_rates(steps: int = 20) -> dict:
     fields = None
     blocks = {}
     for mode in ("eager", "graph_4_branches", "grouped_graph", "grouped_graph_unfused", "mixed_eager",
-                 "mixed_graph_4_branches"):
+                 "mixed_graph_4_branches", "mixed_grouped_graph"):
         prec = "bf16x3" if mode.startswith("mixed") else "f64"
         frags = [_fragment(i, N, n, G, W, dt, 1.0 / np.sqrt(N), prec) for i, (N, n, G) in enumerate(sizes)]
         if fields is None:
@@ -678,7 +678,8 @@ def batch_rates(steps: int = 20) -> dict:
                     f.state = f.step(fields[s][i])
             torch.cuda.synchronize()
         else:
-            fb = batch.FragmentBatch(frags, grouped=mode.startswith("grouped"), fused=mode == "grouped_graph")
+            fb = batch.FragmentBatch(frags, grouped="grouped" in mode, fused=mode == "grouped_graph")
+            assert fb.grouped == ("grouped" in mode)
             if mode.startswith("grouped"):
                 fb.autotune()  # the grouped plan takes the tile shapes of its own batch
             flat = [fb.pack_fields([x.cpu().numpy() for x in fs]) for fs in fields]
@@ -693,7 +694,7 @@ def batch_rates(steps: int = 20) -> dict:
         ms = 1e3 * (time.perf_counter() - t0) / steps
         out[f"{mode}_ms_per_step"] = round(ms, 3)
         log(f"batch {mode}: {ms:.2f} ms per step of all 32 fragments")
-        if mode in ("grouped_graph", "mixed_graph_4_branches"):
+        if mode in ("grouped_graph", "mixed_grouped_graph"):
             # every mode ran the same steps with the same fields from the same states: the block energy of
             # each fragment (each form's own energy kernels, its own weights) after them
             blocks[mode] = [_fragment_block_energy(f) for f in frags]
@@ -707,8 +708,8 @@ def batch_rates(steps: int = 20) -> dict:
     out["grouped_fp64_frac"] = round(out["fp64_t_star_ms"] / out["grouped_graph_ms_per_step"], 4)
     out["speedup_grouped_vs_eager"] = round(out["eager_ms_per_step"] / out["grouped_graph_ms_per_step"], 3)
     out["mixed_speedup_vs_fp64_grouped"] = round(out["grouped_graph_ms_per_step"]
-                                                 / out["mixed_graph_4_branches_ms_per_step"], 3)
-    d = [abs(m - f) * 1e3 for m, f in zip(blocks["mixed_graph_4_branches"], blocks["grouped_graph"])]
+                                                 / out["mixed_grouped_graph_ms_per_step"], 3)
+    d = [abs(m - f) * 1e3 for m, f in zip(blocks["mixed_grouped_graph"], blocks["grouped_graph"])]
     out["mixed_vs_fp64_block_energy_mHa"] = {"steps": steps + 2, "max_over_fragments": round(max(d), 6),
                                              "mean_over_fragments": round(float(np.mean(d)), 6)}
     return out

@@ -362,11 +362,14 @@ class FragmentBatch:
     the graph joins the branches at the end, so fragments on different branches may run concurrently.
     Per fragment the kernels and their order are the same as in a one-by-one run.
 
-    ``grouped`` (default: when every fragment runs the FP64 step on sym-block V): every operation of the
-    step runs as one launch over all fragments (force bias and its slice sums, shifted fields, VHS,
-    one-body, Taylor terms, overlaps, inverses, Green's functions, weight update; the trot_*_grouped
-    entry points; one launch per tile shape where fragments differ); the results stay bit-identical to the
-    one-by-one run.  ``autotune`` before ``capture`` picks every fragment's Taylor tile shape by timing.
+    ``grouped`` (default: when every fragment runs the FP64 step on sym-block V, or every fragment runs the
+    split-BF16 step on sym-block V): every operation of the step runs as one launch over all fragments
+    (force bias and its slice sums, shifted fields, VHS, one-body, Taylor terms, overlaps, inverses, Green's
+    functions, weight update; the trot_*_grouped entry points; one launch per tile shape where fragments
+    differ); the results stay bit-identical to the one-by-one run.  ``autotune`` before ``capture`` picks
+    every fragment's FP64 Taylor tile shape by timing.  In split-BF16 the force-bias and VHS GEMMs, the
+    operand splits, the slice sums and each stage of the split-BF16 propagation (one-body, Taylor terms,
+    copies; the same stage sequence in every walker set) are one launch each over all fragments.
     ``fused`` (default): the propagation of every walker set whose N x n operand fits in LDS (N <= 256,
     n <= 48) is ONE launch over all of them (trot_fused_prop_grouped: one workgroup per walker runs the
     one-body product, all Taylor terms and the last one-body product), also bit-identical.

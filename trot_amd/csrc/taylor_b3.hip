@@ -34,6 +34,15 @@ namespace {
 
 constexpr int PB_BM = 128, PB_BN = 64, PB_BK = 32;
 
+// Waves per workgroup: 8 (32 x 32 each, four per SIMD at two workgroups per CU) for the one-body product
+// and the one-pass term, whose registers allow it; 4 (64 x 32 each) for the three-pass term, which needs
+// more than 128 registers a lane and would drop to one workgroup per CU with 8.  The MFMA sequence of each
+// output element is the same either way.
+template <bool REAL_A, int VM>
+struct PropWaves {
+  static constexpr int value = (REAL_A || VM == 2) ? 8 : 4;
+};
+
 struct PropB3P {
   const float* vsym;    // Taylor: V, sym-block complex64 (W, vcols)
   const unsigned* vh;   // Taylor, split V: hi plane (W, vcols) of words bf16 re | bf16 im << 16
@@ -62,9 +71,10 @@ __device__ __forceinline__ unsigned pack2(bf16raw a, bf16raw b) { return (unsign
 // REAL_A: product (E - I) . planes (one body); else V . planes (one Taylor term).
 // VM (Taylor only): 0 V as complex64, split here; 1 V as split planes (hi, lo), three passes; 2 the hi
 // planes of V and of the term only, one pass (bf16 products for the late, small terms).
-template <bool REAL_A, int VM>
+template <bool REAL_A, int VM, int NW>
 __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
   constexpr int BM = PB_BM, BN = PB_BN, BK = PB_BK;
+  constexpr int TI = NW == 8 ? 2 : 4;  // 16-row MFMA tiles per wave: waves of 64 x 32 (4 waves) or 32 x 32 (8)
   constexpr int PA = REAL_A ? 2 : 4;
   constexpr bool HIONLY = !REAL_A && VM == 2;  // lo planes neither loaded nor multiplied
   constexpr bool BHI = HIONLY;
@@ -87,10 +97,11 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int row0 = t >> 2, kq = (t & 3) * 8;  // 16-byte segment of a plane row
+  const bool stager = NW == 4 || t < 256;     // the first 256 threads stage the operands
+  const int row0 = (t & 255) >> 2, kq = (t & 3) * 8;  // 16-byte segment of a plane row
 
   // V staging: thread t reads stored rows 2 aq, 2 aq + 1 (128 contiguous bytes) of block ablk
-  const int ablk = t >> 2, aq = t & 3;
+  const int ablk = (t & 255) >> 2, aq = t & 3;
   const int rbl = ablk >> 2, kbl = ablk & 3;
   const int rb = (m0 >> 3) + rbl;
   const float* Vb = REAL_A ? nullptr : p.vsym + walker * p.vcols * 2;
@@ -194,22 +205,22 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
     }
   };
 
-  f4_t acc[2][4][2];
+  f4_t acc[2][TI][2];
 #pragma unroll
   for (int q = 0; q < 2; ++q)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[q][i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
   const int nk = p.Np / BK;
   const int seg = lane >> 4;
-  load(0);
+  if (stager) load(0);
   for (int kc = 0; kc < nk; ++kc) {
     __syncthreads();
-    store();
+    if (stager) store();
     __syncthreads();
-    if (kc + 1 < nk) load((kc + 1) * BK);
+    if (stager && kc + 1 < nk) load((kc + 1) * BK);
     // B fragments [j][plane], plane 0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo
     bf16x8_t bf[2][4];
 #pragma unroll
@@ -222,8 +233,8 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
       }
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = wm * 64 + i * 16 + (lane & 15);
+    for (int i = 0; i < TI; ++i) {
+      const int row = wm * 16 * TI + i * 16 + (lane & 15);
       bf16x8_t a[PA];
 #pragma unroll
       for (int q = 0; q < PA; ++q) {
@@ -267,10 +278,10 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
   bf16raw* Tn = p.tn ? p.tn + walker * 4 * tplane : nullptr;
   const bool to_w = REAL_A || p.last;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int gi0 = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
+      const int gi0 = m0 + wm * 16 * TI + i * 16 + 4 * (lane >> 4);
       const int gp = n0 + wn * 32 + j * 16 + (lane & 15);
       bf16raw hr[4], lr[4], hi[4], li[4];
 #pragma unroll
@@ -323,8 +334,8 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
 }
 
 template <bool REAL_A, int VM = 0>
-__global__ __launch_bounds__(256, 2) void prop_b3_kernel(PropB3P p) {
-  prop_b3_body<REAL_A, VM>(p, blockIdx.x);
+__global__ __launch_bounds__((64 * PropWaves<REAL_A, VM>::value), 2) void prop_b3_kernel(PropB3P p) {
+  prop_b3_body<REAL_A, VM, PropWaves<REAL_A, VM>::value>(p, blockIdx.x);
 }
 
 // The propagation kernels of many walker sets in one launch: problem i owns workgroups start[i] ..
@@ -340,11 +351,11 @@ __device__ __forceinline__ int b3_problem_of(const int* __restrict__ start, int 
 }
 
 template <bool REAL_A, int VM>
-__global__ __launch_bounds__(256, 2) void prop_b3_grouped_kernel(const PropB3P* __restrict__ probs,
-                                                                const int* __restrict__ start, int nprob) {
+__global__ __launch_bounds__((64 * PropWaves<REAL_A, VM>::value), 2) void prop_b3_grouped_kernel(
+    const PropB3P* __restrict__ probs, const int* __restrict__ start, int nprob) {
   const int pi = b3_problem_of(start, nprob, blockIdx.x);
   const PropB3P p = probs[pi];
-  prop_b3_body<REAL_A, VM>(p, blockIdx.x - start[pi]);
+  prop_b3_body<REAL_A, VM, PropWaves<REAL_A, VM>::value>(p, blockIdx.x - start[pi]);
 }
 
 // Transposed planes (W, 4, np, Np) of walkers w (W, N, n) complex128, zero padded.  Grid (Np/64, np/64, W),
@@ -425,7 +436,8 @@ int launch_prop(const PropB3P& P, hipStream_t s) {
   const int tpw = (P.Np / PB_BM) * (P.np_ / PB_BN);
   const long nwg = (long)((P.W + 7) / 8) * 8 * tpw;
   if (nwg > 0x7fffffffL) return fail(TROT_ERR_INVALID, "propagation b3: too many walkers");
-  hipLaunchKernelGGL((prop_b3_kernel<REAL_A, VM>), dim3((unsigned)nwg), dim3(256), 0, s, P);
+  constexpr int threads = 64 * PropWaves<REAL_A, VM>::value;
+  hipLaunchKernelGGL((prop_b3_kernel<REAL_A, VM>), dim3((unsigned)nwg), dim3(threads), 0, s, P);
   HIPCHK(hipGetLastError());
   return TROT_OK;
 }
@@ -682,21 +694,23 @@ int trot_prop_b3_grouped(int kind, const void* probs, const int* start, int npro
   if (!probs || !start || nprob < 1 || total_wg < 1) return fail(TROT_ERR_INVALID, "prop_b3_grouped: bad arguments");
   hipStream_t s = (hipStream_t)stream;
   const PropB3P* P = (const PropB3P*)probs;
+  constexpr int nw_ob = 64 * PropWaves<true, 0>::value, nw_c64 = 64 * PropWaves<false, 0>::value;
+  constexpr int nw_t3 = 64 * PropWaves<false, 1>::value, nw_t1 = 64 * PropWaves<false, 2>::value;
   switch (kind) {
     case B3_SPLIT:
       hipLaunchKernelGGL(split_transpose_grouped_kernel, dim3(total_wg), dim3(256), 0, s, P, start, nprob);
       break;
     case B3_ONEBODY:
-      hipLaunchKernelGGL((prop_b3_grouped_kernel<true, 0>), dim3(total_wg), dim3(256), 0, s, P, start, nprob);
+      hipLaunchKernelGGL((prop_b3_grouped_kernel<true, 0>), dim3(total_wg), dim3(nw_ob), 0, s, P, start, nprob);
       break;
     case B3_TERM_C64:
-      hipLaunchKernelGGL((prop_b3_grouped_kernel<false, 0>), dim3(total_wg), dim3(256), 0, s, P, start, nprob);
+      hipLaunchKernelGGL((prop_b3_grouped_kernel<false, 0>), dim3(total_wg), dim3(nw_c64), 0, s, P, start, nprob);
       break;
     case B3_TERM3:
-      hipLaunchKernelGGL((prop_b3_grouped_kernel<false, 1>), dim3(total_wg), dim3(256), 0, s, P, start, nprob);
+      hipLaunchKernelGGL((prop_b3_grouped_kernel<false, 1>), dim3(total_wg), dim3(nw_t3), 0, s, P, start, nprob);
       break;
     case B3_TERM1:
-      hipLaunchKernelGGL((prop_b3_grouped_kernel<false, 2>), dim3(total_wg), dim3(256), 0, s, P, start, nprob);
+      hipLaunchKernelGGL((prop_b3_grouped_kernel<false, 2>), dim3(total_wg), dim3(nw_t1), 0, s, P, start, nprob);
       break;
     default: return fail(TROT_ERR_UNSUPPORTED, "prop_b3_grouped: this stage kind has no grouped launch");
   }

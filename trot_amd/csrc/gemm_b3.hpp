@@ -243,42 +243,55 @@ __device__ __forceinline__ bool b3_tile_of(int b, int n_mt, int n_nt, int GM, in
 
 // Main loop of the wide form: acc[q][i][j] (q = re, im of A) for the wave's 64 x 64 sub-tile
 // (rows wm * 64 .., columns wn * 64 ..; wm = wave / 4, wn = wave % 4) of the tile at (m0, n0).
+//
+// Operands go from global memory straight into LDS (global_load_lds_dwordx4: lane L of a wave writes the
+// 16 bytes at LDS base + 16 L), without passing through registers.  A stage is 64 chunks of 1 KB (16 rows
+// of 64 bytes: 32 of A over its 4 planes, 32 of B over its 2), 8 per wave.  Lane L of a chunk carries
+// row L / 4 of the chunk into LDS slot L % 4, which the swizzle assigns to segment (L % 4) ^ b3_swz(row),
+// so each lane picks the global address of that segment.  Two stages (128 KB): after the barrier that
+// completes stage k, each wave reads all of its stage-k fragments, issues the loads of stage k + 1 into
+// the other buffer (free: every wave is past its reads of stage k - 1) and runs the MFMAs of stage k,
+// which cover those loads.  B rows at or beyond b_rows are read from row b_rows - 1: output column j
+// depends on B row j only, and columns beyond the true count are never written.  Ends on a barrier
+// with no load in flight, so the caller may reuse smem.
+// TOUT: the MFMAs take B as their first operand, so acc[q][i][j][r] holds row 16 i + (lane & 15), column
+// 16 j + 4 (lane >> 4) + r of the wave's sub-tile (the same products in the same order; b3w_epilogue);
+// otherwise row 16 i + 4 (lane >> 4) + r, column 16 j + (lane & 15) (b3_epilogue's layout).
+template <bool TOUT = false>
 __device__ __forceinline__ void b3w_mainloop(const B3P& p, int m0, int n0, int b_rows, char* smem,
                                              f4_t (&acc)[2][4][4], int slice) {
   constexpr int BM = B3W_BM, BN = B3W_BN, BK = B3W_BK;
-  typedef bf16raw AsT[4][BM * BK];
-  typedef bf16raw BsT[2][BN * BK];
-  AsT* As = reinterpret_cast<AsT*>(smem);                                  // [2 stages]
-  BsT* Bs = reinterpret_cast<BsT*>(smem + 2 * sizeof(AsT));                // [2 stages]
+  constexpr int ASTAGE = 4 * BM * BK, BSTAGE = 2 * BN * BK;  // bf16 elements per stage
+  typedef __attribute__((address_space(3))) bf16raw lds_t;
+  lds_t* As = (lds_t*)smem;                  // [2 stages][4 planes][BM * BK]
+  lds_t* Bs = As + 2 * ASTAGE;               // [2 stages][2 planes][BN * BK]
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 2, wn = wave & 3;
-  const int row0 = t >> 2, kq = (t & 3) * 8;  // A row row0; B rows row0, row0 + 128
-  const int st0 = b3_lds(row0, t & 3), st1 = b3_lds(row0 + 128, t & 3);
-  const bool bok0 = n0 + row0 < b_rows, bok1 = n0 + row0 + 128 < b_rows;
-  const bf16raw* Ab = p.A + (long)(m0 + row0) * p.Kp + kq;
-  const bf16raw* Bb = p.B + (long)(n0 + row0) * p.Kp + kq;
-  const long b128 = 128L * p.Kp;
-
-  u32x4_t stg[8];
-  const u32x4_t zero4 = {0u, 0u, 0u, 0u};
-  auto load = [&](int k0) {
+  const int crow = lane >> 2;  // row of the lane within its chunk
+  const bf16raw* asrc[4];
+  const bf16raw* bsrc[4];
+  int aoff[4], boff[4];  // wave-uniform LDS element offsets of the chunks within a stage
 #pragma unroll
-    for (int q = 0; q < 4; ++q) stg[q] = *reinterpret_cast<const u32x4_t*>(Ab + q * p.a_plane + k0);
+  for (int i = 0; i < 4; ++i) {
+    const int c = 4 * wave + i;
+    const int qa = c >> 3, ra = 16 * (c & 7) + crow;
+    asrc[i] = p.A + qa * p.a_plane + (long)(m0 + ra) * p.Kp + 8 * ((lane & 3) ^ b3_swz(ra));
+    aoff[i] = qa * BM * BK + 16 * (c & 7) * BK;
+    const int qb = c >> 4, rb = 16 * (c & 15) + crow;
+    const int gb = min(n0 + rb, b_rows - 1);
+    bsrc[i] = p.B + qb * p.b_plane + (long)gb * p.Kp + 8 * ((lane & 3) ^ b3_swz(rb));
+    boff[i] = qb * BN * BK + 16 * (c & 15) * BK;
+  }
+  auto issue = [&](int buf, int k0) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      stg[4 + 2 * q] = bok0 ? *reinterpret_cast<const u32x4_t*>(Bb + q * p.b_plane + k0) : zero4;
-      stg[5 + 2 * q] = bok1 ? *reinterpret_cast<const u32x4_t*>(Bb + q * p.b_plane + b128 + k0) : zero4;
-    }
-  };
-  auto store = [&](int buf) {
+    for (int i = 0; i < 4; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(asrc[i] + k0), (__attribute__((address_space(3))) void*)(As + buf * ASTAGE + aoff[i]),
+                                       16, 0, 0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) *reinterpret_cast<u32x4_t*>(&As[buf][q][st0]) = stg[q];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      *reinterpret_cast<u32x4_t*>(&Bs[buf][q][st0]) = stg[4 + 2 * q];
-      *reinterpret_cast<u32x4_t*>(&Bs[buf][q][st1]) = stg[5 + 2 * q];
-    }
+    for (int i = 0; i < 4; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(bsrc[i] + k0), (__attribute__((address_space(3))) void*)(Bs + buf * BSTAGE + boff[i]),
+                                       16, 0, 0);
   };
 
 #pragma unroll
@@ -291,55 +304,144 @@ __device__ __forceinline__ void b3w_mainloop(const B3P& p, int m0, int n0, int b
   const int kbeg = slice * p.k_slice;
   const int nk = min(p.k_slice, p.Kp - kbeg) / BK;
   const int seg = lane >> 4;
-  if (nk > 0) {
-    load(kbeg);
-    store(0);
-    if (nk > 1) load(kbeg + BK);
-  }
-  __syncthreads();
+  if (nk > 0) issue(0, kbeg);
   for (int kc = 0; kc < nk; ++kc) {
     const int cur = kc & 1;
-    bf16x8_t bh[4], bl[4];
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's loads of stage kc have landed
+    __syncthreads();                                   // and every wave's
+    const lds_t* A = As + cur * ASTAGE;
+    const lds_t* B = Bs + cur * BSTAGE;
+    // B fragments and row tile 0's A fragments first; the loads of the next stage; then the MFMAs of row
+    // tile i while the A fragments of row tile i + 1 are read
+    bf16x8_t bh[4], bl[4], ah[2][2], al[2][2];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int col = wn * 64 + j * 16 + (lane & 15);
-      bh[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[cur][0][b3_lds(col, seg)]);
-      bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[cur][1][b3_lds(col, seg)]);
+      bh[j] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8_t*>(B + b3_lds(col, seg));
+      bl[j] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8_t*>(B + BN * BK + b3_lds(col, seg));
     }
-    // MFMAs of row tiles 0-1, then the next stage's LDS stores (its loads were issued one K step ago)
-    // and the loads two stages ahead, then row tiles 2-3: the stores overlap MFMA work instead of
-    // forming a tail before the barrier.
-    auto mfma_rows = [&](int i0) {
+    auto read_a = [&](int i) {
+      const int row = wm * 64 + i * 16 + (lane & 15);
 #pragma unroll
-      for (int i = i0; i < i0 + 2; ++i) {
-        const int row = wm * 64 + i * 16 + (lane & 15);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const bf16x8_t ah = *reinterpret_cast<const bf16x8_t*>(&As[cur][2 * q][b3_lds(row, seg)]);
-          const bf16x8_t al = *reinterpret_cast<const bf16x8_t*>(&As[cur][2 * q + 1][b3_lds(row, seg)]);
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            f4_t c = acc[q][i][j];
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], c, 0, 0, 0);
-            acc[q][i][j] = c;
-          }
-        }
+      for (int q = 0; q < 2; ++q) {
+        ah[i & 1][q] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8_t*>(A + 2 * q * BM * BK + b3_lds(row, seg));
+        al[i & 1][q] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8_t*>(A + (2 * q + 1) * BM * BK + b3_lds(row, seg));
       }
     };
+    read_a(0);
+    if (kc + 1 < nk) issue(cur ^ 1, kbeg + (kc + 1) * BK);
     __builtin_amdgcn_s_setprio(1);
-    mfma_rows(0);
-    __builtin_amdgcn_s_setprio(0);
-    if (kc + 1 < nk) {
-      store(cur ^ 1);
-      if (kc + 2 < nk) load(kbeg + (kc + 2) * BK);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i < 3) read_a(i + 1);
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          f4_t c = acc[q][i][j];
+          if constexpr (TOUT) {
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i & 1][q], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i & 1][q], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i & 1][q], c, 0, 0, 0);
+          } else {
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i & 1][q], bh[j], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i & 1][q], bl[j], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i & 1][q], bh[j], c, 0, 0, 0);
+          }
+          acc[q][i][j] = c;
+        }
     }
-    __builtin_amdgcn_s_setprio(1);
-    mfma_rows(2);
     __builtin_amdgcn_s_setprio(0);
-    __syncthreads();
   }
+  __syncthreads();
+}
+
+// Epilogue of the wide kernels for the TOUT layout (complex A): the four values of a lane are consecutive
+// columns gj0 .. gj0 + 3 of row gi, so each plane of a run leaves as one 16-byte store (two for complex64
+// and FP32 partials) when the output's leading dimension keeps runs 16-byte aligned; otherwise, and for
+// runs cut by the column bound, element by element.  Each element is computed by the same expressions as
+// b3_epilogue's.
+template <typename TC>
+__device__ __forceinline__ void b3w_epilogue(const B3P& p, const f4_t (&acc)[2][4][4], int mrow, int ncol, int lane,
+                                             int slice = -1) {
+  const long sl = slice < 0 ? (long)blockIdx.y : (long)slice;
+  const bool c16 = (reinterpret_cast<uintptr_t>(p.C) & 15) == 0;
+  bool vec;
+  if (p.part) {
+    vec = (p.N & 1) == 0;
+  } else if constexpr (__is_same(TC, unsigned)) {
+    vec = c16 && (p.ldc & 3) == 0;
+  } else if constexpr (sizeof(TC) == 4) {
+    vec = c16 && (p.ldc & 1) == 0 && p.beta == 0.f;
+  } else {
+    vec = false;  // complex128: one 16-byte store per element either way
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gi = mrow + i * 16 + (lane & 15);
+      const int gj0 = ncol + j * 16 + 4 * (lane >> 4);
+      if (gi >= p.M) continue;
+      float vr[4], vi[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float xr = acc[0][i][j][r], xi = acc[1][i][j][r];
+        vr[r] = p.part ? xr : p.ar * xr - p.ai * xi;
+        vi[r] = p.part ? xi : p.ar * xi + p.ai * xr;
+      }
+      if (vec && gj0 + 3 < p.N) {
+        if (p.part) {
+          float4* o = reinterpret_cast<float4*>(reinterpret_cast<float2*>(p.part) + (sl * p.M + gi) * p.N + gj0);
+          o[0] = make_float4(vr[0], vi[0], vr[1], vi[1]);
+          o[1] = make_float4(vr[2], vi[2], vr[3], vi[3]);
+        } else if constexpr (__is_same(TC, unsigned)) {
+          u32x4_t hw, lw;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            bf16raw rh, rl, ih, il;
+            split_bf16_raw(vr[r], rh, rl);
+            split_bf16_raw(vi[r], ih, il);
+            hw[r] = (unsigned)rh | ((unsigned)ih << 16);
+            lw[r] = (unsigned)rl | ((unsigned)il << 16);
+          }
+          unsigned* c = reinterpret_cast<unsigned*>(p.C) + (long)gi * p.ldc + gj0;
+          *reinterpret_cast<u32x4_t*>(c) = hw;
+          *reinterpret_cast<u32x4_t*>(c + (long)p.M * p.ldc) = lw;
+        } else if constexpr (sizeof(TC) == 4) {
+          float4* o = reinterpret_cast<float4*>(reinterpret_cast<float2*>(p.C) + (long)gi * p.ldc + gj0);
+          o[0] = make_float4(vr[0], vi[0], vr[1], vi[1]);
+          o[1] = make_float4(vr[2], vi[2], vr[3], vi[3]);
+        }
+        continue;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = gj0 + r;
+        if (gj >= p.N) continue;
+        if (p.part) {
+          reinterpret_cast<float2*>(p.part)[(sl * p.M + gi) * p.N + gj] = make_float2(vr[r], vi[r]);
+          continue;
+        }
+        const long off = (long)gi * p.ldc + gj;
+        if constexpr (__is_same(TC, unsigned)) {
+          bf16raw rh, rl, ih, il;
+          split_bf16_raw(vr[r], rh, rl);
+          split_bf16_raw(vi[r], ih, il);
+          unsigned* c = reinterpret_cast<unsigned*>(p.C);
+          c[off] = (unsigned)rh | ((unsigned)ih << 16);
+          c[(long)p.M * p.ldc + off] = (unsigned)rl | ((unsigned)il << 16);
+        } else if constexpr (sizeof(TC) == 8) {
+          double2* c = reinterpret_cast<double2*>(p.C) + off;
+          double2 o = p.beta != 0.f ? *c : make_double2(0.0, 0.0);
+          *c = make_double2((double)vr[r] + (double)p.beta * o.x, (double)vi[r] + (double)p.beta * o.y);
+        } else {
+          float2* c = reinterpret_cast<float2*>(p.C) + off;
+          float2 o = p.beta != 0.f ? *c : make_float2(0.f, 0.f);
+          *c = make_float2(vr[r] + p.beta * o.x, vi[r] + p.beta * o.y);
+        }
+      }
+    }
 }
 
 template <typename TC>
@@ -349,9 +451,9 @@ __global__ __launch_bounds__(512, 1) void gemm_b3w_kernel(B3P p, int n_mt, int n
   if (!b3_tile_of(blockIdx.x, n_mt, n_nt, 8, mt, nt)) return;
   const int m0 = mt * B3W_BM, n0 = nt * B3W_BN;
   f4_t acc[2][4][4];
-  b3w_mainloop(p, m0, n0, b_rows, smem, acc, blockIdx.y);
+  b3w_mainloop<true>(p, m0, n0, b_rows, smem, acc, blockIdx.y);
   const int wave = threadIdx.x >> 6;
-  b3_epilogue<true, TC>(p, acc, m0 + (wave >> 2) * 64, n0 + (wave & 3) * 64, threadIdx.x & 63);
+  b3w_epilogue<TC>(p, acc, m0 + (wave >> 2) * 64, n0 + (wave & 3) * 64, threadIdx.x & 63);
 }
 
 // The wide GEMMs of many problems in one launch (the batching form's force bias and VHS in mixed precision):
@@ -379,9 +481,9 @@ __global__ __launch_bounds__(512, 1) void gemm_b3w_grouped_kernel(const B3GProb*
   if (!b3_tile_of(tb, q.n_mt, q.n_nt, 8, mt, nt)) return;
   const int m0 = mt * B3W_BM, n0 = nt * B3W_BN;
   f4_t acc[2][4][4];
-  b3w_mainloop(q.p, m0, n0, q.b_rows, smem, acc, slice);
+  b3w_mainloop<true>(q.p, m0, n0, q.b_rows, smem, acc, slice);
   const int wave = threadIdx.x >> 6;
-  b3_epilogue<true, TC>(q.p, acc, m0 + (wave >> 2) * 64, n0 + (wave & 3) * 64, threadIdx.x & 63, slice);
+  b3w_epilogue<TC>(q.p, acc, m0 + (wave >> 2) * 64, n0 + (wave & 3) * 64, threadIdx.x & 63, slice);
 }
 
 // C (complex128, M x N, ldc) = alpha * sum_s part[s] (+ beta C), slices summed in order in FP64.

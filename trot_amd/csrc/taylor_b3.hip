@@ -16,8 +16,8 @@
 // S + T_K to w in FP64.  The terms are small next to w, so FP32 rounding of their sum stays
 // far below the split-BF16 product error.
 //
-// prop_b3_kernel: one 128 (orbitals) x 64 (occupied) output tile per workgroup, 256 threads,
-// K step 32; waves 2 x 2 of 64 x 32 (4 x 2 MFMA tiles).  The V tile is read straight from
+// prop_b3_kernel: one 128 (orbitals) x 64 (occupied) output tile per workgroup; 4 waves of 64 x 32
+// (K staged 32 at a time) or 8 waves of 32 x 32 (K staged 64 at a time).  The V tile is read straight from
 // sym-block storage: the 8 x 8 blocks above the diagonal land in LDS as stored, the mirrored
 // ones transposed, and both are split into bf16 planes on the way.  Workgroups are ordered so
 // that the row tiles of one walker run on one XCD (workgroup b runs on XCD b % 8) and share
@@ -32,12 +32,13 @@
 
 namespace {
 
-constexpr int PB_BM = 128, PB_BN = 64, PB_BK = 32;
+constexpr int PB_BM = 128, PB_BN = 64;
 
 // Waves per workgroup: 8 (32 x 32 each, four per SIMD at two workgroups per CU) for the one-body product
 // and the one-pass term, whose registers allow it; 4 (64 x 32 each) for the three-pass term, which needs
 // more than 128 registers a lane and would drop to one workgroup per CU with 8.  The MFMA sequence of each
 // output element is the same either way.
+// The second launch bound (waves per SIMD) keeps two workgroups per CU: 4 for 8 waves, 2 for 4.
 template <bool REAL_A, int VM>
 struct PropWaves {
   static constexpr int value = (REAL_A || VM == 2) ? 8 : 4;
@@ -68,20 +69,42 @@ __device__ __forceinline__ bf16x8_t neg_bf16x8(bf16x8_t x) {
 
 __device__ __forceinline__ unsigned pack2(bf16raw a, bf16raw b) { return (unsigned)a | ((unsigned)b << 16); }
 
+// LDS offset (bf16 elements) of 16-byte segment seg of row r in a tile of BK-wide rows.  BK = 32: the
+// swizzle of the split-BF16 GEMMs.  BK = 64 (128-byte rows): segment seg ^ ((r >> 1) & 7), so that the 16
+// rows of a fragment read (one segment each) fall on 16 distinct 16-byte bank groups, and the two 16-lane
+// halves of a 32-lane read (segments s, s + 1) on 32 distinct ones.
+template <int BK>
+__device__ __forceinline__ int prop_lds(int r, int seg) {
+  if constexpr (BK == 32) {
+    return b3_lds(r, seg);
+  } else {
+    return r * 64 + 8 * (seg ^ ((r >> 1) & 7));
+  }
+}
+
 // REAL_A: product (E - I) . planes (one body); else V . planes (one Taylor term).
 // VM (Taylor only): 0 V as complex64, split here; 1 V as split planes (hi, lo), three passes; 2 the hi
 // planes of V and of the term only, one pass (bf16 products for the late, small terms).
+// K advances BK = 64 per staged step with 8 waves (every thread stages a share of the A and B tiles) and
+// 32 with 4 waves; either way each accumulator sees its MFMAs in ascending K steps of 32, so the result
+// does not depend on BK.
 template <bool REAL_A, int VM, int NW>
 __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
-  constexpr int BM = PB_BM, BN = PB_BN, BK = PB_BK;
+  constexpr int BM = PB_BM, BN = PB_BN;
+  constexpr int BK = NW == 8 ? 64 : 32;
+  constexpr int NSEG = BK / 8;     // 16-byte segments per LDS row
+  static_assert(64 * NW == BN * NSEG, "one B segment per thread per plane");
   constexpr int TI = NW == 8 ? 2 : 4;  // 16-row MFMA tiles per wave: waves of 64 x 32 (4 waves) or 32 x 32 (8)
   constexpr int PA = REAL_A ? 2 : 4;
   constexpr bool HIONLY = !REAL_A && VM == 2;  // lo planes neither loaded nor multiplied
   constexpr bool BHI = HIONLY;
-  // 64-byte LDS rows (32 bf16 of K), 16-byte segments XOR-swizzled by b3_lds: 48 KB.  Two workgroups
-  // per CU (a third, possible for the one-body form, measured no faster)
-  __shared__ __attribute__((aligned(16))) bf16raw As[PA][BM * BK];
-  __shared__ __attribute__((aligned(16))) bf16raw Bs[4][BN * BK];
+  // planes held in LDS: the one-pass form keeps only re_hi, im_hi (slots 0, 1)
+  constexpr int LA = HIONLY ? 2 : PA, LB = BHI ? 2 : 4;
+  // 8 waves: 48 KB (one-pass) or 64 KB (one body); 4 waves: 48 KB.  Two workgroups per CU
+  __shared__ __attribute__((aligned(16))) bf16raw As[LA][BM * BK];
+  __shared__ __attribute__((aligned(16))) bf16raw Bs[LB][BN * BK];
+  auto aslot = [](int q) { return HIONLY ? q >> 1 : q; };
+  auto bslot = [](int q) { return BHI ? q >> 1 : q; };
 
   const int n_rt = p.Np / BM;
   const int tpw = n_rt * (p.np_ / BN);
@@ -97,29 +120,27 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const bool stager = NW == 4 || t < 256;     // the first 256 threads stage the operands
-  const int row0 = (t & 255) >> 2, kq = (t & 3) * 8;  // 16-byte segment of a plane row
+  const int row0 = t / NSEG, sq = t % NSEG;  // B row (and E rows row0, row0 + 64), 16-byte segment
 
-  // V staging: thread t reads stored rows 2 aq, 2 aq + 1 (128 contiguous bytes) of block ablk
-  const int ablk = (t & 255) >> 2, aq = t & 3;
-  const int rbl = ablk >> 2, kbl = ablk & 3;
+  // V staging: thread t reads stored rows 2 aq, 2 aq + 1 (128 contiguous bytes) of 8 x 8 block ablk
+  const int ablk = t >> 2, aq = t & 3;
+  const int rbl = ablk / NSEG, kbl = ablk % NSEG;
   const int rb = (m0 >> 3) + rbl;
   const float* Vb = REAL_A ? nullptr : p.vsym + walker * p.vcols * 2;
   const bf16raw* Eb = REAL_A ? p.ep + (long)m0 * p.Np : nullptr;
   bool up = true;
 
-  f4_t ast[VM == 0 ? 8 : 1];
+  u32x4_t est[REAL_A ? 4 : 1];
+  f4_t ast[!REAL_A && VM == 0 ? 8 : 1];
   u32x4_t vhw[VM != 0 ? 4 : 1], vlw[VM == 1 ? 4 : 1];
   u32x4_t bst[4];
   auto load = [&](int k0) {
     if constexpr (REAL_A) {
-      const long off0 = (long)row0 * p.Np + k0 + kq;
+      const long off0 = (long)row0 * p.Np + k0 + 8 * sq;
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        const u32x4_t x0 = *reinterpret_cast<const u32x4_t*>(Eb + q * (long)p.Np * p.Np + off0);
-        const u32x4_t x1 = *reinterpret_cast<const u32x4_t*>(Eb + q * (long)p.Np * p.Np + off0 + 64L * p.Np);
-        ast[4 * q] = __builtin_bit_cast(f4_t, x0);
-        ast[4 * q + 1] = __builtin_bit_cast(f4_t, x1);
+        est[2 * q] = *reinterpret_cast<const u32x4_t*>(Eb + q * (long)p.Np * p.Np + off0);
+        est[2 * q + 1] = *reinterpret_cast<const u32x4_t*>(Eb + q * (long)p.Np * p.Np + off0 + 64L * p.Np);
       }
     } else {
       const int kb = (k0 >> 3) + kbl;
@@ -143,7 +164,7 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
         }
       }
     }
-    const long offb = (long)row0 * p.Np + k0 + kq;
+    const long offb = (long)row0 * p.Np + k0 + 8 * sq;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if (BHI && (q & 1)) continue;
@@ -154,8 +175,8 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
     if constexpr (REAL_A) {
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        *reinterpret_cast<u32x4_t*>(&As[q][b3_lds(row0, t & 3)]) = __builtin_bit_cast(u32x4_t, ast[4 * q]);
-        *reinterpret_cast<u32x4_t*>(&As[q][b3_lds(row0 + 64, t & 3)]) = __builtin_bit_cast(u32x4_t, ast[4 * q + 1]);
+        *reinterpret_cast<u32x4_t*>(&As[q][prop_lds<BK>(row0, sq)]) = est[2 * q];
+        *reinterpret_cast<u32x4_t*>(&As[q][prop_lds<BK>(row0 + 64, sq)]) = est[2 * q + 1];
       }
     } else {
       // complex c = 8 srl + sc of the two stored rows: (re, im) = ast[c / 2][2 (c % 2)], [2 (c % 2) + 1]
@@ -185,7 +206,7 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
           for (int q = 0; q < 4; ++q) {
             if (HIONLY && (q & 1)) continue;
             const bf16raw* v = &h[q][8 * srl];
-            *reinterpret_cast<u32x4_t*>(&As[q][b3_lds(8 * rbl + 2 * aq + srl, kbl)]) =
+            *reinterpret_cast<u32x4_t*>(&As[aslot(q)][prop_lds<BK>(8 * rbl + 2 * aq + srl, kbl)]) =
                 (u32x4_t){pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7])};
           }
       } else {  // mirrored block: stored (2 aq + srl, sc) is tile row 8 rbl + sc, k = 8 kbl + 2 aq + srl
@@ -194,14 +215,15 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             if (HIONLY && (q & 1)) continue;
-            *reinterpret_cast<unsigned*>(&As[q][b3_lds(8 * rbl + sc, kbl) + 2 * aq]) = pack2(h[q][sc], h[q][8 + sc]);
+            *reinterpret_cast<unsigned*>(&As[aslot(q)][prop_lds<BK>(8 * rbl + sc, kbl) + 2 * aq]) =
+                pack2(h[q][sc], h[q][8 + sc]);
           }
       }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if (BHI && (q & 1)) continue;
-      *reinterpret_cast<u32x4_t*>(&Bs[q][b3_lds(row0, t & 3)]) = bst[q];
+      *reinterpret_cast<u32x4_t*>(&Bs[bslot(q)][prop_lds<BK>(row0, sq)]) = bst[q];
     }
   };
 
@@ -214,62 +236,65 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
       for (int j = 0; j < 2; ++j) acc[q][i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
   const int nk = p.Np / BK;
-  const int seg = lane >> 4;
-  if (stager) load(0);
+  load(0);
   for (int kc = 0; kc < nk; ++kc) {
     __syncthreads();
-    if (stager) store();
+    store();
     __syncthreads();
-    if (stager && kc + 1 < nk) load((kc + 1) * BK);
-    // B fragments [j][plane], plane 0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo
-    bf16x8_t bf[2][4];
+    if (kc + 1 < nk) load((kc + 1) * BK);
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = wn * 32 + j * 16 + (lane & 15);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (BHI && (q & 1)) continue;
-        bf[j][q] = *reinterpret_cast<const bf16x8_t*>(&Bs[q][b3_lds(col, seg)]);
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < TI; ++i) {
-      const int row = wm * 16 * TI + i * 16 + (lane & 15);
-      bf16x8_t a[PA];
-#pragma unroll
-      for (int q = 0; q < PA; ++q) {
-        if (HIONLY && (q & 1)) continue;
-        a[q] = *reinterpret_cast<const bf16x8_t*>(&As[q][b3_lds(row, seg)]);
-      }
+    for (int u = 0; u < BK / 32; ++u) {
+      const int seg = (lane >> 4) + 4 * u;
+      // B fragments [j][plane], plane 0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo
+      bf16x8_t bf[2][4];
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        f4_t cr = acc[0][i][j], ci = acc[1][i][j];
-        if constexpr (REAL_A) {  // re += E.Br, im += E.Bi
-          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][0], cr, 0, 0, 0);
-          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][1], cr, 0, 0, 0);
-          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][0], cr, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][2], ci, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][3], ci, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][2], ci, 0, 0, 0);
-        } else {
-          const bf16x8_t nh = neg_bf16x8(bf[j][2]);
-          // re += Ar.Br - Ai.Bi
-          if constexpr (!HIONLY) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][0], cr, 0, 0, 0);
-          if constexpr (!BHI) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][1], cr, 0, 0, 0);
-          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][0], cr, 0, 0, 0);
-          if constexpr (!HIONLY) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], nh, cr, 0, 0, 0);
-          if constexpr (!BHI) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], neg_bf16x8(bf[j][3]), cr, 0, 0, 0);
-          cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], nh, cr, 0, 0, 0);
-          // im += Ar.Bi + Ai.Br
-          if constexpr (!HIONLY) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][2], ci, 0, 0, 0);
-          if constexpr (!BHI) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][3], ci, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][2], ci, 0, 0, 0);
-          if constexpr (!HIONLY) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], bf[j][0], ci, 0, 0, 0);
-          if constexpr (!BHI) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][1], ci, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][0], ci, 0, 0, 0);
+        const int col = wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (BHI && (q & 1)) continue;
+          bf[j][q] = *reinterpret_cast<const bf16x8_t*>(&Bs[bslot(q)][prop_lds<BK>(col, seg)]);
         }
-        acc[0][i][j] = cr;
-        acc[1][i][j] = ci;
+      }
+#pragma unroll
+      for (int i = 0; i < TI; ++i) {
+        const int row = wm * 16 * TI + i * 16 + (lane & 15);
+        bf16x8_t a[PA];
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+          if (HIONLY && (q & 1)) continue;
+          a[q] = *reinterpret_cast<const bf16x8_t*>(&As[aslot(q)][prop_lds<BK>(row, seg)]);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          f4_t cr = acc[0][i][j], ci = acc[1][i][j];
+          if constexpr (REAL_A) {  // re += E.Br, im += E.Bi
+            cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][0], cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][1], cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][0], cr, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][2], ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][3], ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][2], ci, 0, 0, 0);
+          } else {
+            const bf16x8_t nh = neg_bf16x8(bf[j][2]);
+            // re += Ar.Br - Ai.Bi
+            if constexpr (!HIONLY) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][0], cr, 0, 0, 0);
+            if constexpr (!BHI) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][1], cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][0], cr, 0, 0, 0);
+            if constexpr (!HIONLY) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], nh, cr, 0, 0, 0);
+            if constexpr (!BHI) cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], neg_bf16x8(bf[j][3]), cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], nh, cr, 0, 0, 0);
+            // im += Ar.Bi + Ai.Br
+            if constexpr (!HIONLY) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf[j][2], ci, 0, 0, 0);
+            if constexpr (!BHI) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][3], ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf[j][2], ci, 0, 0, 0);
+            if constexpr (!HIONLY) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], bf[j][0], ci, 0, 0, 0);
+            if constexpr (!BHI) ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][1], ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bf[j][0], ci, 0, 0, 0);
+          }
+          acc[0][i][j] = cr;
+          acc[1][i][j] = ci;
+        }
       }
     }
   }
@@ -334,7 +359,7 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
 }
 
 template <bool REAL_A, int VM = 0>
-__global__ __launch_bounds__((64 * PropWaves<REAL_A, VM>::value), 2) void prop_b3_kernel(PropB3P p) {
+__global__ __launch_bounds__((64 * PropWaves<REAL_A, VM>::value), (PropWaves<REAL_A, VM>::value / 2)) void prop_b3_kernel(PropB3P p) {
   prop_b3_body<REAL_A, VM, PropWaves<REAL_A, VM>::value>(p, blockIdx.x);
 }
 
@@ -351,7 +376,7 @@ __device__ __forceinline__ int b3_problem_of(const int* __restrict__ start, int 
 }
 
 template <bool REAL_A, int VM>
-__global__ __launch_bounds__((64 * PropWaves<REAL_A, VM>::value), 2) void prop_b3_grouped_kernel(
+__global__ __launch_bounds__((64 * PropWaves<REAL_A, VM>::value), (PropWaves<REAL_A, VM>::value / 2)) void prop_b3_grouped_kernel(
     const PropB3P* __restrict__ probs, const int* __restrict__ start, int nprob) {
   const int pi = b3_problem_of(start, nprob, blockIdx.x);
   const PropB3P p = probs[pi];

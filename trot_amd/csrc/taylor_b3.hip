@@ -384,10 +384,12 @@ __global__ __launch_bounds__((64 * PropWaves<REAL_A, VM>::value), (PropWaves<REA
 }
 
 // Transposed planes (W, 4, np, Np) of walkers w (W, N, n) complex128, zero padded.  Grid (Np/64, np/64, W),
-// 256 threads; a 64 x 64 tile goes through LDS so that reads run along p and writes along i.
+// 256 threads; a 64 x 64 tile goes through LDS (as float2: the split starts from the FP32 value, and 33 KB
+// leave room for four workgroups per CU) so that reads run along p, and each thread then converts 8
+// consecutive orbitals of one column and writes them as one 16-byte store per plane.
 __device__ __forceinline__ void split_transpose_w_body(const double* __restrict__ w, bf16raw* __restrict__ tp, int N,
                                                        int n, int Np, int np_, int bx, int by, long bz) {
-  __shared__ double2 tile[64][65];
+  __shared__ float2 tile[64][65];
   const int i0 = bx * 64, p0 = by * 64;
   const long b = bz;
   const int t = threadIdx.x;
@@ -395,22 +397,32 @@ __device__ __forceinline__ void split_transpose_w_body(const double* __restrict_
   for (int e = t; e < 64 * 64; e += 256) {
     const int ii = e >> 6, pp = e & 63;  // read along p
     const int gi = i0 + ii, gp = p0 + pp;
-    tile[ii][pp] = (gi < N && gp < n) ? wb[(long)gi * n + gp] : make_double2(0.0, 0.0);
+    const double2 v = (gi < N && gp < n) ? wb[(long)gi * n + gp] : make_double2(0.0, 0.0);
+    tile[ii][pp] = make_float2((float)v.x, (float)v.y);
   }
   __syncthreads();
   const long plane = (long)np_ * Np;
   bf16raw* base = tp + b * 4 * plane;
-  for (int e = t; e < 64 * 64; e += 256) {
-    const int pp = e >> 6, ii = e & 63;  // write along i
-    const double2 v = tile[ii][pp];
-    const long o = (long)(p0 + pp) * Np + i0 + ii;
-    bf16raw h, l;
-    split_bf16_raw((float)v.x, h, l);
-    base[o] = h;
-    base[plane + o] = l;
-    split_bf16_raw((float)v.y, h, l);
-    base[2 * plane + o] = h;
-    base[3 * plane + o] = l;
+  typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+  for (int e = t; e < 64 * 8; e += 256) {
+    const int pp = e >> 3, i8 = (e & 7) * 8;  // column pp, orbitals i8 .. i8 + 7 of the tile
+    us8 rh, rl, ih, il;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float2 v = tile[i8 + q][pp];
+      bf16raw h, l;
+      split_bf16_raw(v.x, h, l);
+      rh[q] = h;
+      rl[q] = l;
+      split_bf16_raw(v.y, h, l);
+      ih[q] = h;
+      il[q] = l;
+    }
+    const long o = (long)(p0 + pp) * Np + i0 + i8;
+    *reinterpret_cast<us8*>(base + o) = rh;
+    *reinterpret_cast<us8*>(base + plane + o) = rl;
+    *reinterpret_cast<us8*>(base + 2 * plane + o) = ih;
+    *reinterpret_cast<us8*>(base + 3 * plane + o) = il;
   }
 }
 

@@ -77,9 +77,23 @@ __global__ __launch_bounds__(512, 1) void exchange_b3_kernel(B3P p, int n_mt, in
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int w = mt * 2 + (wave >> 2), g = nt * 4 + (wave & 3);
-  // X = f_g^T (64 x 64 complex, FP32) of this wave, element (r, c) at r * 64 + (c ^ r);
-  // four waves at a time (4 x 32 KB = the main loop's LDS)
-  float2* X = reinterpret_cast<float2*>(smem) + (wave & 3) * 4096;
+  // X = f_g (64 x 64 complex, FP32, re and im planes) of this wave, element (r, c) at r * 64 + (c ^ r); four
+  // waves at a time
+  // (4 x 32 KB = the main loop's LDS).  Each lane holds 64 elements f_ij of its wave's f_g in registers
+  // (row 16 a + 4 (lane / 16) + r, column 16 b + lane % 16) and reads only their partners f_ji from X:
+  // K_g = sum_ij f_ij f_ji visits every ordered pair once; c_g = tr f_g from the diagonal ones.  Each product
+  // is formed in FP32 (one rounding, 2^-24, far below the split-BF16 error of f) and summed in FP64.
+  float* Xr = reinterpret_cast<float*>(smem) + (wave & 3) * 8192;  // re plane, im plane 4096 floats further
+  float* Xi = Xr + 4096;
+  // row 16 i + 4 q + r, column 16 j + c (q = lane / 16, c = lane % 16): (row, col) sits at
+  // 1024 i + 16 (i ^ j) + wr[r] and its partner (col, row) at 1024 j + 16 (i ^ j) + pr[r]
+  const int q = lane >> 4, c = lane & 15;
+  int wr[4], pr[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    wr[r] = 64 * (4 * q + r) + (c ^ (4 * q + r));
+    pr[r] = 64 * c + (c ^ (4 * q + r));
+  }
   for (int round = 0; round < 2; ++round) {
     const bool mine = (wave >> 2) == round;
     if (mine) {
@@ -89,19 +103,31 @@ __global__ __launch_bounds__(512, 1) void exchange_b3_kernel(B3P p, int n_mt, in
         for (int j = 0; j < 4; ++j)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int row = i * 16 + 4 * (lane >> 4) + r, col = j * 16 + (lane & 15);
-            X[row * 64 + (col ^ row)] = make_float2(acc[0][i][j][r], acc[1][i][j][r]);
+            const int o = 1024 * i + 16 * (i ^ j) + wr[r];
+            Xr[o] = acc[0][i][j][r];
+            Xi[o] = acc[1][i][j][r];
           }
     }
     __syncthreads();
     if (mine) {
-      const float2 d = X[lane * 64];
-      double cr = d.x, ci = d.y, kr = 0.0, ki = 0.0;
-      for (int it = 0; it < 64; ++it) {
-        const float2 a = X[it * 64 + (lane ^ it)];    // X[it][lane]
-        const float2 b = X[lane * 64 + (it ^ lane)];  // X[lane][it]
-        kr += (double)a.x * b.x - (double)a.y * b.y;
-        ki += (double)a.x * b.y + (double)a.y * b.x;
+      double cr = 0.0, ci = 0.0, kr = 0.0, ki = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int o = 1024 * j + 16 * (i ^ j) + pr[r];
+            const float xr = acc[0][i][j][r], xi = acc[1][i][j][r];
+            const float tr = Xr[o], ti = Xi[o];  // f_ji
+            kr += (double)(xr * tr - xi * ti);
+            ki += (double)(xr * ti + xi * tr);
+            if (i == j && c == 4 * q + r) {  // diagonal
+              cr += xr;
+              ci += xi;
+            }
+          }
+        __builtin_amdgcn_sched_barrier(0);  // one row tile's reads in flight at a time
       }
 #pragma unroll
       for (int m = 32; m > 0; m >>= 1) {

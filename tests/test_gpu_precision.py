@@ -106,6 +106,16 @@ def test_split_bf16_nt_gemm(gpu, shape, cx):
         out = torch.from_numpy(ref.copy()).cuda()
         _b3.gemm_nt(ap, bp, out, alpha=0.5j, beta=1.0, ksplit=ks)
         assert rel(out.cpu().numpy(), ref + 0.5j * (a @ b.T)) < TOL["bf16x3"]
+        # split complex64 output (hi and lo words of bf16 re | bf16 im << 16); N = 70, 129 take the
+        # element-by-element stores, 128 and 300 the 16-byte ones
+        words = _b3.gemm_nt_split(ap, bp).cpu().numpy().view(np.uint32)
+
+        def bf(u):
+            return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+        v = bf(words & 0xFFFF) + bf(words >> 16) * 1j  # each plane as complex
+        assert rel(v[0] + v[1], a @ b.T) < TOL["bf16x3"]
+        assert np.abs(v[1]).max() <= 2.0**-8 * np.abs(v[0]).max()
 
 
 B3_SHAPES = [(64, 16, 30, 7), (37, 5, 21, 3), (130, 64, 40, 4), (512, 64, 16, 2), (96, 70, 20, 9)]

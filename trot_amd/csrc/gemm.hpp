@@ -11,7 +11,7 @@
 // (BM/2) x (BN/2) block of 16 x 16 MFMA tiles.  K is staged through LDS in
 // chunks of 16 with one chunk of global loads in flight in registers while
 // the previous chunk is multiplied.  LDS tiles are stored k-major
-// (As[plane][k][i]) so each MFMA operand fetch is 16 consecutive doubles.
+// (As[plane][k][i], columns XOR-swizzled by k) so each MFMA operand fetch is 16 doubles of one row.
 //
 // MFMA f64 16x16x4 operand layout (lane l):  A[i = l % 16][k = l / 16],
 // B[k = l / 16][j = l % 16];  D[i = 4 r + l / 16][j = l % 16] in register r (measured on
@@ -58,6 +58,11 @@ struct TileF64 {
     double As[PA][BK][BM + PAD];
     double Bs[PB][BK][BN + PAD];
   };
+  // Column i of LDS row k is stored at i ^ ((k >> 1) & 7).  A staging store of K-contiguous global data
+  // puts the 16 k of one i in 16 lanes: with the 80-double pitch alone they fall on two bank offsets, with
+  // the XOR on 16 distinct ones.  A fragment read (16 consecutive i of one k per 16 lanes) is permuted
+  // inside aligned groups of 8 and stays conflict-free.
+  __device__ __forceinline__ static int sw(int k, int i) { return i ^ ((k >> 1) & 7); }
   typedef d4_t Acc[PC][TM][TN];
 
   // acc = op(A)[m0:m0+BM, :K] * B[:K, n0:n0+BN] (zero padded), A/B already offset to the batch.
@@ -124,7 +129,7 @@ struct TileF64 {
           if (AKC) { k = e % BK; i = e / BK; } else { i = e % BM; k = e / BM; }
         }
 #pragma unroll
-        for (int q = 0; q < PA; ++q) sm.As[q][k][i] = ra[r][q];
+        for (int q = 0; q < PA; ++q) sm.As[q][k][sw(k, i)] = ra[r][q];
       }
 #pragma unroll
       for (int r = 0; r < EB; ++r) {
@@ -132,7 +137,7 @@ struct TileF64 {
         int j, k;
         if (BNC) { j = e % BN; k = e / BN; } else { k = e % BK; j = e / BK; }
 #pragma unroll
-        for (int q = 0; q < PB; ++q) sm.Bs[q][k][j] = rb[r][q];
+        for (int q = 0; q < PB; ++q) sm.Bs[q][k][sw(k, j)] = rb[r][q];
       }
     };
 
@@ -157,11 +162,11 @@ struct TileF64 {
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int q = 0; q < PA; ++q) af[i][q] = sm.As[q][kr][wm * WM + i * 16 + (lane & 15)];
+          for (int q = 0; q < PA; ++q) af[i][q] = sm.As[q][kr][sw(kr, wm * WM + i * 16 + (lane & 15))];
 #pragma unroll
         for (int j = 0; j < TN; ++j)
 #pragma unroll
-          for (int q = 0; q < PB; ++q) bf[j][q] = sm.Bs[q][kr][wn * WN + j * 16 + (lane & 15)];
+          for (int q = 0; q < PB; ++q) bf[j][q] = sm.Bs[q][kr][sw(kr, wn * WN + j * 16 + (lane & 15))];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll

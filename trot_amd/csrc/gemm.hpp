@@ -58,11 +58,12 @@ struct TileF64 {
     double As[PA][BK][BM + PAD];
     double Bs[PB][BK][BN + PAD];
   };
-  // Column i of LDS row k is stored at i ^ ((k >> 1) & 7).  A staging store of K-contiguous global data
-  // puts the 16 k of one i in 16 lanes: with the 80-double pitch alone they fall on two bank offsets, with
-  // the XOR on 16 distinct ones.  A fragment read (16 consecutive i of one k per 16 lanes) is permuted
-  // inside aligned groups of 8 and stays conflict-free.
-  __device__ __forceinline__ static int sw(int k, int i) { return i ^ ((k >> 1) & 7); }
+  // Column i of LDS row k is stored at i ^ (2 ((k >> 1) & 7)).  A staging store of K-contiguous global data
+  // puts the 16 k of one i in 16 lanes: with the 80-double pitch alone rows k and k + 2 share banks (8-way
+  // conflicts); the XOR gives the 32 doubles of a 32-lane pass 32 distinct bank pairs.  A fragment read
+  // (16 consecutive i of one k per 16 lanes) is permuted inside aligned groups of 16 and stays
+  // conflict-free.
+  __device__ __forceinline__ static int sw(int k, int i) { return i ^ (((k >> 1) & 7) << 1); }
   typedef d4_t Acc[PC][TM][TN];
 
   // acc = op(A)[m0:m0+BM, :K] * B[:K, n0:n0+BN] (zero padded), A/B already offset to the batch.

@@ -63,6 +63,11 @@ struct TileLP {
     float As[PA][BK][BM + PAD];
     float Bs[PB][BK][BN + PAD];
   };
+  // MODE_F32: column i of LDS row k is stored at i ^ (4 ((k >> 2) & 3)).  A staging store of K-contiguous
+  // global data puts the 16 k of one i in 16 lanes, and with the 80-float pitch alone rows k, k + 4, k + 8,
+  // k + 12 share banks (4-way conflicts); with the XOR the 64 lanes of a store take 64 distinct banks.  A
+  // fragment read (16 consecutive i of one k per 16 lanes) is permuted inside aligned groups of 16.
+  __device__ __forceinline__ static int sw(int k, int i) { return i ^ (((k >> 2) & 3) << 2); }
   struct SmemBF16 {
     __bf16 Ah[PA][BM][BK + KP];
     __bf16 Al[PA][BM][BK + KP];
@@ -137,7 +142,7 @@ struct TileLP {
 #pragma unroll
         for (int q = 0; q < PA; ++q) {
           if constexpr (MODE == MODE_F32) {
-            sm.As[q][k][i] = ra[r][q];
+            sm.As[q][k][sw(k, i)] = ra[r][q];
           } else {
             __bf16 h, l;
             split_bf16(ra[r][q], h, l);
@@ -154,7 +159,7 @@ struct TileLP {
 #pragma unroll
         for (int q = 0; q < PB; ++q) {
           if constexpr (MODE == MODE_F32) {
-            sm.Bs[q][k][j] = rb[r][q];
+            sm.Bs[q][k][sw(k, j)] = rb[r][q];
           } else {
             __bf16 h, l;
             split_bf16(rb[r][q], h, l);
@@ -187,11 +192,11 @@ struct TileLP {
 #pragma unroll
           for (int i = 0; i < TM; ++i)
 #pragma unroll
-            for (int q = 0; q < PA; ++q) af[i][q] = sm.As[q][kr][wm * WM + i * 16 + (lane & 15)];
+            for (int q = 0; q < PA; ++q) af[i][q] = sm.As[q][kr][sw(kr, wm * WM + i * 16 + (lane & 15))];
 #pragma unroll
           for (int j = 0; j < TN; ++j)
 #pragma unroll
-            for (int q = 0; q < PB; ++q) bf[j][q] = sm.Bs[q][kr][wn * WN + j * 16 + (lane & 15)];
+            for (int q = 0; q < PB; ++q) bf[j][q] = sm.Bs[q][kr][sw(kr, wn * WN + j * 16 + (lane & 15))];
 #pragma unroll
           for (int i = 0; i < TM; ++i)
 #pragma unroll

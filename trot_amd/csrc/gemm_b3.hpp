@@ -43,11 +43,10 @@ struct B3P {
 };
 
 // XOR swizzle of the 16-byte segments of 64-byte LDS rows.  A fragment read takes one segment of
-// 16 consecutive rows (two segments per 32 lanes); a staging store takes the 4 segments of 4
-// consecutive rows.  With swz(r) = (r >> 2) & 3 both patterns land on distinct 4-bank groups for
-// 64 banks served 16 lanes at a time and for 128 banks served 32 lanes at a time: rows r and r + 8
-// share banks in the second case and get segments that differ in bit 1, while the two segments of
-// one 32-lane read differ in bit 0.
+// 16 consecutive rows; a staging store takes the 4 segments of 4 consecutive rows (256 contiguous
+// bytes, every bank once).  Rows r, r + 4, r + 8, r + 12 share the same 16 banks (64 banks of 4
+// bytes); swz gives them slots 0, 2, 1, 3 in some order, so the 16 lanes of a fragment read that
+// the LDS serves together touch 16 distinct 4-bank groups.
 __device__ __forceinline__ int b3_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
 // element offset of (row r, segment seg) in a tile of 64-byte rows
 __device__ __forceinline__ int b3_lds(int r, int seg) { return r * 32 + 8 * (seg ^ b3_swz(r)); }

@@ -300,62 +300,89 @@ __device__ __forceinline__ void prop_b3_body(const PropB3P& p, const int b) {
   }
 
   // epilogue.  Lane holds rows 4 (l / 16) + r, column l % 16 of each 16 x 16 tile.
-  bf16raw* Tn = p.tn ? p.tn + walker * 4 * tplane : nullptr;
+  // The loads of a group of EI row tiles (old running sums, FP64 walkers) are issued together, before any
+  // of the group's stores: the counter a wave waits on for a load also counts its earlier stores, so loads
+  // placed between stores (as a per-element read-modify-write compiles) would each wait for every store
+  // before them.  EI keeps the loaded values within the registers the main loop leaves free.
+  // Each array is addressed from one per-lane base (row gib, column gpb: r = 0 of row tile 0, column tile 0)
+  // plus offsets that are the same for the whole wave.
   const bool to_w = REAL_A || p.last;
+  const int gib = m0 + wm * 16 * TI + 4 * (lane >> 4), gpb = n0 + wn * 32 + (lane & 15);
+  const long sld = p.np_, wld = p.n;
+  float2* Sb = REAL_A ? nullptr : reinterpret_cast<float2*>(p.S) + (walker * p.Np + gib) * sld + gpb;
+  const double2* Wi = to_w ? reinterpret_cast<const double2*>(p.win) + (walker * p.N + gib) * wld + gpb : nullptr;
+  double2* Wo = to_w ? reinterpret_cast<double2*>(p.w) + (walker * p.N + gib) * wld + gpb : nullptr;
+  bf16raw* Tb = p.tn ? p.tn + walker * 4 * tplane + (long)gpb * p.Np + gib : nullptr;
+  constexpr int EI = REAL_A ? TI : (NW == 8 ? 1 : 2);
+  static_assert(TI % EI == 0, "row tiles per epilogue group");
 #pragma unroll
-  for (int i = 0; i < TI; ++i)
+  for (int i0 = 0; i0 < TI; i0 += EI) {
+    float2 so[EI][2][4];
+    double2 wo[EI][2][4];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int gi0 = m0 + wm * 16 * TI + i * 16 + 4 * (lane >> 4);
-      const int gp = n0 + wn * 32 + j * 16 + (lane & 15);
-      bf16raw hr[4], lr[4], hi[4], li[4];
+    for (int ii = 0; ii < EI; ++ii)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gi = gi0 + r;
-        float tre, tim;
-        if constexpr (REAL_A) {
-          tre = acc[0][i][j][r];
-          tim = acc[1][i][j][r];
-        } else {  // T = i c acc
-          tre = -p.c * acc[1][i][j][r];
-          tim = p.c * acc[0][i][j][r];
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int di = 16 * (i0 + ii) + r;
+          so[ii][j][r] = make_float2(0.f, 0.f);
+          if (!REAL_A && !p.first) so[ii][j][r] = Sb[di * sld + 16 * j];
+          wo[ii][j][r] = make_double2(0.0, 0.0);
+          if (to_w && gib + di < p.N && gpb + 16 * j < p.n) wo[ii][j][r] = Wi[di * wld + 16 * j];
         }
-        float pr = tre, pi = tim;  // value whose planes are written
-        if (!REAL_A) {
-          float2* sp = reinterpret_cast<float2*>(p.S) + (walker * p.Np + gi) * p.np_ + gp;
-          const float2 so = p.first ? make_float2(0.f, 0.f) : *sp;
-          if (!p.last) {
-            *sp = make_float2(so.x + tre, so.y + tim);
-          } else {
-            tre += so.x;
-            tim += so.y;
+#pragma unroll
+    for (int ii = 0; ii < EI; ++ii)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int i = i0 + ii;
+        bf16raw hr[4], lr[4], hi[4], li[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int di = 16 * i + r;
+          float tre, tim;
+          if constexpr (REAL_A) {
+            tre = acc[0][i][j][r];
+            tim = acc[1][i][j][r];
+          } else {  // T = i c acc
+            tre = -p.c * acc[1][i][j][r];
+            tim = p.c * acc[0][i][j][r];
+          }
+          float pr = tre, pi = tim;  // value whose planes are written
+          if (!REAL_A) {
+            const float2 o = so[ii][j][r];
+            if (!p.last) {
+              Sb[di * sld + 16 * j] = make_float2(o.x + tre, o.y + tim);
+            } else {
+              tre += o.x;
+              tim += o.y;
+            }
+          }
+          if (to_w) {
+            pr = pi = 0.f;
+            if (gib + di < p.N && gpb + 16 * j < p.n) {
+              const double2 o = wo[ii][j][r];
+              const double2 v = make_double2(o.x + (double)tre, o.y + (double)tim);
+              Wo[di * wld + 16 * j] = v;
+              pr = (float)v.x;
+              pi = (float)v.y;
+            }
+          }
+          split_bf16_raw(pr, hr[r], lr[r]);
+          split_bf16_raw(pi, hi[r], li[r]);
+        }
+        if (Tb) {  // 4 consecutive rows of row gp of the transposed planes: one 8-byte store per plane
+          bf16raw* o = Tb + (long)(16 * j) * p.Np + 16 * i;
+          typedef unsigned short us4 __attribute__((ext_vector_type(4)));
+          *reinterpret_cast<us4*>(o + 0 * tplane) = (us4){hr[0], hr[1], hr[2], hr[3]};
+          *reinterpret_cast<us4*>(o + 2 * tplane) = (us4){hi[0], hi[1], hi[2], hi[3]};
+          if (!p.tn_hi) {
+            *reinterpret_cast<us4*>(o + 1 * tplane) = (us4){lr[0], lr[1], lr[2], lr[3]};
+            *reinterpret_cast<us4*>(o + 3 * tplane) = (us4){li[0], li[1], li[2], li[3]};
           }
         }
-        if (to_w) {
-          pr = pi = 0.f;
-          if (gi < p.N && gp < p.n) {
-            const long e = (walker * p.N + gi) * p.n + gp;
-            const double2 o = reinterpret_cast<const double2*>(p.win)[e];
-            const double2 v = make_double2(o.x + (double)tre, o.y + (double)tim);
-            reinterpret_cast<double2*>(p.w)[e] = v;
-            pr = (float)v.x;
-            pi = (float)v.y;
-          }
-        }
-        split_bf16_raw(pr, hr[r], lr[r]);
-        split_bf16_raw(pi, hi[r], li[r]);
       }
-      if (Tn) {  // 4 consecutive rows of row gp of the transposed planes: one 8-byte store per plane
-        const long o = (long)gp * p.Np + gi0;
-        typedef unsigned short us4 __attribute__((ext_vector_type(4)));
-        *reinterpret_cast<us4*>(Tn + 0 * tplane + o) = (us4){hr[0], hr[1], hr[2], hr[3]};
-        *reinterpret_cast<us4*>(Tn + 2 * tplane + o) = (us4){hi[0], hi[1], hi[2], hi[3]};
-        if (!p.tn_hi) {
-          *reinterpret_cast<us4*>(Tn + 1 * tplane + o) = (us4){lr[0], lr[1], lr[2], lr[3]};
-          *reinterpret_cast<us4*>(Tn + 3 * tplane + o) = (us4){li[0], li[1], li[2], li[3]};
-        }
-      }
-    }
+  }
 }
 
 template <bool REAL_A, int VM = 0>

@@ -349,6 +349,20 @@ __device__ __forceinline__ void b3w_mainloop(const B3P& p, int m0, int n0, int b
           }
           acc[q][i][j] = c;
         }
+      // Issue order of row tile i: one of the next row tile's four fragment reads after each group of 4
+      // MFMAs, then the last 8 MFMAs.  Left to itself the compiler clusters the reads and waits for them a
+      // few MFMAs later (its LDS waits here drain every outstanding read); spread out, each read has 4-16
+      // MFMAs of cover before the wait at the next row tile.
+      if (i < 3) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);  // MFMA
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+      } else {
+        __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
+      }
     }
     __builtin_amdgcn_s_setprio(0);
   }

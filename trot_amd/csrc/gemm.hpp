@@ -252,12 +252,21 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmP& p, int b, int nbatch,
       }
 }
 
+// Tile order: the (batch, row tile, column tile) triples in that order, column tile fastest, are dealt to the
+// 8 XCDs in 8 contiguous ranges (workgroup w runs on XCD w % 8), so the tiles of one batch run on one XCD
+// one after another and share the operand they have in common (a walker's right-hand matrix, a slice's
+// panels) in that XCD's L2 instead of each XCD fetching it.
 template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false,
           bool CONJB = false>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
+__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p, int ntn, int ntm, int nbatch) {
   __shared__ typename TileF64<BM, BN, AC, BC, AKC, BNC, CONJA, ASYM, CONJB>::Smem sm;
-  gemm_f64_tile<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM, CONJB>(p, blockIdx.z, gridDim.z, blockIdx.y * BM,
-                                                                   blockIdx.x * BN, sm);
+  const long per_batch = (long)ntn * ntm, total = per_batch * nbatch;
+  const long per = (total + 7) / 8;
+  const long q = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (q >= total) return;
+  const int b = (int)(q / per_batch);
+  const int t = (int)(q - (long)b * per_batch);
+  gemm_f64_tile<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM, CONJB>(p, b, nbatch, (t / ntn) * BM, (t % ntn) * BN, sm);
 }
 
 // Many independent problems in one launch: problem i owns workgroups start[i] .. start[i+1] - 1, laid out
@@ -299,8 +308,11 @@ template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int 
           bool CONJB = false>
 static hipError_t launch_gemm_f64(const GemmP& p, int batch, hipStream_t s) {
   if (p.M <= 0 || p.N <= 0 || batch <= 0) return hipSuccess;
-  dim3 grid((p.N + BN - 1) / BN, (p.M + BM - 1) / BM, batch);
-  if (grid.y > 65535 || grid.z > 65535) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM, CONJB>), grid, dim3(256), 0, s, p);
+  const int ntn = (p.N + BN - 1) / BN, ntm = (p.M + BM - 1) / BM;
+  const long total = (long)ntn * ntm * batch;
+  const long nwg = (total + 7) / 8 * 8;
+  if (nwg > 0x7fffffffL) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AC, BC, AKC, BNC, CONJA, EPI, ASYM, CONJB>), dim3((unsigned)nwg),
+                     dim3(256), 0, s, p, ntn, ntm, batch);
   return hipGetLastError();
 }

@@ -270,8 +270,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p, int ntn, int ntm
 }
 
 // Many independent problems in one launch: problem i owns workgroups start[i] .. start[i+1] - 1, laid out
-// (batch, row tile, column tile) with the column tile fastest, as gemm_f64_kernel's grid.  Each workgroup
-// computes exactly what it would in a launch of its own problem.
+// (batch, row tile, column tile) with the column tile fastest, in that order (gemm_f64_kernel deals the same
+// triples to the XCDs in ranges instead).  Each workgroup computes exactly what it would in a launch of its
+// own problem.
 template <int BM, int BN, bool AC, bool BC, bool AKC, bool BNC, bool CONJA, int EPI, bool ASYM = false>
 __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const GemmP* __restrict__ probs,
                                                                const int* __restrict__ start, int nprob) {

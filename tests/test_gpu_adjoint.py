@@ -66,6 +66,19 @@ def test_gemm_batched_all_ops(gpu, ac, bc):
     assert np.array_equal(after[:, :, :N], before[:, :, :N]) and np.array_equal(after[:, :, 2 * N:], before[:, :, 2 * N:])
 
 
+def test_gemm_batched_many_small_products(gpu):
+    """70,000 batch entries: more than a 3D grid's 65,535, and not a multiple of the 8 XCDs the tiles are
+    dealt to (the last group of eight workgroups is partly empty)."""
+    from trot_amd import _lib
+
+    rng = np.random.default_rng(9)
+    B, M, N, K = 70000, 5, 3, 4
+    a = _rand(rng, (B, M, K), True)
+    b = _rand(rng, (B, K, N), False)
+    got = _lib.gemm_batched(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), alpha=2.0).cpu().numpy()
+    assert rel(got, 2.0 * (a @ b)) < 1e-13
+
+
 def test_gemm_batched_refusals_and_inverse(gpu):
     from trot_amd import _lib
 

@@ -65,7 +65,9 @@ int trot_gemm_batched(int opa, int opb, int a_complex, int b_complex, int M, int
   const bool a_kc = opa == TROT_OP_N || opa == TROT_OP_R;  // A stored M x K
   const bool b_nc = opb == TROT_OP_N || opb == TROT_OP_R;  // B stored K x N
   if (lda < (a_kc ? K : M) || ldb < (b_nc ? N : K) || ldc < N) return fail(TROT_ERR_INVALID, "gemm_batched: leading dimension");
-  if (batch > 65535 || (M + 63) / 64 > 65535) return fail(TROT_ERR_UNSUPPORTED, "gemm_batched: grid too large");
+  // the FP64 tile GEMM runs one workgroup per (batch, 64-row tile, 32- or 64-column tile) on a 1D grid
+  if ((long)((M + 63) / 64) * ((N + 31) / 32) * batch > 0x7ffffff0L)
+    return fail(TROT_ERR_UNSUPPORTED, "gemm_batched: too many tiles");
   if (!(a_complex || b_complex) && alpha_im != 0.0)
     return fail(TROT_ERR_INVALID, "gemm_batched: complex alpha with a real product");
   if (M == 0 || N == 0 || batch == 0) return TROT_OK;

@@ -345,7 +345,6 @@ int trot_get_taylor_tile(int W, int N, int n, int symblk) { return taylor_tile(s
 int trot_autotune_taylor(int W, int N, int n, int symblk, void* ws, size_t ws_bytes, void* stream, int* shape_out,
                          float* ms_out) {
   if (W < 1 || N < 1 || n < 1) return fail(TROT_ERR_INVALID, "autotune_taylor: bad sizes");
-  if (W > 65535) return fail(TROT_ERR_INVALID, "autotune_taylor: at most 65535 walkers");
   if (ws_bytes < trot_autotune_taylor_workspace(W, N, n, symblk))
     return fail(TROT_ERR_WORKSPACE, "autotune_taylor: workspace");
   hipStream_t s = (hipStream_t)stream;
